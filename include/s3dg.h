@@ -1,0 +1,314 @@
+/*
+ * s3dg.h -- C ABI of the MI355X rendering hot path (BVH traversal, ray/triangle
+ * intersection, wavefront path tracing, tonemapping, NLM denoising).
+ *
+ * Every entry point returns an int status (S3DG_OK == 0).  Plain pointers and
+ * sizes only.  Device memory is owned by the context; calls on one context are
+ * serialised on the context's single HIP stream.  Functions with a `_device`
+ * suffix take device pointers that already live in HBM; the others take host
+ * pointers and include the PCIe copies.
+ *
+ * The reference (ProjectAsura/Salty) has no C ABI: its hot path sits behind C++
+ * abstract interfaces.  Each declaration below names the reference interface it
+ * replaces (paths relative to the reference root).
+ */
+#ifndef S3DG_H
+#define S3DG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Status codes
+ * ------------------------------------------------------------------------- */
+#define S3DG_OK                 0
+#define S3DG_ERR_INVALID       -1   /* bad argument (null pointer, size, enum) */
+#define S3DG_ERR_NOMEM         -2   /* host or device allocation failed        */
+#define S3DG_ERR_DEVICE        -3   /* a HIP call failed                       */
+#define S3DG_ERR_NODEVICE      -4   /* no usable gfx950 device                 */
+#define S3DG_ERR_UNSUPPORTED   -5   /* request outside the supported envelope  */
+
+/* ---------------------------------------------------------------------------
+ * Scene description (plain data; copied by s3dg_scene_create)
+ * ------------------------------------------------------------------------- */
+
+/* s3d::Vertex (src/include/s3d_shape.h:27-32): 32 bytes. */
+typedef struct s3dg_vertex {
+    float position[3];
+    float normal[3];
+    float texcoord[2];
+} s3dg_vertex;
+
+/* Material kinds: s3d::Lambert, Mirror, Glass, Phong, Plastic
+ * (src/src/s3d_{lambert,mirror,glass,phong,plastic}.cpp). */
+enum {
+    S3DG_MAT_LAMBERT = 0,
+    S3DG_MAT_MIRROR  = 1,
+    S3DG_MAT_GLASS   = 2,
+    S3DG_MAT_PHONG   = 3,
+    S3DG_MAT_PLASTIC = 4
+};
+
+/* Union of the MaterialFactory::Create* parameters
+ * (src/include/s3d_materialfactory.h:22-49). */
+typedef struct s3dg_material {
+    int32_t type;           /* S3DG_MAT_*                              */
+    float   diffuse[3];     /* Lambert / Plastic diffuse colour         */
+    float   specular[3];    /* Mirror / Glass / Phong / Plastic colour  */
+    float   emissive[3];    /* IMaterial::GetEmissive                   */
+    float   ior;            /* Glass index of refraction                */
+    float   power;          /* Phong / Plastic exponent                 */
+} s3dg_material;
+
+/* Acceleration structure kind built for a mesh: BVH2::Create, BVH4::Create
+ * or BVH8::Create (src/include/s3d_bvh{2,4,8}.h).  Mesh::Init always uses
+ * BVH8 (src/src/s3d_mesh.cpp:495). */
+enum { S3DG_BVH2 = 2, S3DG_BVH4 = 4, S3DG_BVH8 = 8 };
+
+/* Mesh::Create(u32 vertexCount, Vertex*, IMaterial*) (src/include/s3d_mesh.h:42-47):
+ * vertex_count is a multiple of 3, one Triangle per 3 vertices.  A per-triangle
+ * material table (as the SMD loader produces) may replace the single material. */
+typedef struct s3dg_mesh_desc {
+    uint32_t           vertex_count;
+    const s3dg_vertex *vertices;
+    int32_t            material;           /* used when triangle_material == NULL */
+    const int32_t     *triangle_material;  /* optional, vertex_count/3 entries    */
+    int32_t            bvh_kind;           /* S3DG_BVH8 (default), BVH4 or BVH2   */
+} s3dg_mesh_desc;
+
+/* Top-level shapes, intersected in list order like Scene::Intersect
+ * (src/include/s3d_scene.h:63-75). */
+enum {
+    S3DG_SHAPE_MESH     = 0,   /* Mesh in world space                          */
+    S3DG_SHAPE_INSTANCE = 1,   /* Instance::Create(mesh, world) (s3d_instance.cpp:22-32) */
+    S3DG_SHAPE_SPHERE   = 2    /* Sphere::Create(radius, center, material)     */
+};
+
+typedef struct s3dg_shape_desc {
+    int32_t type;          /* S3DG_SHAPE_*                                   */
+    int32_t mesh;          /* mesh index for MESH / INSTANCE                 */
+    int32_t material;      /* SPHERE: material; INSTANCE: material override  */
+                           /* when >= 0 (an extension: the reference's       */
+                           /* Instance keeps the mesh's materials), else -1  */
+    float   center[3];     /* sphere centre                                  */
+    float   radius;        /* sphere radius                                  */
+    float   world[16];     /* instance world matrix, row-vector convention,  */
+                           /* translation in world[12..14] (Matrix _41.._43) */
+} s3dg_shape_desc;
+
+/* Light list entries (Scene::m_pLightList, s3d_scene.h:84-90).  SHAPE lights
+ * name a top-level sphere and use Sphere::Sample.  QUAD lights are a
+ * parallelogram made of triangles of a MESH shape; sampling it uniformly is an
+ * extension (the reference only samples spheres). */
+enum { S3DG_LIGHT_SHAPE = 0, S3DG_LIGHT_QUAD = 1 };
+
+typedef struct s3dg_light_desc {
+    int32_t type;            /* S3DG_LIGHT_*                                */
+    int32_t shape;           /* top-level shape index                       */
+    int32_t first_triangle;  /* QUAD: first triangle of the light in mesh   */
+    int32_t triangle_count;  /* QUAD: number of triangles (2 for a quad)    */
+    float   corner[3];       /* QUAD: p = corner + u*edge1 + v*edge2        */
+    float   edge1[3];
+    float   edge2[3];
+} s3dg_light_desc;
+
+enum { S3DG_CAMERA_PINHOLE = 0, S3DG_CAMERA_THINLENS = 1 };
+
+/* PinholeCamera::Update / ThinLensCamera::Update (src/include/s3d_camera.h:63-90,171-202).
+ * fov is used linearly by the reference: the vertical field of view is
+ * 2*atan(fov/2). */
+typedef struct s3dg_camera_desc {
+    int32_t type;
+    float   position[3];
+    float   target[3];
+    float   upward[3];
+    float   fov;
+    float   aspect;
+    float   near_clip;
+    float   lens_radius;   /* THINLENS; focus distance is |target-position| */
+    float   reserved;
+} s3dg_camera_desc;
+
+typedef struct s3dg_scene_desc {
+    uint32_t               mesh_count;
+    const s3dg_mesh_desc  *meshes;
+    uint32_t               shape_count;
+    const s3dg_shape_desc *shapes;
+    uint32_t               material_count;
+    const s3dg_material   *materials;
+    uint32_t               light_count;
+    const s3dg_light_desc *lights;
+    s3dg_camera_desc       camera;
+    float                  environment[3];  /* radiance on a miss; replaces IBL*10 */
+} s3dg_scene_desc;
+
+/* ---------------------------------------------------------------------------
+ * Rays and hits
+ * ------------------------------------------------------------------------- */
+
+/* 32 bytes.  The reference hard-codes t in [F_HIT_MIN=1e-3, F_HIT_MAX=1e12]
+ * (src/include/s3d_math.h:44-45); per-ray tmin/tmax generalise that. */
+typedef struct s3dg_ray {
+    float org[3];
+    float tmin;
+    float dir[3];
+    float tmax;
+} s3dg_ray;
+
+/* 16 bytes: HitRecord (src/include/s3d_shape.h:38-54) with the triangle pointer
+ * replaced by the triangle's index in its mesh's input order (-1 = miss, -2 =
+ * sphere).  u, v are the barycentrics (beta, gamma) of Triangle::IsHit. */
+typedef struct s3dg_hit {
+    int32_t prim;
+    float   t;
+    float   u;
+    float   v;
+} s3dg_hit;
+
+enum { S3DG_TRACE_CLOSEST = 0, S3DG_TRACE_ANY = 1 };
+
+/* ---------------------------------------------------------------------------
+ * Render configuration: superset of PathTracer::Config (src/include/s3d_pt.h:33-40)
+ * ------------------------------------------------------------------------- */
+typedef struct s3dg_render_config {
+    int32_t  width;
+    int32_t  height;
+    int32_t  max_bounce_count;
+    int32_t  spp;              /* passes of one sample per pixel               */
+    uint64_t seed;             /* counter-based RNG key                        */
+    int32_t  rr_start_depth;   /* dice honoured from this depth on (0 = reference) */
+    int32_t  first_pass;       /* pass index of the first sample (progressive) */
+} s3dg_render_config;
+
+typedef struct s3dg_render_stats {
+    uint64_t extension_rays;   /* closest-hit rays traced                      */
+    uint64_t shadow_rays;      /* NEE visibility rays traced                   */
+    uint64_t paths;            /* camera samples completed                     */
+    double   milliseconds;     /* device time of the render                    */
+} s3dg_render_stats;
+
+typedef struct s3dg_scene_stats {
+    uint64_t triangles;        /* unique triangles stored                      */
+    uint64_t blas_nodes;       /* compressed 8-wide nodes of all meshes        */
+    uint64_t tlas_nodes;
+    uint64_t device_bytes;     /* node + triangle + shading data in HBM        */
+    int32_t  max_depth;        /* deepest node path of the two-level tree      */
+    double   build_ms;         /* host SAH build + re-layout time              */
+} s3dg_scene_stats;
+
+/* Tonemap operators (src/include/s3d_tonemapper.h:20-33). */
+enum {
+    S3DG_TONEMAP_REINHARD   = 0,
+    S3DG_TONEMAP_UNCHARTED2 = 1,
+    S3DG_TONEMAP_ACES       = 2
+};
+
+typedef struct s3dg_context s3dg_context;
+typedef struct s3dg_scene   s3dg_scene;
+
+/* ---------------------------------------------------------------------------
+ * Context
+ * ------------------------------------------------------------------------- */
+/* No reference counterpart (the reference has no device); owns the HIP stream
+ * and device allocations.  device < 0 selects the current device. */
+int         s3dg_create(s3dg_context **out, int device);
+int         s3dg_destroy(s3dg_context *ctx);
+const char *s3dg_last_error(const s3dg_context *ctx);
+int         s3dg_synchronize(s3dg_context *ctx);
+int         s3dg_malloc(s3dg_context *ctx, size_t bytes, void **out_device_ptr);
+int         s3dg_free(s3dg_context *ctx, void *device_ptr);
+int         s3dg_memcpy_htod(s3dg_context *ctx, void *dst_device, const void *src_host, size_t bytes);
+int         s3dg_memcpy_dtoh(s3dg_context *ctx, void *dst_host, const void *src_device, size_t bytes);
+/* HIP stream of the context (hipStream_t), for callers that time on it. */
+void       *s3dg_stream(s3dg_context *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Scene: replaces Mesh::Create -> BVH8::Create (src/src/s3d_mesh.cpp:470-498,
+ * src/src/s3d_bvh8.cpp:401-479), Instance::Create, Sphere::Create and the
+ * Scene container (src/include/s3d_scene.h:22-109).  Builds every mesh's tree
+ * with the reference binned-SAH algorithm on the host, re-lays it out into the
+ * compressed 128-byte 8-wide node format, adds a top-level tree over the shapes
+ * and uploads everything.
+ * ------------------------------------------------------------------------- */
+int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *desc, s3dg_scene **out);
+int s3dg_scene_destroy(s3dg_scene *scene);
+int s3dg_scene_get_stats(const s3dg_scene *scene, s3dg_scene_stats *out);
+
+/* Permutation the reference builder applied to a mesh's triangles: order[i] is
+ * the input index of the triangle at depth-first position i (BVH8::Create
+ * partitions its IShape** argument in place, s3d_bvh8.cpp:321-334).  `order`
+ * holds vertex_count/3 entries. */
+int s3dg_scene_mesh_order(const s3dg_scene *scene, uint32_t mesh, int32_t *order);
+
+/* Host-only: the depth-first permutation that BVH2/4/8::Create would apply to
+ * the triangles of Mesh::Create(vertex_count, vertices, ...) (src/src/s3d_mesh.cpp:470-498),
+ * and the number of tree nodes, without a device.  kind is S3DG_BVH2/4/8. */
+int s3dg_build_order(uint32_t vertex_count, const s3dg_vertex *vertices, int32_t kind, int32_t *order,
+                     uint32_t *node_count);
+
+/* ---------------------------------------------------------------------------
+ * Traversal: replaces Scene::Intersect / IShape::IsHit(const RaySet&, HitRecord&)
+ * (src/include/s3d_scene.h:63-75, src/include/s3d_shape.h:60-68).
+ * CLOSEST returns the hit Scene::Intersect records; ANY reports whether any
+ * hit exists in [tmin, tmax] (prim >= 0 or -2 if so, -1 if not).
+ * shapes (optional) receives the top-level shape index of each hit.
+ * ------------------------------------------------------------------------- */
+int s3dg_trace(s3dg_scene *scene, const s3dg_ray *rays, uint32_t count, int32_t mode,
+               s3dg_hit *hits, int32_t *shapes);
+int s3dg_trace_device(s3dg_scene *scene, const s3dg_ray *d_rays, uint32_t count, int32_t mode,
+                      s3dg_hit *d_hits, int32_t *d_shapes);
+/* s3dg_trace_device that also adds the number of compressed nodes visited and
+ * triangles tested to d_counters[0] and d_counters[1] (uint32, device), the
+ * inputs of the traversal kernel's algorithmic-bytes figure.  No reference
+ * counterpart (instrumentation). */
+int s3dg_trace_counted_device(s3dg_scene *scene, const s3dg_ray *d_rays, uint32_t count, int32_t mode,
+                              s3dg_hit *d_hits, uint32_t *d_counters);
+
+/* ---------------------------------------------------------------------------
+ * Rendering: replaces PathTracer::TracePath / Radiance / NextEventEstimation
+ * (src/src/s3d_pt.cpp:182-353).  Adds cfg->spp passes of one pixel-centre
+ * sample to the accumulation buffer (a SUM, like m_RenderTarget), width*height
+ * RGB floats.  The buffer is read-modify-written, so progressive calls with
+ * increasing first_pass continue a render.
+ * ------------------------------------------------------------------------- */
+int s3dg_render(s3dg_scene *scene, const s3dg_render_config *cfg, float *accum_rgb,
+                s3dg_render_stats *stats);
+int s3dg_render_device(s3dg_scene *scene, const s3dg_render_config *cfg, float *d_accum_rgb,
+                       s3dg_render_stats *stats);
+
+/* Guide buffers for denoisers (no reference counterpart): first-hit albedo
+ * (GetBaseColor), shading normal and distance per pixel, each W*H*3 / W*H floats. */
+int s3dg_render_guides_device(s3dg_scene *scene, const s3dg_render_config *cfg,
+                              float *d_albedo, float *d_normal, float *d_depth);
+
+/* ---------------------------------------------------------------------------
+ * Post: replaces ToneMapper::Map (src/src/s3d_tonemapper.cpp:206-232) and the
+ * sRGB/8-bit conversion of PathTracer::Capture (src/src/s3d_pt.cpp:137-171).
+ * out_rgb (optional) receives the tonemapped float image, out_rgb8 (optional)
+ * the clamped, sRGB-encoded 8-bit image.
+ * ------------------------------------------------------------------------- */
+int s3dg_tonemap(s3dg_context *ctx, int32_t type, int32_t width, int32_t height,
+                 const float *in_rgb, float *out_rgb, uint8_t *out_rgb8);
+int s3dg_tonemap_device(s3dg_context *ctx, int32_t type, int32_t width, int32_t height,
+                        const float *d_in_rgb, float *d_out_rgb, uint8_t *d_out_rgb8);
+
+/* Replaces FilterNLM(int w, int h, float coeff, const Color4*, Color4*)
+ * (src/include/s3d_denoiser.h:16, src/src/s3d_denoiser.cpp:19-95).  The
+ * reference fixes search radius 6 and patch radius 2; both are parameters
+ * here.  src and dst are RGBA floats and must not alias. */
+int s3dg_nlm(s3dg_context *ctx, int32_t width, int32_t height, float coeff,
+             int32_t search_radius, int32_t patch_radius,
+             const float *src_rgba, float *dst_rgba);
+int s3dg_nlm_device(s3dg_context *ctx, int32_t width, int32_t height, float coeff,
+                    int32_t search_radius, int32_t patch_radius,
+                    const float *d_src_rgba, float *d_dst_rgba);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* S3DG_H */

@@ -1,0 +1,242 @@
+"""salty_amd: MI355X (gfx950) rendering hot path with a C ABI (include/s3dg.h).
+
+This module loads the in-tree ``libs3dg.so`` through ctypes and exposes thin
+Python handles over it.  There is no CPU fallback: if the library or a gfx950
+device is missing, the constructors raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (HIT_DTYPE, RAY_DTYPE, VERTEX_DTYPE, RenderConfig, RenderStats, SceneDesc,  # noqa: F401
+                  SceneStats)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libs3dg.so")
+
+_lib = None
+
+# Exported C entry points and their argument types.
+_SIGNATURES = {
+    "s3dg_create": [C.POINTER(C.c_void_p), C.c_int],
+    "s3dg_destroy": [C.c_void_p],
+    "s3dg_last_error": [C.c_void_p],
+    "s3dg_synchronize": [C.c_void_p],
+    "s3dg_malloc": [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)],
+    "s3dg_free": [C.c_void_p, C.c_void_p],
+    "s3dg_memcpy_htod": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t],
+    "s3dg_memcpy_dtoh": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t],
+    "s3dg_stream": [C.c_void_p],
+    "s3dg_scene_create": [C.c_void_p, C.POINTER(abi.SceneDescC), C.POINTER(C.c_void_p)],
+    "s3dg_scene_destroy": [C.c_void_p],
+    "s3dg_scene_get_stats": [C.c_void_p, C.POINTER(abi.SceneStats)],
+    "s3dg_scene_mesh_order": [C.c_void_p, C.c_uint32, C.c_void_p],
+    "s3dg_build_order": [C.c_uint32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_trace": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_trace_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_trace_counted_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_render": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.POINTER(RenderStats)],
+    "s3dg_render_device": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.POINTER(RenderStats)],
+    "s3dg_render_guides_device": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_tonemap": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_tonemap_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_nlm": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_nlm_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
+                        C.c_void_p],
+}
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libs3dg.so and declares every entry point; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = C.CDLL(path)
+    for name, args in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    lib.s3dg_last_error.restype = C.c_char_p
+    lib.s3dg_stream.restype = C.c_void_p
+    _lib = lib
+    return lib
+
+
+class S3dgError(RuntimeError):
+    pass
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data
+
+
+def build_order(vertices: np.ndarray, kind: int = abi.BVH8):
+    """Depth-first permutation of the reference SAH build (host only, no device)."""
+    lib = load_library()
+    v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+    order = np.empty(len(v) // 3, dtype=np.int32)
+    nodes = C.c_uint32()
+    rc = lib.s3dg_build_order(len(v), _ptr(v), kind, _ptr(order), C.byref(nodes))
+    if rc != abi.OK:
+        raise S3dgError(f"s3dg_build_order failed ({rc})")
+    return order, nodes.value
+
+
+class Context:
+    """Owns a HIP stream and the device memory of everything created from it."""
+
+    def __init__(self, device: int = -1):
+        self.lib = load_library()
+        h = C.c_void_p()
+        rc = self.lib.s3dg_create(C.byref(h), device)
+        if rc != abi.OK:
+            raise S3dgError(f"s3dg_create failed ({rc}): no usable gfx950 device")
+        self.handle = h
+
+    def check(self, rc: int, what: str):
+        if rc != abi.OK:
+            msg = self.lib.s3dg_last_error(self.handle)
+            raise S3dgError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.s3dg_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self.check(self.lib.s3dg_synchronize(self.handle), "s3dg_synchronize")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.s3dg_stream(self.handle)
+
+    # -- device buffers -------------------------------------------------
+    def malloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self.check(self.lib.s3dg_malloc(self.handle, nbytes, C.byref(p)), "s3dg_malloc")
+        return p.value
+
+    def free(self, ptr: int):
+        self.check(self.lib.s3dg_free(self.handle, C.c_void_p(ptr)), "s3dg_free")
+
+    def upload(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr)
+        p = self.malloc(arr.nbytes)
+        self.check(self.lib.s3dg_memcpy_htod(self.handle, C.c_void_p(p), _ptr(arr), arr.nbytes), "htod")
+        return p
+
+    def download(self, ptr: int, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        self.check(self.lib.s3dg_memcpy_dtoh(self.handle, _ptr(out), C.c_void_p(ptr), out.nbytes), "dtoh")
+        return out
+
+    def copy_to_device(self, ptr: int, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        self.check(self.lib.s3dg_memcpy_htod(self.handle, C.c_void_p(ptr), _ptr(arr), arr.nbytes), "htod")
+
+    # -- post -------------------------------------------------------------
+    def tonemap(self, rgb: np.ndarray, kind: int = abi.TONEMAP_ACES):
+        """Returns (tonemapped float RGB, 8-bit sRGB) of an HxWx3 float image."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        h, w = rgb.shape[:2]
+        out = np.empty_like(rgb)
+        out8 = np.empty(rgb.shape, dtype=np.uint8)
+        self.check(self.lib.s3dg_tonemap(self.handle, kind, w, h, _ptr(rgb), _ptr(out), _ptr(out8)),
+                   "s3dg_tonemap")
+        return out, out8
+
+    def nlm(self, rgba: np.ndarray, coeff: float, search_radius: int = 6, patch_radius: int = 2) -> np.ndarray:
+        rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+        h, w = rgba.shape[:2]
+        out = np.empty_like(rgba)
+        self.check(self.lib.s3dg_nlm(self.handle, w, h, coeff, search_radius, patch_radius, _ptr(rgba),
+                                     _ptr(out)), "s3dg_nlm")
+        return out
+
+
+class Scene:
+    """A scene uploaded to the device (two-level compressed BVH + shading data)."""
+
+    def __init__(self, ctx: Context, desc: SceneDesc):
+        self.ctx = ctx
+        self.desc = desc
+        cdesc, keep = desc.to_c()
+        h = C.c_void_p()
+        ctx.check(ctx.lib.s3dg_scene_create(ctx.handle, C.byref(cdesc), C.byref(h)), "s3dg_scene_create")
+        del keep
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.s3dg_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self) -> SceneStats:
+        s = SceneStats()
+        self.ctx.check(self.ctx.lib.s3dg_scene_get_stats(self.handle, C.byref(s)), "stats")
+        return s
+
+    def mesh_order(self, mesh: int) -> np.ndarray:
+        n = len(self.desc.meshes[mesh].vertices) // 3
+        out = np.empty(n, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.s3dg_scene_mesh_order(self.handle, mesh, _ptr(out)), "mesh_order")
+        return out
+
+    def trace(self, rays: np.ndarray, mode: int = abi.TRACE_CLOSEST, with_shapes: bool = False):
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.empty(len(rays), dtype=HIT_DTYPE)
+        shapes = np.empty(len(rays), dtype=np.int32) if with_shapes else None
+        self.ctx.check(self.ctx.lib.s3dg_trace(self.handle, _ptr(rays), len(rays), mode, _ptr(hits),
+                                               _ptr(shapes)), "s3dg_trace")
+        return (hits, shapes) if with_shapes else hits
+
+    def trace_device(self, d_rays: int, n: int, mode: int, d_hits: int, d_shapes: int | None = None):
+        self.ctx.check(self.ctx.lib.s3dg_trace_device(self.handle, C.c_void_p(d_rays), n, mode,
+                                                      C.c_void_p(d_hits),
+                                                      None if d_shapes is None else C.c_void_p(d_shapes)),
+                       "s3dg_trace_device")
+
+    def trace_counted_device(self, d_rays: int, n: int, mode: int, d_hits: int, d_counters: int):
+        self.ctx.check(self.ctx.lib.s3dg_trace_counted_device(self.handle, C.c_void_p(d_rays), n, mode,
+                                                              C.c_void_p(d_hits), C.c_void_p(d_counters)),
+                       "s3dg_trace_counted_device")
+
+    def render(self, width: int, height: int, spp: int, max_depth: int, seed: int = 0, rr_start_depth: int = 0,
+               first_pass: int = 0, accum: np.ndarray | None = None):
+        """Adds spp passes to accum (HxWx3 sum buffer) and returns (accum, stats)."""
+        if accum is None:
+            accum = np.zeros((height, width, 3), dtype=np.float32)
+        cfg = RenderConfig(width, height, max_depth, spp, seed, rr_start_depth, first_pass)
+        st = RenderStats()
+        self.ctx.check(self.ctx.lib.s3dg_render(self.handle, C.byref(cfg), _ptr(accum), C.byref(st)),
+                       "s3dg_render")
+        return accum, st
+
+    def render_device(self, cfg: RenderConfig, d_accum: int) -> RenderStats:
+        st = RenderStats()
+        self.ctx.check(self.ctx.lib.s3dg_render_device(self.handle, C.byref(cfg), C.c_void_p(d_accum),
+                                                       C.byref(st)), "s3dg_render_device")
+        return st
+
+    def guides_device(self, cfg: RenderConfig, d_albedo: int, d_normal: int, d_depth: int):
+        self.ctx.check(self.ctx.lib.s3dg_render_guides_device(self.handle, C.byref(cfg), C.c_void_p(d_albedo),
+                                                              C.c_void_p(d_normal), C.c_void_p(d_depth)),
+                       "s3dg_render_guides_device")

@@ -1,0 +1,102 @@
+// gpu_types.h -- data layout in HBM shared by the host builder and the gfx950 kernels.
+#pragma once
+#include <stdint.h>
+
+namespace s3dg {
+
+// Compressed 8-wide BVH node: one 128-byte line per node.
+// Child boxes are quantised to 8 bits per plane relative to `origin` with a
+// power-of-two step per axis (2^(exp-127)); quantisation rounds outward so a
+// child box always contains the exact child box it stands for.
+// Internal children of a node are stored contiguously from child_base in slot
+// order; leaf children name a range of the depth-first triangle array.
+struct alignas(128) GNode {
+    float    origin[3];     //   0
+    uint8_t  exp[3];        //  12  biased exponent of the quantisation step
+    uint8_t  imask;         //  15  bit s set: slot s is an internal child
+    uint32_t child_base;    //  16  index of the first internal child
+    uint8_t  cnt[8];        //  20  leaf slot: triangle (or instance) count
+    uint32_t flags;         //  28  bit 0: top-level node (leaves are instances)
+    uint32_t leaf_start[8]; //  32  leaf slot: first triangle / instance
+    uint8_t  qlo[3][8];     //  64  quantised lower planes, [axis][slot]
+    uint8_t  qhi[3][8];     //  88  quantised upper planes
+    uint8_t  pad[16];       // 112
+};
+static_assert(sizeof(GNode) == 128, "GNode must be one 128-byte line");
+
+// Triangle as Triangle::IsHit uses it: v0 and the two edges computed once on
+// the host exactly like the Triangle constructor (36 bytes).
+struct GTri {
+    float v0[3];
+    float e0[3];   // v1 - v0
+    float e1[3];   // v2 - v0
+};
+static_assert(sizeof(GTri) == 36, "GTri is 9 floats");
+
+// Per-triangle shading data, read once per hit.
+struct GTriShade {
+    float n0[3], n1[3], n2[3];
+    int32_t material;
+    int32_t input_index;   // index in the mesh's input order
+    int32_t pad;
+};
+static_assert(sizeof(GTriShade) == 48, "GTriShade is 48 bytes");
+
+enum { INST_MESH = 0, INST_TRANSFORMED = 1, INST_SPHERE = 2 };
+
+// Top-level shape record (96 bytes).
+struct GInst {
+    float    inv[4][4];    // inverse world matrix (row-vector convention)
+    float    center[3];    // sphere
+    float    radius;
+    int32_t  type;         // INST_*
+    uint32_t blas_root;    // root node of the mesh tree
+    int32_t  shape;        // top-level shape index (tie order)
+    int32_t  material;     // sphere material
+};
+static_assert(sizeof(GInst) == 96, "GInst is 96 bytes");
+
+// Material record (64 bytes).
+struct GMat {
+    float   diffuse[3];
+    float   thr0;
+    float   specular[3];
+    float   thr1;
+    float   emissive[3];
+    float   ior;
+    float   power;
+    int32_t type;
+    int32_t delta;
+    int32_t pad;
+};
+static_assert(sizeof(GMat) == 64, "GMat is 64 bytes");
+
+enum { LIGHT_SPHERE = 0, LIGHT_QUAD = 1 };
+
+// Light record (64 bytes).
+struct GLight {
+    float   corner[3];
+    int32_t type;           // LIGHT_*
+    float   e1[3];
+    int32_t shape;          // top-level shape index
+    float   e2[3];
+    float   pdf;            // QUAD: 1/area; SPHERE: 1/(4 pi r^2)
+    int32_t tri[3];         // QUAD: global depth-first indices of its triangles;
+                            // SPHERE: tri[0] = instance record of the sphere
+    int32_t count;          // QUAD: triangle count (<= 3)
+};
+static_assert(sizeof(GLight) == 64, "GLight is 64 bytes");
+
+// Camera in the reference's precomputed form (PinholeCamera::Update).
+struct GCamera {
+    float pos[3];
+    int32_t type;
+    float cx[3];
+    float lens_radius;
+    float cy[3];
+    float focal_distance;
+    float cz[3];
+    float pad;
+};
+
+}  // namespace s3dg

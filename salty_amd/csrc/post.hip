@@ -1,0 +1,269 @@
+// post.hip -- tonemapping + sRGB capture (ToneMapper::Map, PathTracer::Capture)
+// and the non-local-means denoiser (FilterNLM) on gfx950.
+#include <hip/hip_runtime.h>
+#include <float.h>
+
+#include <algorithm>
+
+#include "internal.h"
+
+using namespace s3dg;
+
+namespace {
+
+constexpr int kLumBlocks = 1024;
+constexpr float kLumEps = 0.00001f;   // epsilon passed by every operator (s3d_tonemapper.cpp:248,291,331)
+
+// RGBToY with BT.601 weights (s3d_tonemapper.cpp:20,31-33)
+__device__ __forceinline__ float rgb_to_y(float r, float g, float b) {
+    return (0.29900f * r) + (0.58700f * g) + (0.11400f * b);
+}
+
+// Per-block partial sums of log(eps + Lw) in double and the maximum Lw
+// (ComputeLogarithmicAverage, s3d_tonemapper.cpp:38-95).  The sum is kept in
+// double and reduced in a fixed order, so it is deterministic.
+__global__ __launch_bounds__(256) void k_lum_partial(const float *__restrict__ in, uint32_t n, double *psum,
+                                                      float *pmax) {
+    __shared__ double ssum[256];
+    __shared__ float smax[256];
+    double s = 0.0;
+    float m = 0.0f;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float lw = rgb_to_y(in[3 * i], in[3 * i + 1], in[3 * i + 2]);
+        if (lw != lw) lw = kLumEps;
+        if (lw < FLT_EPSILON) lw = kLumEps;
+        m = fmaxf(m, lw);
+        s += (double)logf(kLumEps + lw);
+    }
+    ssum[threadIdx.x] = s;
+    smax[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        psum[blockIdx.x] = ssum[0];
+        pmax[blockIdx.x] = smax[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_lum_final(const double *psum, const float *pmax, int nb, uint32_t n,
+                                                    float *out /* [0] aveLw, [1] maxLw */) {
+    __shared__ double ssum[256];
+    __shared__ float smax[256];
+    double s = 0.0;
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        s += psum[i];
+        m = fmaxf(m, pmax[i]);
+    }
+    ssum[threadIdx.x] = s;
+    smax[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float ave = (float)ssum[0];
+        if (ave != ave) ave = kLumEps;
+        ave /= (float)n;
+        out[0] = expf(ave);
+        out[1] = smax[0];
+    }
+}
+
+__device__ __forceinline__ float saturate(float v) {
+    const float lo = (1.0f < v) ? 1.0f : v;   // s3d::Min(maxi, value)
+    return (0.0f > lo) ? 0.0f : lo;           // s3d::Max(mini, ...)
+}
+// ACESFilm (s3d_tonemapper.cpp:117-130)
+__device__ __forceinline__ float aces(float x) {
+    const float a = 2.51f, b = 0.03f, c = 2.43f, d = 0.59f, e = 0.14f, f = 0.665406f, g = 12.0f;
+    return saturate((x * (a * x * f / g + b)) / (x * f / g * (c * x * f + d) + e));
+}
+// Uncharted2Tonemap (s3d_tonemapper.cpp:100-112)
+__device__ __forceinline__ float uncharted2(float x) {
+    const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.01f, F = 0.30f;
+    return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+// sRGB encode and 8-bit quantisation of PathTracer::Capture (s3d_pt.cpp:150-166)
+__device__ __forceinline__ uint8_t capture8(float r) {
+    if (r > 1.0f) r = 1.0f;
+    if (r < 0.0f) r = 0.0f;
+    r = (r <= 0.0031308f) ? 12.92f * r : powf(1.055f * r, 1.0f / 2.4f) - 0.055f;
+    return (uint8_t)(r * 255.0f + 0.5f);
+}
+
+__global__ __launch_bounds__(256) void k_tonemap(int32_t type, const float *__restrict__ in, uint32_t n,
+                                                  const float *lum, float *out, uint8_t *out8) {
+    const float ave = lum[0], maxl = lum[1];
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float c[3] = {in[3 * i], in[3 * i + 1], in[3 * i + 2]};
+        if (type == S3DG_TONEMAP_ACES) {
+            const float coeff = 0.27f / ave;
+            for (int k = 0; k < 3; ++k) c[k] = aces(c[k] * coeff);
+        } else if (type == S3DG_TONEMAP_UNCHARTED2) {
+            const float coeff = 0.18f / ave;
+            const float white = uncharted2(11.2f);
+            for (int k = 0; k < 3; ++k) c[k] = uncharted2(2.0f * (c[k] * coeff)) / white;
+        } else {
+            const float coeff = 0.18f / ave;
+            float m2 = maxl * coeff;
+            m2 *= m2;
+            for (int k = 0; k < 3; ++k) {
+                const float L = c[k] * coeff;
+                c[k] = L * (1.0f + (L / m2)) / (1.0f + L);
+            }
+        }
+        if (out) {
+            out[3 * i] = c[0];
+            out[3 * i + 1] = c[1];
+            out[3 * i + 2] = c[2];
+        }
+        if (out8) {
+            out8[3 * i] = capture8(c[0]);
+            out8[3 * i + 1] = capture8(c[1]);
+            out8[3 * i + 2] = capture8(c[2]);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Non-local means: FilterNLM (src/src/s3d_denoiser.cpp:19-95).  One thread per
+// pixel walks the search window and the patch in the reference's order
+// (x offset outer, y offset inner; RGBA distance summed as (xx+yy)+(zz+ww)),
+// with the clamped source tile staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kTile = 16;
+constexpr int kMaxHalo = 12;   // search radius + patch radius
+
+__global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, float frac_h2, float sigma2, int32_t rs,
+                                                      int32_t rp, const float4 *__restrict__ src,
+                                                      float4 *__restrict__ dst) {
+    __shared__ float4 tile[(kTile + 2 * kMaxHalo) * (kTile + 2 * kMaxHalo)];
+    const int halo = rs + rp;
+    const int tw = kTile + 2 * halo;
+    const int x0 = blockIdx.x * kTile - halo, y0 = blockIdx.y * kTile - halo;
+    for (int i = threadIdx.x; i < tw * tw; i += kTile * kTile) {
+        const int tx = i % tw, ty = i / tw;
+        const int gx = min(max(x0 + tx, 0), W - 1);
+        const int gy = min(max(y0 + ty, 0), H - 1);
+        tile[i] = src[gy * W + gx];
+    }
+    __syncthreads();
+    const int lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
+    const int x = blockIdx.x * kTile + lx, y = blockIdx.y * kTile + ly;
+    if (x >= W || y >= H) return;
+    const int cx = lx + halo, cy = ly + halo;   // pixel position inside the tile
+    float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float sw = 0.0f;
+    for (int ox = -rs; ox <= rs; ++ox) {
+        for (int oy = -rs; oy <= rs; ++oy) {
+            float acc = 0.0f;
+            for (int px = -rp; px <= rp; ++px) {
+                for (int py = -rp; py <= rp; ++py) {
+                    const float4 a = tile[(cy + py) * tw + (cx + px)];
+                    const float4 b = tile[(cy + oy + py) * tw + (cx + ox + px)];
+                    const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+                    acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
+                }
+            }
+            const float arg = -fmaxf(acc - 2.0f * sigma2, 0.0f) * frac_h2;
+            const float w = expf(arg);
+            sw += w;
+            const float4 s = tile[(cy + oy) * tw + (cx + ox)];
+            sum.x += s.x * w;
+            sum.y += s.y * w;
+            sum.z += s.z * w;
+            sum.w += s.w * w;
+        }
+    }
+    dst[y * W + x] = make_float4(sum.x / sw, sum.y / sw, sum.z / sw, sum.w / sw);
+}
+
+}  // namespace
+
+extern "C" int s3dg_tonemap_device(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *d_in,
+                                   float *d_out, uint8_t *d_out8) {
+    if (!ctx || !d_in || w <= 0 || h <= 0) return S3DG_ERR_INVALID;
+    if (type < S3DG_TONEMAP_REINHARD || type > S3DG_TONEMAP_ACES)
+        return set_error(ctx, S3DG_ERR_INVALID, "bad tonemap type");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t n = (uint32_t)w * (uint32_t)h;
+    const int nb = (int)std::min<uint32_t>(kLumBlocks, (n + 255) / 256);
+    int rc = ensure_buffer(ctx, &ctx->small, &ctx->small_bytes, kLumBlocks * (sizeof(double) + sizeof(float)) + 64);
+    if (rc) return rc;
+    double *psum = (double *)ctx->small;
+    float *pmax = (float *)(psum + nb);
+    float *lum = pmax + nb;
+    hipLaunchKernelGGL(k_lum_partial, dim3(nb), dim3(256), 0, ctx->stream, d_in, n, psum, pmax);
+    hipLaunchKernelGGL(k_lum_final, dim3(1), dim3(256), 0, ctx->stream, psum, pmax, nb, n, lum);
+    const uint32_t grid = std::min<uint32_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(k_tonemap, dim3(grid), dim3(256), 0, ctx->stream, type, d_in, n, lum, d_out, d_out8);
+    S3DG_HIP(ctx, hipGetLastError());
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_tonemap(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *in, float *out,
+                            uint8_t *out8) {
+    if (!ctx || !in || w <= 0 || h <= 0) return S3DG_ERR_INVALID;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    const size_t fb = n * 3 * sizeof(float), b8 = n * 3;
+    int rc = ensure_buffer(ctx, &ctx->scratch, &ctx->scratch_bytes, 2 * fb + b8 + 256);
+    if (rc) return rc;
+    float *d_in = (float *)ctx->scratch;
+    float *d_out = (float *)((char *)ctx->scratch + fb);
+    uint8_t *d_8 = (uint8_t *)((char *)ctx->scratch + 2 * fb);
+    S3DG_HIP(ctx, hipMemcpyAsync(d_in, in, fb, hipMemcpyHostToDevice, ctx->stream));
+    rc = s3dg_tonemap_device(ctx, type, w, h, d_in, out ? d_out : nullptr, out8 ? d_8 : nullptr);
+    if (rc) return rc;
+    if (out) S3DG_HIP(ctx, hipMemcpyAsync(out, d_out, fb, hipMemcpyDeviceToHost, ctx->stream));
+    if (out8) S3DG_HIP(ctx, hipMemcpyAsync(out8, d_8, b8, hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_nlm_device(s3dg_context *ctx, int32_t w, int32_t h, float coeff, int32_t rs, int32_t rp,
+                               const float *d_src, float *d_dst) {
+    if (!ctx || !d_src || !d_dst || w <= 0 || h <= 0) return S3DG_ERR_INVALID;
+    if (d_src == d_dst) return set_error(ctx, S3DG_ERR_INVALID, "NLM source and destination alias");
+    if (rs < 0 || rp < 0 || rs + rp > kMaxHalo)
+        return set_error(ctx, S3DG_ERR_UNSUPPORTED, "NLM search + patch radius must be <= 12");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    // parameters exactly as FilterNLM derives them (s3d_denoiser.cpp:23-26)
+    const float param_h = std::max(0.0001f, coeff);
+    const float sigma = std::max(0.0001f, coeff);
+    const float frac_h2 = 1.0f / (param_h * param_h);
+    const float sigma2 = sigma * sigma;
+    dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
+    hipLaunchKernelGGL(k_nlm, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp,
+                       (const float4 *)d_src, (float4 *)d_dst);
+    S3DG_HIP(ctx, hipGetLastError());
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_nlm(s3dg_context *ctx, int32_t w, int32_t h, float coeff, int32_t rs, int32_t rp, const float *src,
+                        float *dst) {
+    if (!ctx || !src || !dst || w <= 0 || h <= 0) return S3DG_ERR_INVALID;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t b = (size_t)w * h * 4 * sizeof(float);
+    int rc = ensure_buffer(ctx, &ctx->scratch, &ctx->scratch_bytes, 2 * b);
+    if (rc) return rc;
+    float *d_src = (float *)ctx->scratch;
+    float *d_dst = (float *)((char *)ctx->scratch + b);
+    S3DG_HIP(ctx, hipMemcpyAsync(d_src, src, b, hipMemcpyHostToDevice, ctx->stream));
+    rc = s3dg_nlm_device(ctx, w, h, coeff, rs, rp, d_src, d_dst);
+    if (rc) return rc;
+    S3DG_HIP(ctx, hipMemcpyAsync(dst, d_dst, b, hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}

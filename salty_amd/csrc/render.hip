@@ -1,0 +1,468 @@
+// render.hip -- wavefront path tracer: PathTracer::TracePath / Radiance /
+// NextEventEstimation (src/src/s3d_pt.cpp:182-353) as a pipeline of gfx950
+// kernels over device-resident ray queues.
+//
+// One pass adds one pixel-centre sample per pixel (like one iteration of the
+// reference's progressive loop).  Per bounce: extend (closest hit of the
+// queued rays), shade (emission, light sample, material sample, Russian
+// roulette, compaction into the next queue and the shadow queue) and shadow
+// (visibility of the light sample, then the NEE term).  The order in which a
+// path adds emission, NEE and miss radiance to L is the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+
+#include "internal.h"
+#include "shading.h"
+#include "traverse.h"
+
+using namespace s3dg;
+
+namespace s3dg {
+uint32_t batch_grid(uint32_t n);
+int ensure_gstack(s3dg_scene *sc, uint32_t threads);
+int check_error_flag(s3dg_scene *sc);
+}  // namespace s3dg
+
+namespace {
+
+// Path state, structure of arrays over N = width*height paths.
+struct PathBuf {
+    float *ox, *oy, *oz, *dx, *dy, *dz;     // current ray
+    float *wr, *wg, *wb;                    // throughput W
+    float *lr, *lg, *lb;                    // radiance L
+    uint64_t *rng;                          // PCG state
+    float *ht, *hu, *hv;                    // extension hit
+    int32_t *hprim, *hshape;
+    float *sx, *sy, *sz;                    // shadow ray direction
+    float *swr, *swg, *swb;                 // W at the shading point
+    float *sfr, *sfg, *sfb;                 // fs = base colour / pi
+    float *scos, *sd2;                      // |n.l|, squared light distance
+    int32_t *slight;                        // light index
+    uint32_t *q[2];                         // extension queues (ping-pong)
+    uint32_t *qs;                           // shadow queue
+    uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count
+    unsigned long long *tot;                // [0] extension rays, [1] shadow rays
+};
+
+size_t path_bytes(uint32_t n) {
+    // 29 four-byte fields + 1 eight-byte field per path, three queues, counters
+    return (size_t)n * (29 * 4 + 8 + 3 * 4) + 4096;
+}
+
+PathBuf carve(void *base, uint32_t n) {
+    char *p = (char *)base;
+    auto take = [&](size_t bytes) {
+        char *r = p;
+        p += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    PathBuf b;
+    b.cnt = (uint32_t *)take(64);
+    b.tot = (unsigned long long *)take(64);
+    b.rng = (uint64_t *)take((size_t)n * 8);
+    float **f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.wr, &b.wg, &b.wb, &b.lr, &b.lg, &b.lb,
+                   &b.ht, &b.hu, &b.hv, &b.sx, &b.sy, &b.sz, &b.swr, &b.swg, &b.swb, &b.sfr, &b.sfg, &b.sfb,
+                   &b.scos, &b.sd2};
+    for (float **x : f) *x = (float *)take((size_t)n * 4);
+    b.hprim = (int32_t *)take((size_t)n * 4);
+    b.hshape = (int32_t *)take((size_t)n * 4);
+    b.slight = (int32_t *)take((size_t)n * 4);
+    b.q[0] = (uint32_t *)take((size_t)n * 4);
+    b.q[1] = (uint32_t *)take((size_t)n * 4);
+    b.qs = (uint32_t *)take((size_t)n * 4);
+    return b;
+}
+size_t carve_bytes(uint32_t n) { return path_bytes(n) + 40 * 256; }
+
+// Wave-aggregated queue append: one atomic per wavefront.
+__device__ __forceinline__ uint32_t queue_slot(uint32_t *counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (!want) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+    const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + rank;
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, int32_t W, int32_t H, uint64_t seed,
+                                                    uint32_t pass, uint32_t scene_id) {
+    const uint32_t n = (uint32_t)W * (uint32_t)H;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gtid == 0) {
+        P.cnt[0] = n;
+        P.cnt[1] = 0;
+        P.cnt[2] = 0;
+    }
+    for (uint32_t p = gtid; p < n; p += gridDim.x * blockDim.x) {
+        const uint32_t x = p % (uint32_t)W, y = p / (uint32_t)W;
+        Pcg rng;
+        rng.seed(path_seed(seed, p, pass, scene_id));
+        // TracePath pixel-centre coordinates (s3d_pt.cpp:326-328)
+        const float fx = (0.5f + (float)x) / (float)W - 0.5f;
+        const float fy = (0.5f + (float)y) / (float)H - 0.5f;
+        V3 o, d;
+        camera_ray(S.cam, fx, fy, rng, o, d);
+        P.ox[p] = o.x; P.oy[p] = o.y; P.oz[p] = o.z;
+        P.dx[p] = d.x; P.dy[p] = d.y; P.dz[p] = d.z;
+        P.wr[p] = 1.0f; P.wg[p] = 1.0f; P.wb[p] = 1.0f;
+        P.lr[p] = 0.0f; P.lg[p] = 0.0f; P.lb[p] = 0.0f;
+        P.rng[p] = rng.state;
+        P.q[0][p] = p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap,
+                                                    int32_t *err) {
+    __shared__ uint2 lds[kLdsStack * kBlock];
+    const uint32_t n = P.cnt[cur];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gtid == 0) {
+        P.cnt[cur ^ 1] = 0;
+        P.cnt[2] = 0;
+        P.tot[0] += n;
+    }
+    Stack st;
+    st.lds = lds + threadIdx.x;
+    st.gst = gstack + gtid;
+    st.gstride = stride;
+    st.gcap = gcap;
+    st.err = err;
+    const uint32_t *q = P.q[cur];
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const uint32_t p = q[i];
+        DRay r;
+        r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
+        r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
+        DHit h = no_hit();
+        st.sp = 0;
+        traverse<false, false>(S, r, kRefHitMin, kRefHitMax, h, st, nullptr);
+        P.ht[p] = h.t;
+        P.hu[p] = h.u;
+        P.hv[p] = h.v;
+        P.hprim[p] = h.prim;
+        P.hshape[p] = h.shape;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t cur, int32_t depth,
+                                                   int32_t rr_start) {
+    const uint32_t n = P.cnt[cur];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t *q = P.q[cur];
+    const uint32_t trips = (n + stride - 1) / stride;
+    for (uint32_t it = 0; it < trips; ++it) {
+        const uint32_t i = gtid + it * stride;
+        const bool active = i < n;
+        bool cont = false, shadow = false;
+        uint32_t p = 0;
+        if (active) {
+            p = q[i];
+            V3 W = v3(P.wr[p], P.wg[p], P.wb[p]);
+            V3 L = v3(P.lr[p], P.lg[p], P.lb[p]);
+            const int32_t prim = P.hprim[p];
+            if (prim == -1) {
+                // miss: L += W * environment (SampleIBL(dir)*10 in the reference)
+                L = L + W * ld3(S.env);
+            } else {
+                const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
+                const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
+                const float t = P.ht[p];
+                const V3 pos = o + d * t;
+                V3 normal;
+                int32_t mat;
+                if (prim == -2) {
+                    const GInst &in = S.insts[S.inst_of_shape[P.hshape[p]]];
+                    normal = unit3(pos - ld3(in.center));
+                    mat = in.material;
+                } else {
+                    const GTriShade &sh = S.shade[prim];
+                    normal = tri_normal(sh, P.hu[p], P.hv[p]);
+                    mat = sh.material;
+                    const int32_t ovr = S.insts[S.inst_of_shape[P.hshape[p]]].material;
+                    if (ovr >= 0) mat = ovr;
+                }
+                const GMat &m = S.mats[mat];
+                L = L + W * ld3(m.emissive);
+                Pcg rng;
+                rng.state = P.rng[p];
+                if (!m.delta && S.light_count > 0) {
+                    // NextEventEstimation sampling half (s3d_pt.cpp:258-268)
+                    const uint32_t li = rng.u32() % (uint32_t)S.light_count;
+                    const GLight &lt = S.lights[li];
+                    V3 lpos;
+                    if (lt.type == LIGHT_SPHERE) {
+                        const GInst &ls = S.insts[lt.tri[0]];
+                        // Sphere::Sample (s3d_sphere.cpp:106-115)
+                        const float r1 = k2Pi * rng.f32();
+                        const float r2 = 1.0f - 2.0f * rng.f32();
+                        const float r3 = sqrtf(1.0f - r2 * r2);
+                        lpos = ld3(ls.center) + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (ls.radius + 1e-1f);
+                    } else {
+                        const float u = rng.f32();
+                        const float v = rng.f32();
+                        lpos = ld3(lt.corner) + ld3(lt.e1) * u + ld3(lt.e2) * v;
+                    }
+                    V3 ldir = lpos - pos;
+                    const float d2 = dot3(ldir, ldir);
+                    ldir = safe_unit3(ldir);
+                    const V3 fs = base_color(m) / kPi;
+                    P.sx[p] = ldir.x; P.sy[p] = ldir.y; P.sz[p] = ldir.z;
+                    P.swr[p] = W.x; P.swg[p] = W.y; P.swb[p] = W.z;
+                    P.sfr[p] = fs.x; P.sfg[p] = fs.y; P.sfb[p] = fs.z;
+                    P.scos[p] = fabsf(dot3(normal, ldir));
+                    P.sd2[p] = d2;
+                    P.slight[p] = (int32_t)li;
+                    shadow = true;
+                }
+                V3 out;
+                bool dice = false;
+                W = W * shade(m, d, normal, rng, out, dice);
+                P.rng[p] = rng.state;
+                P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
+                const bool stop = (dice && depth >= rr_start) ||
+                                  (W.x < FLT_EPSILON && W.y < FLT_EPSILON && W.z < FLT_EPSILON);
+                if (!stop) {
+                    P.dx[p] = out.x; P.dy[p] = out.y; P.dz[p] = out.z;
+                    cont = true;
+                }
+                P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
+            }
+            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+        }
+        const uint32_t si = queue_slot(&P.cnt[2], shadow);
+        if (shadow) P.qs[si] = p;
+        const uint32_t ni = queue_slot(&P.cnt[cur ^ 1], cont);
+        if (cont) P.q[cur ^ 1][ni] = p;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap,
+                                                    int32_t *err) {
+    __shared__ uint2 lds[kLdsStack * kBlock];
+    const uint32_t n = P.cnt[2];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gtid == 0) P.tot[1] += n;
+    Stack st;
+    st.lds = lds + threadIdx.x;
+    st.gst = gstack + gtid;
+    st.gstride = stride;
+    st.gcap = gcap;
+    st.err = err;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const uint32_t p = P.qs[i];
+        const GLight &lt = S.lights[P.slight[p]];
+        DRay r;
+        r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
+        r.dx = P.sx[p]; r.dy = P.sy[p]; r.dz = P.sz[p];
+        // Distance to the sampled light along the shadow ray, with the
+        // reference's own test of that shape.
+        float tl = -1.0f, lu = 0.0f, lv = 0.0f;
+        int32_t ltri = -1;
+        int32_t lmat;
+        const GInst *lsph = nullptr;
+        if (lt.type == LIGHT_SPHERE) {
+            lsph = &S.insts[lt.tri[0]];
+            tl = sphere_test(lsph->center[0], lsph->center[1], lsph->center[2], lsph->radius, r, kRefHitMin);
+            lmat = lsph->material;
+        } else {
+            float best = kFMaxF;
+            for (int k = 0; k < lt.count; ++k) {
+                float t, u, v;
+                if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) && t < best) {
+                    best = t; lu = u; lv = v; ltri = lt.tri[k];
+                }
+            }
+            if (ltri >= 0) tl = best;
+            lmat = ltri >= 0 ? S.shade[ltri].material : 0;
+        }
+        if (!(tl >= 0.0f)) continue;
+        // "closest hit is the light" == no other hit strictly before it
+        DHit occ = no_hit();
+        st.sp = 0;
+        traverse<true, false>(S, r, kRefHitMin, nextafterf(tl, 0.0f), occ, st, nullptr);
+        if (occ.prim != -1) continue;
+        // NextEventEstimation contribution (s3d_pt.cpp:274-289)
+        const V3 o = v3(r.ox, r.oy, r.oz), ld = v3(r.dx, r.dy, r.dz);
+        const V3 ph = o + ld * tl;
+        const V3 ln = (lt.type == LIGHT_SPHERE) ? unit3(ph - ld3(lsph->center)) : tri_normal(S.shade[ltri], lu, lv);
+        const float cos_shadow = P.scos[p];
+        const float cos_light = fabsf(dot3(ln, -ld));
+        const float d2 = P.sd2[p];
+        const V3 fs = v3(P.sfr[p], P.sfg[p], P.sfb[p]);
+        const float G = (cos_shadow * cos_light) / d2;
+        const float brdf_pdf = fabsf(ld.z / kPi) * cos_light / d2;
+        const float mis = lt.pdf / (brdf_pdf + lt.pdf);
+        const V3 nee = ld3(S.mats[lmat].emissive) * fs * mis * (G / lt.pdf);
+        const V3 W = v3(P.swr[p], P.swg[p], P.swb[p]);
+        const V3 c = W * nee;
+        P.lr[p] = P.lr[p] + c.x;
+        P.lg[p] = P.lg[p] + c.y;
+        P.lb[p] = P.lb[p] + c.z;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, float *accum, uint32_t n) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        accum[3 * p + 0] = accum[3 * p + 0] + P.lr[p];
+        accum[3 * p + 1] = accum[3 * p + 1] + P.lg[p];
+        accum[3 * p + 2] = accum[3 * p + 2] + P.lb[p];
+    }
+}
+
+__global__ void k_zero_tot(PathBuf P) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        P.tot[0] = 0;
+        P.tot[1] = 0;
+    }
+}
+
+// First-hit guide buffers for the denoisers.
+__global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_t H, uint64_t seed, uint32_t pass,
+                                                    float *albedo, float *normal, float *depth, uint2 *gstack,
+                                                    int32_t gcap, int32_t *err) {
+    __shared__ uint2 lds[kLdsStack * kBlock];
+    const uint32_t n = (uint32_t)W * (uint32_t)H;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds + threadIdx.x;
+    st.gst = gstack + gtid;
+    st.gstride = stride;
+    st.gcap = gcap;
+    st.err = err;
+    for (uint32_t p = gtid; p < n; p += stride) {
+        const uint32_t x = p % (uint32_t)W, y = p / (uint32_t)W;
+        Pcg rng;
+        rng.seed(path_seed(seed, p, pass, 0u));
+        V3 o, d;
+        camera_ray(S.cam, (0.5f + (float)x) / (float)W - 0.5f, (0.5f + (float)y) / (float)H - 0.5f, rng, o, d);
+        DRay r;
+        r.ox = o.x; r.oy = o.y; r.oz = o.z;
+        r.dx = d.x; r.dy = d.y; r.dz = d.z;
+        DHit h = no_hit();
+        st.sp = 0;
+        traverse<false, false>(S, r, kRefHitMin, kRefHitMax, h, st, nullptr);
+        V3 a = v3(0, 0, 0), nn = v3(0, 0, 0);
+        float t = 0.0f;
+        if (h.prim != -1) {
+            const V3 pos = o + d * h.t;
+            int32_t mat;
+            if (h.prim == -2) {
+                const GInst &in = S.insts[S.inst_of_shape[h.shape]];
+                nn = unit3(pos - ld3(in.center));
+                mat = in.material;
+            } else {
+                nn = tri_normal(S.shade[h.prim], h.u, h.v);
+                mat = S.shade[h.prim].material;
+                const int32_t ovr = S.insts[S.inst_of_shape[h.shape]].material;
+                if (ovr >= 0) mat = ovr;
+            }
+            a = base_color(S.mats[mat]);
+            t = h.t;
+        }
+        albedo[3 * p] = a.x; albedo[3 * p + 1] = a.y; albedo[3 * p + 2] = a.z;
+        normal[3 * p] = nn.x; normal[3 * p + 1] = nn.y; normal[3 * p + 2] = nn.z;
+        depth[p] = t;
+    }
+}
+
+}  // namespace
+
+extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg, float *d_accum,
+                                  s3dg_render_stats *stats) {
+    if (!sc || !cfg || !d_accum) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = sc->ctx;
+    if (cfg->width <= 0 || cfg->height <= 0 || cfg->spp < 0 || cfg->max_bounce_count < 0 ||
+        (int64_t)cfg->width * cfg->height > (1ll << 30))
+        return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t n = (uint32_t)cfg->width * (uint32_t)cfg->height;
+    int rc = ensure_buffer(ctx, &sc->d_render, &sc->render_bytes, carve_bytes(n));
+    if (rc) return rc;
+    PathBuf P = carve(sc->d_render, n);
+    const uint32_t grid = batch_grid(n);
+    rc = ensure_gstack(sc, grid * kBlock);
+    if (rc) return rc;
+    const int32_t gcap = std::max(4, sc->dev.stack_need - kLdsStack + 4);
+    uint2 *gs = (uint2 *)sc->d_gstack;
+
+    hipEvent_t e0, e1;
+    S3DG_HIP(ctx, hipEventCreate(&e0));
+    S3DG_HIP(ctx, hipEventCreate(&e1));
+    S3DG_HIP(ctx, hipEventRecord(e0, ctx->stream));
+    hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
+    for (int32_t s = 0; s < cfg->spp; ++s) {
+        const uint32_t pass = (uint32_t)(cfg->first_pass + s);
+        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cfg->width, cfg->height,
+                           cfg->seed, pass, 0u);
+        int32_t cur = 0;
+        for (int32_t depth = 0; depth < cfg->max_bounce_count; ++depth) {
+            hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs, gcap,
+                               sc->d_error);
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, depth,
+                               cfg->rr_start_depth);
+            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
+                               sc->d_error);
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_accum, n);
+    }
+    S3DG_HIP(ctx, hipGetLastError());
+    S3DG_HIP(ctx, hipEventRecord(e1, ctx->stream));
+    unsigned long long tot[2] = {0, 0};
+    S3DG_HIP(ctx, hipMemcpyAsync(tot, P.tot, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    S3DG_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (stats) {
+        stats->extension_rays = tot[0];
+        stats->shadow_rays = tot[1];
+        stats->paths = (uint64_t)n * (uint64_t)cfg->spp;
+        stats->milliseconds = ms;
+    }
+    return check_error_flag(sc);
+}
+
+extern "C" int s3dg_render(s3dg_scene *sc, const s3dg_render_config *cfg, float *accum, s3dg_render_stats *stats) {
+    if (!sc || !cfg || !accum) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = sc->ctx;
+    if (cfg->width <= 0 || cfg->height <= 0) return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
+    const size_t bytes = (size_t)cfg->width * cfg->height * 3 * sizeof(float);
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_buffer(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes);
+    if (rc) return rc;
+    S3DG_HIP(ctx, hipMemcpyAsync(ctx->scratch, accum, bytes, hipMemcpyHostToDevice, ctx->stream));
+    rc = s3dg_render_device(sc, cfg, (float *)ctx->scratch, stats);
+    if (rc) return rc;
+    S3DG_HIP(ctx, hipMemcpyAsync(accum, ctx->scratch, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_render_guides_device(s3dg_scene *sc, const s3dg_render_config *cfg, float *d_albedo,
+                                         float *d_normal, float *d_depth) {
+    if (!sc || !cfg || !d_albedo || !d_normal || !d_depth) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = sc->ctx;
+    if (cfg->width <= 0 || cfg->height <= 0) return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const uint32_t n = (uint32_t)cfg->width * (uint32_t)cfg->height;
+    const uint32_t grid = batch_grid(n);
+    int rc = ensure_gstack(sc, grid * kBlock);
+    if (rc) return rc;
+    const int32_t gcap = std::max(4, sc->dev.stack_need - kLdsStack + 4);
+    hipLaunchKernelGGL(k_guides, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, cfg->width, cfg->height,
+                       cfg->seed, (uint32_t)cfg->first_pass, d_albedo, d_normal, d_depth, (uint2 *)sc->d_gstack,
+                       gcap, sc->d_error);
+    S3DG_HIP(ctx, hipGetLastError());
+    return check_error_flag(sc);
+}

@@ -1,0 +1,226 @@
+// trace.hip -- context entry points and the batch traversal kernels
+// (closest hit and any hit) behind s3dg_trace / s3dg_trace_device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+
+#include "internal.h"
+#include "traverse.h"
+
+using namespace s3dg;
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+extern "C" int s3dg_create(s3dg_context **out, int device) {
+    if (!out) return S3DG_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return S3DG_ERR_NODEVICE;
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) return S3DG_ERR_NODEVICE;
+    }
+    if (device >= count) return S3DG_ERR_NODEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return S3DG_ERR_NODEVICE;
+    if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos) return S3DG_ERR_NODEVICE;
+    s3dg_context *ctx = new (std::nothrow) s3dg_context();
+    if (!ctx) return S3DG_ERR_NOMEM;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return S3DG_ERR_DEVICE;
+    }
+    *out = ctx;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_destroy(s3dg_context *ctx) {
+    if (!ctx) return S3DG_ERR_INVALID;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->small) (void)hipFree(ctx->small);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return S3DG_OK;
+}
+
+extern "C" const char *s3dg_last_error(const s3dg_context *ctx) {
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+extern "C" int s3dg_synchronize(s3dg_context *ctx) {
+    if (!ctx) return S3DG_ERR_INVALID;
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_malloc(s3dg_context *ctx, size_t bytes, void **out) {
+    if (!ctx || !out) return S3DG_ERR_INVALID;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    S3DG_HIP(ctx, hipMalloc(out, bytes ? bytes : 1));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_free(s3dg_context *ctx, void *p) {
+    if (!ctx) return S3DG_ERR_INVALID;
+    if (p) S3DG_HIP(ctx, hipFree(p));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_memcpy_htod(s3dg_context *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return S3DG_ERR_INVALID;
+    S3DG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_memcpy_dtoh(s3dg_context *ctx, void *dst, const void *src, size_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return S3DG_ERR_INVALID;
+    S3DG_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" void *s3dg_stream(s3dg_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// ---------------------------------------------------------------------------
+// Batch traversal
+// ---------------------------------------------------------------------------
+namespace {
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n,
+                                                   s3dg_hit *__restrict__ hits, int32_t *__restrict__ shapes,
+                                                   uint2 *gstack, int32_t gcap, int32_t *err, uint32_t *counters) {
+    __shared__ uint2 lds[kLdsStack * kBlock];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack st;
+    st.lds = lds + threadIdx.x;
+    st.gst = gstack + gtid;
+    st.gstride = stride;
+    st.gcap = gcap;
+    st.err = err;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const float4 a = reinterpret_cast<const float4 *>(rays + i)[0];
+        const float4 b = reinterpret_cast<const float4 *>(rays + i)[1];
+        DRay r;
+        r.ox = a.x; r.oy = a.y; r.oz = a.z;
+        r.dx = b.x; r.dy = b.y; r.dz = b.z;
+        DHit h = no_hit();
+        st.sp = 0;
+        traverse<ANY, COUNT>(S, r, a.w, b.w, h, st, counters);
+        int32_t prim = h.prim;
+        if (prim >= 0) prim = S.shade[prim].input_index;
+        float4 o;
+        o.x = __int_as_float(prim);
+        o.y = h.t;
+        o.z = h.u;
+        o.w = h.v;
+        reinterpret_cast<float4 *>(hits)[i] = o;
+        if (shapes) shapes[i] = h.shape;
+    }
+}
+
+int stack_capacity(const s3dg_scene *sc) { return std::max(4, sc->dev.stack_need - kLdsStack + 4); }
+
+}  // namespace
+
+namespace s3dg {
+
+// Grid for persistent batch kernels: enough blocks to fill every CU several times.
+uint32_t batch_grid(uint32_t n) {
+    const uint32_t want = (n + kBlock - 1) / kBlock;
+    return std::max(1u, std::min(want, 256u * 8u));
+}
+
+int ensure_gstack(s3dg_scene *sc, uint32_t threads) {
+    const size_t need = (size_t)threads * stack_capacity(sc) * sizeof(uint2);
+    return ensure_buffer(sc->ctx, &sc->d_gstack, &sc->gstack_bytes, need);
+}
+
+int check_error_flag(s3dg_scene *sc) {
+    int32_t flag = 0;
+    S3DG_HIP(sc->ctx, hipMemcpyAsync(&flag, sc->d_error, sizeof(flag), hipMemcpyDeviceToHost, sc->ctx->stream));
+    S3DG_HIP(sc->ctx, hipStreamSynchronize(sc->ctx->stream));
+    if (flag) {
+        (void)hipMemsetAsync(sc->d_error, 0, sizeof(int32_t), sc->ctx->stream);
+        return set_error(sc->ctx, S3DG_ERR_UNSUPPORTED, "traversal stack overflow");
+    }
+    return S3DG_OK;
+}
+
+int launch_trace(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mode, s3dg_hit *d_hits,
+                 int32_t *d_shapes, uint32_t *d_counters) {
+    s3dg_context *ctx = sc->ctx;
+    if (n == 0) return S3DG_OK;
+    const uint32_t grid = batch_grid(n);
+    int rc = ensure_gstack(sc, grid * kBlock);
+    if (rc) return rc;
+    uint2 *gs = (uint2 *)sc->d_gstack;
+    const int32_t cap = stack_capacity(sc);
+    if (mode == S3DG_TRACE_ANY) {
+        if (d_counters)
+            hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+        else
+            hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+    } else {
+        if (d_counters)
+            hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+        else
+            hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+    }
+    S3DG_HIP(ctx, hipGetLastError());
+    return S3DG_OK;
+}
+
+}  // namespace s3dg
+
+extern "C" int s3dg_trace_device(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mode,
+                                 s3dg_hit *d_hits, int32_t *d_shapes) {
+    if (!sc || (n && (!d_rays || !d_hits))) return S3DG_ERR_INVALID;
+    if (mode != S3DG_TRACE_CLOSEST && mode != S3DG_TRACE_ANY) return set_error(sc->ctx, S3DG_ERR_INVALID, "bad mode");
+    S3DG_HIP(sc->ctx, hipSetDevice(sc->ctx->device));
+    int rc = launch_trace(sc, d_rays, n, mode, d_hits, d_shapes, nullptr);
+    if (rc) return rc;
+    return check_error_flag(sc);
+}
+
+// Variant that also counts node visits and triangle tests (counters[0], [1]),
+// for the algorithmic-bytes figure of the roofline.
+extern "C" int s3dg_trace_counted_device(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mode,
+                                         s3dg_hit *d_hits, uint32_t *d_counters) {
+    if (!sc || !d_counters || (n && (!d_rays || !d_hits))) return S3DG_ERR_INVALID;
+    S3DG_HIP(sc->ctx, hipSetDevice(sc->ctx->device));
+    int rc = launch_trace(sc, d_rays, n, mode, d_hits, nullptr, d_counters);
+    if (rc) return rc;
+    return check_error_flag(sc);
+}
+
+extern "C" int s3dg_trace(s3dg_scene *sc, const s3dg_ray *rays, uint32_t n, int32_t mode, s3dg_hit *hits,
+                          int32_t *shapes) {
+    if (!sc || (n && (!rays || !hits))) return S3DG_ERR_INVALID;
+    if (n == 0) return S3DG_OK;
+    s3dg_context *ctx = sc->ctx;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t rb = (size_t)n * sizeof(s3dg_ray), hb = (size_t)n * sizeof(s3dg_hit), sb = (size_t)n * 4;
+    int rc = ensure_buffer(ctx, &ctx->scratch, &ctx->scratch_bytes, rb + hb + sb);
+    if (rc) return rc;
+    char *base = (char *)ctx->scratch;
+    s3dg_ray *d_rays = (s3dg_ray *)base;
+    s3dg_hit *d_hits = (s3dg_hit *)(base + rb);
+    int32_t *d_shapes = (int32_t *)(base + rb + hb);
+    S3DG_HIP(ctx, hipMemcpyAsync(d_rays, rays, rb, hipMemcpyHostToDevice, ctx->stream));
+    rc = s3dg_trace_device(sc, d_rays, n, mode, d_hits, shapes ? d_shapes : nullptr);
+    if (rc) return rc;
+    S3DG_HIP(ctx, hipMemcpyAsync(hits, d_hits, hb, hipMemcpyDeviceToHost, ctx->stream));
+    if (shapes) S3DG_HIP(ctx, hipMemcpyAsync(shapes, d_shapes, sb, hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return S3DG_OK;
+}

@@ -1,0 +1,30 @@
+"""Render parity: the wavefront GPU path tracer against the oracle's
+restatement of PathTracer::Radiance / NextEventEstimation, same counter-keyed
+RNG streams, same spp and seed."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from salty_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_mse(a, b, eps=1e-2):
+    return float(np.mean((a - b) ** 2 / (b ** 2 + eps)))
+
+
+def test_cornell_matches_oracle(ctx):
+    import salty_amd
+    desc = scenes.cornell_box()
+    sc = salty_amd.Scene(ctx, desc)
+    w = h = 96
+    gpu, st = sc.render(w, h, spp=4, max_depth=8, seed=11)
+    ref, counts = O.OracleScene(desc).render(w, h, spp=4, max_depth=8, seed=11)
+    assert np.isfinite(gpu).all()
+    assert rel_mse(gpu, ref) < 1e-4
+    # identical streams: the GPU traced the same number of rays (+-0.1% for
+    # paths that diverge on last-ulp differences of cos/sin)
+    assert abs(int(st.extension_rays) - int(counts[0])) <= 1e-3 * counts[0]
+    assert abs(int(st.shadow_rays) - int(counts[1])) <= 1e-3 * counts[1]
+    assert st.paths == counts[2] == w * h * 4
