@@ -182,13 +182,28 @@ typedef struct s3dg_render_config {
     uint64_t seed;             /* counter-based RNG key                        */
     int32_t  rr_start_depth;   /* dice honoured from this depth on (0 = reference) */
     int32_t  first_pass;       /* pass index of the first sample (progressive) */
+    uint32_t flags;            /* S3DG_RENDER_* instrumentation bits           */
+    uint32_t reserved;
 } s3dg_render_config;
+
+/* flags: time every traversal launch with HIP events on the context stream;
+ * count compressed nodes visited and triangles tested by the traversals. */
+#define S3DG_RENDER_TIME_KERNELS   1u
+#define S3DG_RENDER_COUNT_WORK     2u
 
 typedef struct s3dg_render_stats {
     uint64_t extension_rays;   /* closest-hit rays traced                      */
     uint64_t shadow_rays;      /* NEE visibility rays traced                   */
     uint64_t paths;            /* camera samples completed                     */
     double   milliseconds;     /* device time of the render                    */
+    double   extend_ms;        /* TIME_KERNELS: summed closest-hit kernel time */
+    double   shadow_ms;        /* TIME_KERNELS: summed shadow kernel time      */
+    uint64_t extend_launches;
+    uint64_t shadow_launches;
+    uint64_t extend_nodes;     /* COUNT_WORK: nodes visited by extension rays  */
+    uint64_t extend_tris;      /* COUNT_WORK: triangles tested                 */
+    uint64_t shadow_nodes;
+    uint64_t shadow_tris;
 } s3dg_render_stats;
 
 typedef struct s3dg_scene_stats {
@@ -262,11 +277,11 @@ int s3dg_trace(s3dg_scene *scene, const s3dg_ray *rays, uint32_t count, int32_t 
 int s3dg_trace_device(s3dg_scene *scene, const s3dg_ray *d_rays, uint32_t count, int32_t mode,
                       s3dg_hit *d_hits, int32_t *d_shapes);
 /* s3dg_trace_device that also adds the number of compressed nodes visited and
- * triangles tested to d_counters[0] and d_counters[1] (uint32, device), the
+ * triangles tested to d_counters[0] and d_counters[1] (uint64, device), the
  * inputs of the traversal kernel's algorithmic-bytes figure.  No reference
  * counterpart (instrumentation). */
 int s3dg_trace_counted_device(s3dg_scene *scene, const s3dg_ray *d_rays, uint32_t count, int32_t mode,
-                              s3dg_hit *d_hits, uint32_t *d_counters);
+                              s3dg_hit *d_hits, uint64_t *d_counters);
 
 /* ---------------------------------------------------------------------------
  * Rendering: replaces PathTracer::TracePath / Radiance / NextEventEstimation

@@ -200,14 +200,14 @@ class OracleScene:
                accum=None, threads: int = 0):
         if accum is None:
             accum = np.zeros((height, width, 3), np.float32)
-        cfg = abi.RenderConfig(width, height, max_depth, spp, seed, rr_start_depth, first_pass)
+        cfg = abi.RenderConfig(width, height, max_depth, spp, seed, rr_start_depth, first_pass, 0, 0)
         r0, r1 = (0, height) if rows is None else rows
         counts = np.zeros(3, np.uint64)
         lib().orc_render(self.h, C.byref(cfg), r0, r1, _p(accum), _p(counts), threads)
         return accum, counts
 
     def guides(self, width, height, seed=0, first_pass=0):
-        cfg = abi.RenderConfig(width, height, 1, 1, seed, 0, first_pass)
+        cfg = abi.RenderConfig(width, height, 1, 1, seed, 0, first_pass, 0, 0)
         a = np.zeros((height, width, 3), np.float32)
         n = np.zeros((height, width, 3), np.float32)
         d = np.zeros((height, width), np.float32)

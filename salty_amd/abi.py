@@ -72,12 +72,20 @@ class SceneDescC(C.Structure):
 
 class RenderConfig(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_bounce_count", C.c_int32), ("spp", C.c_int32),
-                ("seed", C.c_uint64), ("rr_start_depth", C.c_int32), ("first_pass", C.c_int32)]
+                ("seed", C.c_uint64), ("rr_start_depth", C.c_int32), ("first_pass", C.c_int32),
+                ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+RENDER_TIME_KERNELS = 1
+RENDER_COUNT_WORK = 2
 
 
 class RenderStats(C.Structure):
     _fields_ = [("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
-                ("milliseconds", C.c_double)]
+                ("milliseconds", C.c_double), ("extend_ms", C.c_double), ("shadow_ms", C.c_double),
+                ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
+                ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64)]
 
 
 class SceneStats(C.Structure):

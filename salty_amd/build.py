@@ -72,6 +72,10 @@ def build(jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # C++ drop-in example: plain g++ against the C ABI, no HIP headers.
+    host = os.path.join(HERE, "host")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", host, os.path.join(host, "cornell_main.cpp"),
+          "-L", HERE, "-ls3dg", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(host, "s3d_cornell")])
     return LIB
 
 

@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstring>
+#include <vector>
 
 #include "internal.h"
 #include "shading.h"
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, int32_
     }
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap,
                                                     int32_t *err) {
     __shared__ uint2 lds[kLdsStack * kBlock];
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_
         r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
         DHit h = no_hit();
         st.sp = 0;
-        traverse<false, false>(S, r, kRefHitMin, kRefHitMax, h, st, nullptr);
+        traverse<false, COUNT>(S, r, kRefHitMin, kRefHitMax, h, st, P.tot + 2);
         P.ht[p] = h.t;
         P.hu[p] = h.u;
         P.hv[p] = h.v;
@@ -243,6 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
     }
 }
 
+template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap,
                                                     int32_t *err) {
     __shared__ uint2 lds[kLdsStack * kBlock];
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 
         // "closest hit is the light" == no other hit strictly before it
         DHit occ = no_hit();
         st.sp = 0;
-        traverse<true, false>(S, r, kRefHitMin, nextafterf(tl, 0.0f), occ, st, nullptr);
+        traverse<true, COUNT>(S, r, kRefHitMin, nextafterf(tl, 0.0f), occ, st, P.tot + 4);
         if (occ.prim != -1) continue;
         // NextEventEstimation contribution (s3d_pt.cpp:274-289)
         const V3 o = v3(r.ox, r.oy, r.oz), ld = v3(r.dx, r.dy, r.dz);
@@ -318,10 +322,7 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, float *accum, 
 }
 
 __global__ void k_zero_tot(PathBuf P) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        P.tot[0] = 0;
-        P.tot[1] = 0;
-    }
+    if (threadIdx.x < 6 && blockIdx.x == 0) P.tot[threadIdx.x] = 0;
 }
 
 // First-hit guide buffers for the denoisers.
@@ -394,41 +395,82 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
     const int32_t gcap = std::max(4, sc->dev.stack_need - kLdsStack + 4);
     uint2 *gs = (uint2 *)sc->d_gstack;
 
+    const bool timed = (cfg->flags & S3DG_RENDER_TIME_KERNELS) != 0;
+    const bool count = (cfg->flags & S3DG_RENDER_COUNT_WORK) != 0;
+    const size_t launches = (size_t)cfg->spp * (size_t)cfg->max_bounce_count;
+    std::vector<hipEvent_t> ev;
+    if (timed) {
+        ev.resize(4 * launches);
+        for (auto &e : ev) S3DG_HIP(ctx, hipEventCreate(&e));
+    }
     hipEvent_t e0, e1;
     S3DG_HIP(ctx, hipEventCreate(&e0));
     S3DG_HIP(ctx, hipEventCreate(&e1));
     S3DG_HIP(ctx, hipEventRecord(e0, ctx->stream));
     hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
+    size_t li = 0;
     for (int32_t s = 0; s < cfg->spp; ++s) {
         const uint32_t pass = (uint32_t)(cfg->first_pass + s);
         hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cfg->width, cfg->height,
                            cfg->seed, pass, 0u);
         int32_t cur = 0;
-        for (int32_t depth = 0; depth < cfg->max_bounce_count; ++depth) {
-            hipLaunchKernelGGL(k_extend, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs, gcap,
-                               sc->d_error);
+        for (int32_t depth = 0; depth < cfg->max_bounce_count; ++depth, ++li) {
+            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li], ctx->stream));
+            if (count)
+                hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
+                                   gcap, sc->d_error);
+            else
+                hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
+                                   gcap, sc->d_error);
+            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 1], ctx->stream));
             hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, depth,
                                cfg->rr_start_depth);
-            hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
-                               sc->d_error);
+            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 2], ctx->stream));
+            if (count)
+                hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
+                                   sc->d_error);
+            else
+                hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
+                                   sc->d_error);
+            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 3], ctx->stream));
             cur ^= 1;
         }
         hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_accum, n);
     }
     S3DG_HIP(ctx, hipGetLastError());
     S3DG_HIP(ctx, hipEventRecord(e1, ctx->stream));
-    unsigned long long tot[2] = {0, 0};
+    unsigned long long tot[6] = {0, 0, 0, 0, 0, 0};
     S3DG_HIP(ctx, hipMemcpyAsync(tot, P.tot, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
     S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     S3DG_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+    double ext_ms = 0.0, sh_ms = 0.0;
+    if (timed) {
+        for (size_t i = 0; i < launches; ++i) {
+            float a = 0.0f, b = 0.0f;
+            S3DG_HIP(ctx, hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
+            S3DG_HIP(ctx, hipEventElapsedTime(&b, ev[4 * i + 2], ev[4 * i + 3]));
+            ext_ms += a;
+            sh_ms += b;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (stats) {
+        memset(stats, 0, sizeof(*stats));
         stats->extension_rays = tot[0];
         stats->shadow_rays = tot[1];
         stats->paths = (uint64_t)n * (uint64_t)cfg->spp;
         stats->milliseconds = ms;
+        stats->extend_ms = ext_ms;
+        stats->shadow_ms = sh_ms;
+        stats->extend_launches = timed ? launches : 0;
+        stats->shadow_launches = timed ? launches : 0;
+        stats->extend_nodes = tot[2];
+        stats->extend_tris = tot[3];
+        stats->shadow_nodes = tot[4];
+        stats->shadow_tris = tot[5];
     }
     return check_error_flag(sc);
 }

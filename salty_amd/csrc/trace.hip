@@ -93,7 +93,7 @@ namespace {
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n,
                                                    s3dg_hit *__restrict__ hits, int32_t *__restrict__ shapes,
-                                                   uint2 *gstack, int32_t gcap, int32_t *err, uint32_t *counters) {
+                                                   uint2 *gstack, int32_t gcap, int32_t *err, unsigned long long *counters) {
     __shared__ uint2 lds[kLdsStack * kBlock];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -153,7 +153,7 @@ int check_error_flag(s3dg_scene *sc) {
 }
 
 int launch_trace(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mode, s3dg_hit *d_hits,
-                 int32_t *d_shapes, uint32_t *d_counters) {
+                 int32_t *d_shapes, unsigned long long *d_counters) {
     s3dg_context *ctx = sc->ctx;
     if (n == 0) return S3DG_OK;
     const uint32_t grid = batch_grid(n);
@@ -195,10 +195,10 @@ extern "C" int s3dg_trace_device(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_
 // Variant that also counts node visits and triangle tests (counters[0], [1]),
 // for the algorithmic-bytes figure of the roofline.
 extern "C" int s3dg_trace_counted_device(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mode,
-                                         s3dg_hit *d_hits, uint32_t *d_counters) {
+                                         s3dg_hit *d_hits, uint64_t *d_counters) {
     if (!sc || !d_counters || (n && (!d_rays || !d_hits))) return S3DG_ERR_INVALID;
     S3DG_HIP(sc->ctx, hipSetDevice(sc->ctx->device));
-    int rc = launch_trace(sc, d_rays, n, mode, d_hits, nullptr, d_counters);
+    int rc = launch_trace(sc, d_rays, n, mode, d_hits, nullptr, (unsigned long long *)d_counters);
     if (rc) return rc;
     return check_error_flag(sc);
 }

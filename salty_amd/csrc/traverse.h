@@ -190,7 +190,7 @@ struct Stack {
 // Closest hit (ANY = false) or first hit found (ANY = true) in [tmin, tmax].
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ void traverse(const SceneDev &S, const DRay &wray, float tmin, float tmax,
-                                         DHit &best, Stack &st, uint32_t *counters) {
+                                         DHit &best, Stack &st, unsigned long long *counters) {
     DRay ray = wray;
     RayPre pre = precompute(ray);
     int32_t cur_shape = S.single_shape;
@@ -325,7 +325,7 @@ __device__ __forceinline__ void traverse(const SceneDev &S, const DRay &wray, fl
                             best.v = v;
                             best.prim = (int32_t)i;
                             best.shape = cur_shape;
-                            if (COUNT) { atomicAdd(&counters[0], visits); atomicAdd(&counters[1], tests); }
+                            if (COUNT) { atomicAdd(&counters[0], (unsigned long long)visits); atomicAdd(&counters[1], (unsigned long long)tests); }
                             return;
                         }
                         const bool take = t < best.t ||
@@ -355,7 +355,7 @@ __device__ __forceinline__ void traverse(const SceneDev &S, const DRay &wray, fl
             have_node = false;
         }
     }
-    if (COUNT) { atomicAdd(&counters[0], visits); atomicAdd(&counters[1], tests); }
+    if (COUNT) { atomicAdd(&counters[0], (unsigned long long)visits); atomicAdd(&counters[1], (unsigned long long)tests); }
 }
 
 }  // namespace s3dg
