@@ -322,6 +322,31 @@ int s3dg_nlm_device(s3dg_context *ctx, int32_t width, int32_t height, float coef
                     int32_t search_radius, int32_t patch_radius,
                     const float *d_src_rgba, float *d_dst_rgba);
 
+/* ---------------------------------------------------------------------------
+ * Learned denoiser (no reference counterpart: the learned stand-in for
+ * FilterNLM).  A 4-level U-Net (3x3 convolutions, 32/64/96/128 channels, ReLU,
+ * average pooling, nearest upsampling, concatenated skips, 1x1 head to log
+ * radiance) run with BF16 MFMA kernels and FP32 accumulation.
+ * Input features: N x H x W x 10 floats (radiance 3, albedo 3, normal 3,
+ * depth 1), H and W multiples of 8; output N x H x W x 3 linear radiance.
+ * Training minimises the L1 loss on log(1 + radiance) with Adam.
+ * ------------------------------------------------------------------------- */
+typedef struct s3dg_cnn s3dg_cnn;
+
+int s3dg_cnn_create(s3dg_context *ctx, uint64_t seed, s3dg_cnn **out);
+int s3dg_cnn_destroy(s3dg_cnn *cnn);
+int s3dg_cnn_param_count(const s3dg_cnn *cnn, uint64_t *count);
+/* Flat FP32 parameters: per layer W[9][cout][cin] then b[cout], head last. */
+int s3dg_cnn_get_params(s3dg_cnn *cnn, float *host, uint64_t count);
+int s3dg_cnn_set_params(s3dg_cnn *cnn, const float *host, uint64_t count);
+int s3dg_cnn_infer_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
+                          float *d_out_rgb);
+/* Mean L1 loss and the gradient of every parameter (host copy optional). */
+int s3dg_cnn_gradient_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
+                             const float *d_target_rgb, double *loss, float *host_grad, uint64_t count);
+int s3dg_cnn_train_step_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
+                               const float *d_target_rgb, float learning_rate, double *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,16 @@ _SIGNATURES = {
     "s3dg_nlm": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
     "s3dg_nlm_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                         C.c_void_p],
+    "s3dg_cnn_create": [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)],
+    "s3dg_cnn_destroy": [C.c_void_p],
+    "s3dg_cnn_param_count": [C.c_void_p, C.POINTER(C.c_uint64)],
+    "s3dg_cnn_get_params": [C.c_void_p, C.c_void_p, C.c_uint64],
+    "s3dg_cnn_set_params": [C.c_void_p, C.c_void_p, C.c_uint64],
+    "s3dg_cnn_infer_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_cnn_gradient_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_double), C.c_void_p, C.c_uint64],
+    "s3dg_cnn_train_step_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_float,
+                                   C.POINTER(C.c_double)],
 }
 
 
@@ -240,3 +250,76 @@ class Scene:
         self.ctx.check(self.ctx.lib.s3dg_render_guides_device(self.handle, C.byref(cfg), C.c_void_p(d_albedo),
                                                               C.c_void_p(d_normal), C.c_void_p(d_depth)),
                        "s3dg_render_guides_device")
+
+
+class Denoiser:
+    """The U-Net denoiser (BF16 MFMA kernels) on a context."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.lib.s3dg_cnn_create(ctx.handle, seed, C.byref(h)), "s3dg_cnn_create")
+        self.handle = h
+        n = C.c_uint64()
+        ctx.check(ctx.lib.s3dg_cnn_param_count(h, C.byref(n)), "s3dg_cnn_param_count")
+        self.nparam = n.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.s3dg_cnn_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.nparam, np.float32)
+        self.ctx.check(self.ctx.lib.s3dg_cnn_get_params(self.handle, _ptr(out), self.nparam), "get_params")
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparam
+        self.ctx.check(self.ctx.lib.s3dg_cnn_set_params(self.handle, _ptr(p), self.nparam), "set_params")
+
+    def infer_device(self, n, h, w, d_feat: int, d_out: int):
+        self.ctx.check(self.ctx.lib.s3dg_cnn_infer_device(self.handle, n, h, w, C.c_void_p(d_feat),
+                                                          C.c_void_p(d_out)), "s3dg_cnn_infer_device")
+
+    def infer(self, features: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        n, h, w, _ = f.shape
+        d_f = self.ctx.upload(f)
+        d_o = self.ctx.malloc(n * h * w * 3 * 4)
+        try:
+            self.infer_device(n, h, w, d_f, d_o)
+            return self.ctx.download(d_o, (n, h, w, 3), np.float32)
+        finally:
+            self.ctx.free(d_f)
+            self.ctx.free(d_o)
+
+    def gradient(self, features: np.ndarray, target: np.ndarray):
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        n, h, w, _ = f.shape
+        d_f, d_t = self.ctx.upload(f), self.ctx.upload(t)
+        g = np.empty(self.nparam, np.float32)
+        loss = C.c_double()
+        try:
+            self.ctx.check(self.ctx.lib.s3dg_cnn_gradient_device(self.handle, n, h, w, C.c_void_p(d_f),
+                                                                 C.c_void_p(d_t), C.byref(loss), _ptr(g),
+                                                                 self.nparam), "s3dg_cnn_gradient_device")
+        finally:
+            self.ctx.free(d_f)
+            self.ctx.free(d_t)
+        return loss.value, g
+
+    def train_step_device(self, n, h, w, d_feat: int, d_target: int, lr: float) -> float:
+        loss = C.c_double()
+        self.ctx.check(self.ctx.lib.s3dg_cnn_train_step_device(self.handle, n, h, w, C.c_void_p(d_feat),
+                                                               C.c_void_p(d_target), lr, C.byref(loss)),
+                       "s3dg_cnn_train_step_device")
+        return loss.value

@@ -1,0 +1,766 @@
+// cnn.hip -- albedo/normal/depth-guided U-Net denoiser with hand-written BF16
+// MFMA kernels (gfx950 v_mfma_f32_16x16x32_bf16, FP32 accumulation) for the
+// forward pass, the backward pass (data and weight gradients) and Adam.
+//
+// Not in the reference (it has only FilterNLM); this is the learned stand-in
+// for NLM.  Network: 4 levels, 3x3 convolutions with 32/64/96/128 channels,
+// ReLU, 2x2 average pooling, nearest upsampling, concatenated skips, and a
+// 1x1 head to log radiance.  Activations are NHWC bf16; weights keep an FP32
+// master copy and two bf16 copies: [tap][cout][cin] for the forward pass and
+// the flipped transpose [tap][cin][cout] for the data gradient.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <random>
+#include <vector>
+
+#include "internal.h"
+
+using namespace s3dg;
+
+typedef __bf16 bf16;
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+namespace {
+
+constexpr int kInChannels = 10;     // radiance 3, albedo 3, normal 3, depth 1
+constexpr int kInPadded = 32;       // first layer input padded to one MFMA K step
+constexpr int kOut = 3;
+
+__device__ __forceinline__ float b2f(bf16 x) { return (float)x; }
+__device__ __forceinline__ bf16 f2b(float x) { return (bf16)x; }
+
+// ---------------------------------------------------------------------------
+// Virtual convolution input: concat(upsample_s1(in1), in2) along channels.
+// in1: [N, H/s1, W/s1, C1], in2: [N, H, W, C2] (may be null with C2 = 0).
+// ---------------------------------------------------------------------------
+struct ConvIn {
+    const bf16 *in1;
+    const bf16 *in2;
+    int32_t C1, C2, s1;
+};
+
+// 8 consecutive channels c..c+7 (all in the same source) at pixel (n, y, x);
+// zero outside the image.
+__device__ __forceinline__ bf16x8 load8(const ConvIn &I, int n, int y, int x, int H, int W, int c) {
+    bf16x8 v;
+    if (y < 0 || y >= H || x < 0 || x >= W) {
+        for (int j = 0; j < 8; ++j) v[j] = f2b(0.0f);
+        return v;
+    }
+    if (c < I.C1) {
+        const int h1 = (H + I.s1 - 1) / I.s1, w1 = (W + I.s1 - 1) / I.s1;
+        const size_t off = (((size_t)n * h1 + y / I.s1) * w1 + x / I.s1) * I.C1 + c;
+        return *reinterpret_cast<const bf16x8 *>(I.in1 + off);
+    }
+    const size_t off = (((size_t)n * H + y) * W + x) * I.C2 + (c - I.C1);
+    return *reinterpret_cast<const bf16x8 *>(I.in2 + off);
+}
+
+__device__ __forceinline__ float load1(const ConvIn &I, int n, int y, int x, int H, int W, int c) {
+    if (y < 0 || y >= H || x < 0 || x >= W) return 0.0f;
+    if (c < I.C1) {
+        const int h1 = (H + I.s1 - 1) / I.s1, w1 = (W + I.s1 - 1) / I.s1;
+        return b2f(I.in1[(((size_t)n * h1 + y / I.s1) * w1 + x / I.s1) * I.C1 + c]);
+    }
+    return b2f(I.in2[(((size_t)n * H + y) * W + x) * I.C2 + (c - I.C1)]);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution (stride 1, zero padding) as an implicit GEMM:
+// M = pixels, N = output channels, K = 9 taps x Cin.  A workgroup of four
+// waves covers 64 consecutive pixels of one row; each wave 16 pixels x 16*NT
+// output channels.  Used for the forward pass (weights [tap][cout][cin]) and
+// for the data gradient (flipped transposed weights, dY as input).
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
+                                                  const bf16 *__restrict__ wt, const float *__restrict__ bias,
+                                                  int32_t Cout, int32_t relu, bf16 *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * 64 + wave * 16;
+    const int y = blockIdx.y;
+    const int n = blockIdx.z / ((Cout + 16 * NT - 1) / (16 * NT));
+    const int co0 = (blockIdx.z % ((Cout + 16 * NT - 1) / (16 * NT))) * 16 * NT;
+    const int Cin = I.C1 + I.C2;
+    const int px = x0 + (lane & 15);         // A row of this lane
+    const int kg = (lane >> 4) * 8;          // K sub-block of this lane
+    f32x4 acc[NT];
+    for (int t = 0; t < NT; ++t)
+        for (int i = 0; i < 4; ++i) acc[t][i] = 0.0f;
+    if (x0 < W) {
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+            const bf16 *wtap = wt + (size_t)tap * Cout * Cin;
+            for (int c0 = 0; c0 < Cin; c0 += 32) {
+                const bf16x8 a = load8(I, n, y + dy, px + dx, H, W, c0 + kg);
+#pragma unroll
+                for (int t = 0; t < NT; ++t) {
+                    const int co = co0 + 16 * t + (lane & 15);
+                    bf16x8 b;
+                    if (co < Cout) b = *reinterpret_cast<const bf16x8 *>(wtap + (size_t)co * Cin + c0 + kg);
+                    else
+                        for (int j = 0; j < 8; ++j) b[j] = f2b(0.0f);
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D layout: row (pixel) = 4*(lane/16) + i, column (channel) = lane % 16
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = co0 + 16 * t + (lane & 15);
+        if (co >= Cout) continue;
+        const float bv = bias ? bias[co] : 0.0f;
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + 4 * (lane >> 4) + i;
+            if (x >= W) continue;
+            float v = acc[t][i] + bv;
+            if (relu) v = fmaxf(v, 0.0f);
+            out[(((size_t)n * H + y) * W + x) * Cout + co] = f2b(v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[tap][co][ci] += sum_p dPre[p][co] * X[p + off(tap)][ci]
+// as a GEMM with M = co, N = ci, K = pixels.  Grid: x = tap * (Cin/32) +
+// ci-chunk, y = co-chunk of 64, z = pixel slice.  Each wave accumulates a
+// 64 x 32 tile over every fourth 32-pixel run of the slice; the four waves are
+// reduced through LDS and added to dW with one FP32 atomic per element.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H, int32_t W,
+                                                  const bf16 *__restrict__ dpre, int32_t Cout,
+                                                  int32_t runs_per_slice, float *__restrict__ dw) {
+    __shared__ float red[4][64 * 32];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Cin = I.C1 + I.C2;
+    const int nchunk = Cin / 32;
+    const int tap = blockIdx.x / nchunk;
+    const int ci0 = (blockIdx.x % nchunk) * 32;
+    const int co0 = blockIdx.y * 64;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const long long npix = (long long)N * H * W;
+    const long long nruns = (npix + 31) / 32;
+    const long long r0 = (long long)blockIdx.z * runs_per_slice;
+    f32x4 acc[4][2];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int i = 0; i < 4; ++i) acc[a][b][i] = 0.0f;
+    const int kg = (lane >> 4) * 8;
+    for (long long r = r0 + wave; r < r0 + runs_per_slice && r < nruns; r += 4) {
+        const long long pbase = r * 32 + kg;
+        bf16x8 A[4], B[2];
+        int pn[8], py[8], pxx[8];
+        bool pv[8];
+        for (int j = 0; j < 8; ++j) {
+            const long long p = pbase + j;
+            pv[j] = p < npix;
+            const long long pp = pv[j] ? p : 0;
+            pxx[j] = (int)(pp % W);
+            py[j] = (int)((pp / W) % H);
+            pn[j] = (int)(pp / ((long long)W * H));
+        }
+        for (int m = 0; m < 4; ++m) {
+            const int co = co0 + 16 * m + (lane & 15);
+            for (int j = 0; j < 8; ++j) {
+                float v = 0.0f;
+                if (pv[j] && co < Cout) v = b2f(dpre[(((size_t)pn[j] * H + py[j]) * W + pxx[j]) * Cout + co]);
+                A[m][j] = f2b(v);
+            }
+        }
+        for (int t = 0; t < 2; ++t) {
+            const int ci = ci0 + 16 * t + (lane & 15);
+            for (int j = 0; j < 8; ++j)
+                B[t][j] = f2b(pv[j] ? load1(I, pn[j], py[j] + dy, pxx[j] + dx, H, W, ci) : 0.0f);
+        }
+        for (int m = 0; m < 4; ++m)
+            for (int t = 0; t < 2; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B[t], acc[m][t], 0, 0, 0);
+    }
+    // stash: element (co_local = 16m + 4*(lane/16) + i, ci_local = 16t + lane%16)
+    for (int m = 0; m < 4; ++m)
+        for (int t = 0; t < 2; ++t)
+            for (int i = 0; i < 4; ++i)
+                red[wave][(16 * m + 4 * (lane >> 4) + i) * 32 + 16 * t + (lane & 15)] = acc[m][t][i];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+        const float s = red[0][e] + red[1][e] + red[2][e] + red[3][e];
+        const int co = co0 + e / 32, ci = ci0 + e % 32;
+        if (co < Cout && s != 0.0f) atomicAdd(&dw[((size_t)tap * Cout + co) * Cin + ci], s);
+    }
+}
+
+__global__ void k_bias_grad(const bf16 *__restrict__ dpre, long long npix, int32_t C, float *__restrict__ db) {
+    // one block per channel group of 64 channels x slice of pixels
+    const int c = blockIdx.y * 64 + (threadIdx.x & 63);
+    const int part = threadIdx.x >> 6;   // 4 pixel phases
+    float s = 0.0f;
+    if (c < C)
+        for (long long p = (long long)blockIdx.x * 4 + part; p < npix; p += (long long)gridDim.x * 4)
+            s += b2f(dpre[p * C + c]);
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < 64 && c < C) {
+        const float t = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
+        if (t != 0.0f) atomicAdd(&db[c], t);
+    }
+}
+
+// dY *= (Y > 0): gradient through ReLU, in place.
+__global__ void k_relu_back(bf16 *__restrict__ d, const bf16 *__restrict__ y, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+        if (!(b2f(y[i]) > 0.0f)) d[i] = f2b(0.0f);
+}
+
+// 2x2 average pooling, NHWC.
+__global__ void k_pool2(const bf16 *__restrict__ x, int32_t N, int32_t H, int32_t W, int32_t C, bf16 *__restrict__ y) {
+    const int Ho = H / 2, Wo = W / 2;
+    const long long n_out = (long long)N * Ho * Wo * C;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n_out; i += (long long)gridDim.x * 256) {
+        const int c = (int)(i % C);
+        const long long p = i / C;
+        const int xo = (int)(p % Wo), yo = (int)((p / Wo) % Ho), n = (int)(p / ((long long)Wo * Ho));
+        const size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
+        const float s = b2f(x[b]) + b2f(x[b + C]) + b2f(x[b + (size_t)W * C]) + b2f(x[b + (size_t)W * C + C]);
+        y[i] = f2b(0.25f * s);
+    }
+}
+// dX += upsample(dY) / 4
+__global__ void k_pool2_back(const bf16 *__restrict__ dy, int32_t N, int32_t H, int32_t W, int32_t C,
+                             bf16 *__restrict__ dx) {
+    const long long n_in = (long long)N * H * W * C;
+    const int Ho = H / 2, Wo = W / 2;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n_in; i += (long long)gridDim.x * 256) {
+        const int c = (int)(i % C);
+        const long long p = i / C;
+        const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
+        float g = 0.0f;
+        if (y / 2 < Ho && x / 2 < Wo) g = 0.25f * b2f(dy[(((size_t)n * Ho + y / 2) * Wo + x / 2) * C + c]);
+        dx[i] = f2b(b2f(dx[i]) + g);
+    }
+}
+// Split the data gradient of a concat(upsample2(low), skip) input:
+// dLow = 2x2 sum of the first C1 channels, dSkip = the rest (both written).
+__global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H, int32_t W, int32_t C1, int32_t C2,
+                             bf16 *__restrict__ dlow, bf16 *__restrict__ dskip) {
+    const int C = C1 + C2;
+    const int Hl = (H + 1) / 2, Wl = (W + 1) / 2;
+    const long long n_low = (long long)N * Hl * Wl * C1;
+    const long long n_skip = (long long)N * H * W * C2;
+    const long long total = n_low + n_skip;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+        if (i < n_low) {
+            const int c = (int)(i % C1);
+            const long long p = i / C1;
+            const int xl = (int)(p % Wl), yl = (int)((p / Wl) % Hl), n = (int)(p / ((long long)Wl * Hl));
+            float s = 0.0f;
+            for (int k = 0; k < 4; ++k) {
+                const int y = 2 * yl + (k >> 1), x = 2 * xl + (k & 1);
+                if (y < H && x < W) s += b2f(dcat[(((size_t)n * H + y) * W + x) * C + c]);
+            }
+            dlow[i] = f2b(s);
+        } else {
+            const long long j = i - n_low;
+            const int c = (int)(j % C2);
+            const long long p = j / C2;
+            dskip[j] = dcat[p * C + C1 + c];
+        }
+    }
+}
+
+// Input features -> padded bf16: log1p radiance, albedo, normal, log1p depth.
+__global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *__restrict__ x) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < npix * kInPadded; i += (long long)gridDim.x * 256) {
+        const long long p = i / kInPadded;
+        const int c = (int)(i % kInPadded);
+        float v = 0.0f;
+        if (c < kInChannels) {
+            v = f[p * kInChannels + c];
+            if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
+        }
+        x[i] = f2b(v);
+    }
+}
+
+// 1x1 head (32 -> 3, FP32) with the L1 loss on log radiance.  Inference
+// writes expm1(prediction) clamped at 0; training writes dL/dX (bf16) and
+// accumulates the head gradients and the loss.
+__global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long long npix, const float *__restrict__ wh,
+                                               const float *__restrict__ bh, const float *__restrict__ target,
+                                               float *__restrict__ out, bf16 *__restrict__ dx, float *__restrict__ dwh,
+                                               float *__restrict__ dbh, double *__restrict__ loss, float inv_count) {
+    __shared__ float sw[kOut * 32 + kOut];
+    __shared__ float sg[256][kOut * 33];
+    __shared__ double sl[256];
+    for (int i = threadIdx.x; i < kOut * 32; i += 256) sw[i] = wh[i];
+    if (threadIdx.x < kOut) sw[kOut * 32 + threadIdx.x] = bh[threadIdx.x];
+    __syncthreads();
+    float gw[kOut * 33];
+    for (int k = 0; k < kOut * 33; ++k) gw[k] = 0.0f;
+    double l = 0.0;
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
+        float xi[32];
+        for (int c = 0; c < 32; ++c) xi[c] = b2f(x[p * 32 + c]);
+        float o[kOut];
+        for (int k = 0; k < kOut; ++k) {
+            float s = sw[kOut * 32 + k];
+            for (int c = 0; c < 32; ++c) s += sw[k * 32 + c] * xi[c];
+            o[k] = s;
+        }
+        if (!target) {
+            for (int k = 0; k < kOut; ++k) out[p * kOut + k] = fmaxf(expm1f(o[k]), 0.0f);
+            continue;
+        }
+        float g[kOut];
+        for (int k = 0; k < kOut; ++k) {
+            const float t = log1pf(fmaxf(target[p * kOut + k], 0.0f));
+            const float d = o[k] - t;
+            l += fabs((double)d);
+            g[k] = (d > 0.0f ? inv_count : (d < 0.0f ? -inv_count : 0.0f));
+            if (out) out[p * kOut + k] = fmaxf(expm1f(o[k]), 0.0f);
+        }
+        for (int c = 0; c < 32; ++c) {
+            float s = 0.0f;
+            for (int k = 0; k < kOut; ++k) {
+                s += sw[k * 32 + c] * g[k];
+                gw[k * 33 + c] += g[k] * xi[c];
+            }
+            dx[p * 32 + c] = f2b(s);
+        }
+        for (int k = 0; k < kOut; ++k) gw[k * 33 + 32] += g[k];
+    }
+    if (!target) return;
+    for (int k = 0; k < kOut * 33; ++k) sg[threadIdx.x][k] = gw[k];
+    sl[threadIdx.x] = l;
+    __syncthreads();
+    if (threadIdx.x < kOut * 33) {
+        float s = 0.0f;
+        for (int t = 0; t < 256; ++t) s += sg[t][threadIdx.x];
+        const int k = threadIdx.x / 33, c = threadIdx.x % 33;
+        if (c < 32) atomicAdd(&dwh[k * 32 + c], s);
+        else atomicAdd(&dbh[k], s);
+    }
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int t = 0; t < 256; ++t) s += sl[t];
+        atomicAdd(loss, s);
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters.
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       long long n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// bf16 copies of one layer: wf[tap][co][ci] and wb[8 - tap][ci][co].
+__global__ void k_pack_weights(const float *__restrict__ w, int32_t Cout, int32_t Cin, bf16 *__restrict__ wf,
+                               bf16 *__restrict__ wb) {
+    const long long n = 9LL * Cout * Cin;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ci = (int)(i % Cin);
+        const int co = (int)((i / Cin) % Cout);
+        const int tap = (int)(i / ((long long)Cin * Cout));
+        const bf16 b = f2b(w[i]);
+        wf[i] = b;
+        wb[((size_t)(8 - tap) * Cin + ci) * Cout + co] = b;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+namespace {
+
+struct ConvLayer {
+    int cin, cout;
+    size_t w_off, b_off;      // offsets of W and b in the flat parameter vector
+    bf16 *wf = nullptr, *wb = nullptr;
+};
+
+enum {
+    L_E0A, L_E0B, L_E1A, L_E1B, L_E2A, L_E2B, L_B0, L_B1,
+    L_D2A, L_D2B, L_D1A, L_D1B, L_D0A, L_D0B, L_COUNT
+};
+
+}  // namespace
+
+struct s3dg_cnn {
+    s3dg_context *ctx = nullptr;
+    ConvLayer layer[L_COUNT];
+    size_t head_w = 0, head_b = 0;
+    size_t nparam = 0;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // FP32 master, grad, Adam moments
+    bf16 *wbuf = nullptr;
+    int step = 0;
+    // activation workspace
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    double *d_loss = nullptr;
+};
+
+namespace {
+
+const int kLayerIO[L_COUNT][2] = {
+    {kInPadded, 32}, {32, 32}, {32, 64}, {64, 64}, {64, 96}, {96, 96}, {96, 128}, {128, 128},
+    {128 + 96, 96}, {96, 96}, {96 + 64, 64}, {64, 64}, {64 + 32, 32}, {32, 32}};
+
+size_t grid1(long long n) { return (size_t)std::max<long long>(1, std::min<long long>((n + 255) / 256, 65536)); }
+
+struct Acts {
+    // forward activations (post-ReLU), bf16 NHWC
+    bf16 *x, *e0a, *e0b, *p0, *e1a, *e1b, *p1, *e2a, *e2b, *p2, *b0, *b1, *d2a, *d2b, *d1a, *d1b, *d0a, *d0b;
+    // gradients
+    bf16 *g_cat, *g_a, *g_b, *g_e0b, *g_e1b, *g_e2b, *g_low;
+};
+
+size_t act_bytes(long long n0) {
+    // element counts at full, 1/4, 1/16, 1/64 resolution
+    const long long n1 = n0 / 4 + 64, n2 = n0 / 16 + 64, n3 = n0 / 64 + 64;
+    long long e = 0;
+    e += n0 * kInPadded + n0 * 32 * 2;           // x, e0a, e0b
+    e += n1 * 32 + n1 * 64 * 2;                  // p0, e1a, e1b
+    e += n2 * 64 + n2 * 96 * 2;                  // p1, e2a, e2b
+    e += n3 * 96 + n3 * 128 * 2;                 // p2, b0, b1
+    e += n2 * 96 * 2 + n1 * 64 * 2 + n0 * 32 * 2;  // decoder
+    // gradients: concat buffer (largest: full res x 96), two layer buffers, three skips, low
+    e += n0 * 224 + 2 * n0 * 96 + n0 * 32 + n1 * 64 + n2 * 96 + n0 * 128;
+    return (size_t)e * 2 + 64 * 256;
+}
+
+Acts carve_acts(void *base, long long n0) {
+    char *p = (char *)base;
+    auto take = [&](long long elems) {
+        bf16 *r = (bf16 *)p;
+        p += ((size_t)elems * 2 + 255) & ~(size_t)255;
+        return r;
+    };
+    const long long n1 = n0 / 4 + 64, n2 = n0 / 16 + 64, n3 = n0 / 64 + 64;
+    Acts a;
+    a.x = take(n0 * kInPadded);
+    a.e0a = take(n0 * 32); a.e0b = take(n0 * 32); a.p0 = take(n1 * 32);
+    a.e1a = take(n1 * 64); a.e1b = take(n1 * 64); a.p1 = take(n2 * 64);
+    a.e2a = take(n2 * 96); a.e2b = take(n2 * 96); a.p2 = take(n3 * 96);
+    a.b0 = take(n3 * 128); a.b1 = take(n3 * 128);
+    a.d2a = take(n2 * 96); a.d2b = take(n2 * 96);
+    a.d1a = take(n1 * 64); a.d1b = take(n1 * 64);
+    a.d0a = take(n0 * 32); a.d0b = take(n0 * 32);
+    a.g_cat = take(n0 * 224); a.g_a = take(n0 * 96); a.g_b = take(n0 * 96);
+    a.g_e0b = take(n0 * 32); a.g_e1b = take(n1 * 64); a.g_e2b = take(n2 * 96);
+    a.g_low = take(n0 * 128);
+    return a;
+}
+
+void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
+          bf16 *out) {
+    hipStream_t s = c->ctx->stream;
+    const int nt = (Cout % 64 == 0) ? 4 : 2;
+    const int chunks = (Cout + 16 * nt - 1) / (16 * nt);
+    dim3 grid((W + 63) / 64, H, N * chunks);
+    if (nt == 4) hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+    else hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+}
+
+void conv_dw(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *dpre, int Cout, float *dw) {
+    const int Cin = I.C1 + I.C2;
+    const long long npix = (long long)N * H * W;
+    const long long nruns = (npix + 31) / 32;
+    const int xy = 9 * (Cin / 32) * ((Cout + 63) / 64);
+    long long slices = std::max<long long>(1, std::min<long long>(4096 / xy + 1, nruns / 4 + 1));
+    const int runs = (int)((nruns + slices - 1) / slices);
+    slices = (nruns + runs - 1) / runs;
+    dim3 grid(9 * (Cin / 32), (Cout + 63) / 64, (unsigned)slices);
+    hipLaunchKernelGGL(k_conv_dw, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, runs, dw);
+}
+
+void bias_grad(s3dg_cnn *c, const bf16 *dpre, long long npix, int C, float *db) {
+    dim3 grid((unsigned)std::max<long long>(1, std::min<long long>(npix / 64 + 1, 1024)), (C + 63) / 64);
+    hipLaunchKernelGGL(k_bias_grad, grid, dim3(256), 0, c->ctx->stream, dpre, npix, C, db);
+}
+
+ConvIn cin1(const bf16 *a, int C) { return ConvIn{a, nullptr, C, 0, 1}; }
+ConvIn cin_cat(const bf16 *low, int C1, const bf16 *skip, int C2) { return ConvIn{low, skip, C1, C2, 2}; }
+
+void pack_weights(s3dg_cnn *c) {
+    for (int l = 0; l < L_COUNT; ++l) {
+        ConvLayer &L = c->layer[l];
+        const long long n = 9LL * L.cout * L.cin;
+        hipLaunchKernelGGL(k_pack_weights, dim3(grid1(n)), dim3(256), 0, c->ctx->stream, c->p + L.w_off, L.cout, L.cin,
+                           L.wf, L.wb);
+    }
+}
+
+// Forward pass; returns the decoder output (d0b) for the head.
+void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
+    hipStream_t s = c->ctx->stream;
+    const long long n0 = (long long)N * H * W;
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    hipLaunchKernelGGL(k_pack_input, dim3(grid1(n0 * kInPadded)), dim3(256), 0, s, feat, n0, a.x);
+    ConvLayer *L = c->layer;
+    auto B = [&](int l) { return c->p + L[l].b_off; };
+    conv(c, cin1(a.x, kInPadded), N, H, W, L[L_E0A].wf, B(L_E0A), 32, true, a.e0a);
+    conv(c, cin1(a.e0a, 32), N, H, W, L[L_E0B].wf, B(L_E0B), 32, true, a.e0b);
+    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H1 * W1 * 32)), dim3(256), 0, s, a.e0b, N, H, W, 32, a.p0);
+    conv(c, cin1(a.p0, 32), N, H1, W1, L[L_E1A].wf, B(L_E1A), 64, true, a.e1a);
+    conv(c, cin1(a.e1a, 64), N, H1, W1, L[L_E1B].wf, B(L_E1B), 64, true, a.e1b);
+    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H2 * W2 * 64)), dim3(256), 0, s, a.e1b, N, H1, W1, 64, a.p1);
+    conv(c, cin1(a.p1, 64), N, H2, W2, L[L_E2A].wf, B(L_E2A), 96, true, a.e2a);
+    conv(c, cin1(a.e2a, 96), N, H2, W2, L[L_E2B].wf, B(L_E2B), 96, true, a.e2b);
+    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H3 * W3 * 96)), dim3(256), 0, s, a.e2b, N, H2, W2, 96, a.p2);
+    conv(c, cin1(a.p2, 96), N, H3, W3, L[L_B0].wf, B(L_B0), 128, true, a.b0);
+    conv(c, cin1(a.b0, 128), N, H3, W3, L[L_B1].wf, B(L_B1), 128, true, a.b1);
+    conv(c, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, L[L_D2A].wf, B(L_D2A), 96, true, a.d2a);
+    conv(c, cin1(a.d2a, 96), N, H2, W2, L[L_D2B].wf, B(L_D2B), 96, true, a.d2b);
+    conv(c, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, L[L_D1A].wf, B(L_D1A), 64, true, a.d1a);
+    conv(c, cin1(a.d1a, 64), N, H1, W1, L[L_D1B].wf, B(L_D1B), 64, true, a.d1b);
+    conv(c, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, L[L_D0A].wf, B(L_D0A), 32, true, a.d0a);
+    conv(c, cin1(a.d0a, 32), N, H, W, L[L_D0B].wf, B(L_D0B), 32, true, a.d0b);
+}
+
+// Backward of one conv layer whose output gradient (w.r.t. its post-ReLU
+// output) is in dy: masks it by the output, accumulates dW and db, and
+// (if dx) writes the input gradient.
+void layer_back(s3dg_cnn *c, int l, const ConvIn &I, int N, int H, int W, bf16 *dy, const bf16 *y, bf16 *dx) {
+    hipStream_t s = c->ctx->stream;
+    ConvLayer &L = c->layer[l];
+    const long long n = (long long)N * H * W;
+    hipLaunchKernelGGL(k_relu_back, dim3(grid1(n * L.cout)), dim3(256), 0, s, dy, y, n * L.cout);
+    conv_dw(c, I, N, H, W, dy, L.cout, c->g + L.w_off);
+    bias_grad(c, dy, n, L.cout, c->g + L.b_off);
+    if (dx) conv(c, cin1(dy, L.cout), N, H, W, L.wb, nullptr, L.cin, false, dx);
+}
+
+void split(s3dg_cnn *c, const bf16 *dcat, int N, int H, int W, int C1, int C2, bf16 *dlow, bf16 *dskip) {
+    const long long total = (long long)N * ((H + 1) / 2) * ((W + 1) / 2) * C1 + (long long)N * H * W * C2;
+    hipLaunchKernelGGL(k_split_grad, dim3(grid1(total)), dim3(256), 0, c->ctx->stream, dcat, N, H, W, C1, C2, dlow,
+                       dskip);
+}
+
+void pool_back(s3dg_cnn *c, const bf16 *dy, int N, int H, int W, int C, bf16 *dx) {
+    hipLaunchKernelGGL(k_pool2_back, dim3(grid1((long long)N * H * W * C)), dim3(256), 0, c->ctx->stream, dy, N, H, W,
+                       C, dx);
+}
+
+// Backward pass.  a.g_a holds d(d0b) from the head on entry.  Skip gradients
+// get the decoder's contribution first (the skip buffers are zeroed before),
+// then the pooling path's.
+void backward(s3dg_cnn *c, Acts &a, int N, int H, int W) {
+    const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
+    layer_back(c, L_D0B, cin1(a.d0a, 32), N, H, W, a.g_a, a.d0b, a.g_b);
+    layer_back(c, L_D0A, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, a.g_b, a.d0a, a.g_cat);
+    split(c, a.g_cat, N, H, W, 64, 32, a.g_low, a.g_e0b);
+    layer_back(c, L_D1B, cin1(a.d1a, 64), N, H1, W1, a.g_low, a.d1b, a.g_a);
+    layer_back(c, L_D1A, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, a.g_a, a.d1a, a.g_cat);
+    split(c, a.g_cat, N, H1, W1, 96, 64, a.g_low, a.g_e1b);
+    layer_back(c, L_D2B, cin1(a.d2a, 96), N, H2, W2, a.g_low, a.d2b, a.g_a);
+    layer_back(c, L_D2A, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, a.g_a, a.d2a, a.g_cat);
+    split(c, a.g_cat, N, H2, W2, 128, 96, a.g_low, a.g_e2b);
+    layer_back(c, L_B1, cin1(a.b0, 128), N, H3, W3, a.g_low, a.b1, a.g_a);
+    layer_back(c, L_B0, cin1(a.p2, 96), N, H3, W3, a.g_a, a.b0, a.g_b);
+    pool_back(c, a.g_b, N, H2, W2, 96, a.g_e2b);
+    layer_back(c, L_E2B, cin1(a.e2a, 96), N, H2, W2, a.g_e2b, a.e2b, a.g_a);
+    layer_back(c, L_E2A, cin1(a.p1, 64), N, H2, W2, a.g_a, a.e2a, a.g_b);
+    pool_back(c, a.g_b, N, H1, W1, 64, a.g_e1b);
+    layer_back(c, L_E1B, cin1(a.e1a, 64), N, H1, W1, a.g_e1b, a.e1b, a.g_a);
+    layer_back(c, L_E1A, cin1(a.p0, 32), N, H1, W1, a.g_a, a.e1a, a.g_b);
+    pool_back(c, a.g_b, N, H, W, 32, a.g_e0b);
+    layer_back(c, L_E0B, cin1(a.e0a, 32), N, H, W, a.g_e0b, a.e0b, a.g_a);
+    layer_back(c, L_E0A, cin1(a.x, kInPadded), N, H, W, a.g_a, a.e0a, nullptr);
+}
+
+int check_dims(s3dg_context *ctx, int n, int h, int w) {
+    if (n <= 0 || h <= 0 || w <= 0 || h % 8 || w % 8)
+        return set_error(ctx, S3DG_ERR_INVALID, "denoiser input: N, H, W > 0 and H, W multiples of 8");
+    if ((long long)n * h * w > (1LL << 27)) return set_error(ctx, S3DG_ERR_UNSUPPORTED, "denoiser batch too large");
+    return S3DG_OK;
+}
+
+int prepare(s3dg_cnn *c, long long n0, Acts &a) {
+    int rc = ensure_buffer(c->ctx, &c->ws, &c->ws_bytes, act_bytes(n0));
+    if (rc) return rc;
+    a = carve_acts(c->ws, n0);
+    return S3DG_OK;
+}
+
+// Loss and gradient of one batch into c->g (zeroed first); returns the mean loss.
+int loss_and_grad(s3dg_cnn *c, int n, int h, int w, const float *d_feat, const float *d_target, double *loss) {
+    s3dg_context *ctx = c->ctx;
+    const long long n0 = (long long)n * h * w;
+    Acts a;
+    int rc = prepare(c, n0, a);
+    if (rc) return rc;
+    hipStream_t s = ctx->stream;
+    S3DG_HIP(ctx, hipMemsetAsync(c->g, 0, c->nparam * sizeof(float), s));
+    S3DG_HIP(ctx, hipMemsetAsync(c->d_loss, 0, sizeof(double), s));
+    forward(c, a, d_feat, n, h, w);
+    const float inv = 1.0f / (float)(n0 * kOut);
+    hipLaunchKernelGGL(k_head, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 2048)), dim3(256), 0, s, a.d0b, n0,
+                       c->p + c->head_w, c->p + c->head_b, d_target, nullptr, a.g_a, c->g + c->head_w,
+                       c->g + c->head_b, c->d_loss, inv);
+    backward(c, a, n, h, w);
+    S3DG_HIP(ctx, hipGetLastError());
+    double l = 0.0;
+    S3DG_HIP(ctx, hipMemcpyAsync(&l, c->d_loss, sizeof(double), hipMemcpyDeviceToHost, s));
+    S3DG_HIP(ctx, hipStreamSynchronize(s));
+    if (loss) *loss = l / (double)(n0 * kOut);
+    return S3DG_OK;
+}
+
+}  // namespace
+
+extern "C" int s3dg_cnn_create(s3dg_context *ctx, uint64_t seed, s3dg_cnn **out) {
+    if (!ctx || !out) return S3DG_ERR_INVALID;
+    *out = nullptr;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    s3dg_cnn *c = new (std::nothrow) s3dg_cnn();
+    if (!c) return set_error(ctx, S3DG_ERR_NOMEM, "host allocation");
+    c->ctx = ctx;
+    size_t off = 0, wbf = 0;
+    for (int l = 0; l < L_COUNT; ++l) {
+        ConvLayer &L = c->layer[l];
+        L.cin = kLayerIO[l][0];
+        L.cout = kLayerIO[l][1];
+        L.w_off = off;
+        off += 9ull * L.cin * L.cout;
+        L.b_off = off;
+        off += L.cout;
+        wbf += 2 * 9ull * L.cin * L.cout;
+    }
+    c->head_w = off;
+    off += kOut * 32;
+    c->head_b = off;
+    off += kOut;
+    c->nparam = off;
+    // He-normal initialisation (fan-in 9 x real input channels), zero biases.
+    std::vector<float> init(off, 0.0f);
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<float> nd(0.0f, 1.0f);
+    for (int l = 0; l < L_COUNT; ++l) {
+        ConvLayer &L = c->layer[l];
+        const int real_in = (l == L_E0A) ? kInChannels : L.cin;
+        const float sd = sqrtf(2.0f / (9.0f * real_in));
+        for (int tap = 0; tap < 9; ++tap)
+            for (int co = 0; co < L.cout; ++co)
+                for (int ci = 0; ci < L.cin; ++ci)
+                    init[L.w_off + ((size_t)tap * L.cout + co) * L.cin + ci] = ci < real_in ? nd(rng) * sd : 0.0f;
+    }
+    for (int i = 0; i < kOut * 32; ++i) init[c->head_w + i] = nd(rng) * sqrtf(1.0f / 32.0f);
+    hipError_t e;
+    if ((e = hipMalloc(&c->p, off * 4)) != hipSuccess || (e = hipMalloc(&c->g, off * 4)) != hipSuccess ||
+        (e = hipMalloc(&c->m, off * 4)) != hipSuccess || (e = hipMalloc(&c->v, off * 4)) != hipSuccess ||
+        (e = hipMalloc(&c->wbuf, wbf * 2)) != hipSuccess || (e = hipMalloc(&c->d_loss, 256)) != hipSuccess) {
+        s3dg_cnn_destroy(c);
+        return hip_check(ctx, e, "hipMalloc(cnn)");
+    }
+    bf16 *wp = c->wbuf;
+    for (int l = 0; l < L_COUNT; ++l) {
+        ConvLayer &L = c->layer[l];
+        L.wf = wp;
+        wp += 9ull * L.cin * L.cout;
+        L.wb = wp;
+        wp += 9ull * L.cin * L.cout;
+    }
+    S3DG_HIP(ctx, hipMemcpy(c->p, init.data(), off * 4, hipMemcpyHostToDevice));
+    S3DG_HIP(ctx, hipMemset(c->m, 0, off * 4));
+    S3DG_HIP(ctx, hipMemset(c->v, 0, off * 4));
+    pack_weights(c);
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = c;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_destroy(s3dg_cnn *c) {
+    if (!c) return S3DG_ERR_INVALID;
+    void *ptrs[] = {c->p, c->g, c->m, c->v, c->wbuf, c->ws, c->d_loss};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete c;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_param_count(const s3dg_cnn *c, uint64_t *n) {
+    if (!c || !n) return S3DG_ERR_INVALID;
+    *n = c->nparam;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_get_params(s3dg_cnn *c, float *host, uint64_t n) {
+    if (!c || !host || n != c->nparam) return S3DG_ERR_INVALID;
+    S3DG_HIP(c->ctx, hipMemcpyAsync(host, c->p, n * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+    S3DG_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_set_params(s3dg_cnn *c, const float *host, uint64_t n) {
+    if (!c || !host || n != c->nparam) return S3DG_ERR_INVALID;
+    S3DG_HIP(c->ctx, hipMemcpyAsync(c->p, host, n * 4, hipMemcpyHostToDevice, c->ctx->stream));
+    pack_weights(c);
+    S3DG_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_infer_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t w, const float *d_feat,
+                                     float *d_out) {
+    if (!c || !d_feat || !d_out) return S3DG_ERR_INVALID;
+    int rc = check_dims(c->ctx, n, h, w);
+    if (rc) return rc;
+    S3DG_HIP(c->ctx, hipSetDevice(c->ctx->device));
+    const long long n0 = (long long)n * h * w;
+    Acts a;
+    rc = prepare(c, n0, a);
+    if (rc) return rc;
+    forward(c, a, d_feat, n, h, w);
+    hipLaunchKernelGGL(k_head, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 2048)), dim3(256), 0,
+                       c->ctx->stream, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, nullptr, d_out, nullptr, nullptr,
+                       nullptr, nullptr, 0.0f);
+    S3DG_HIP(c->ctx, hipGetLastError());
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_gradient_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t w, const float *d_feat,
+                                        const float *d_target, double *loss, float *host_grad, uint64_t ngrad) {
+    if (!c || !d_feat || !d_target) return S3DG_ERR_INVALID;
+    if (host_grad && ngrad != c->nparam) return S3DG_ERR_INVALID;
+    int rc = check_dims(c->ctx, n, h, w);
+    if (rc) return rc;
+    S3DG_HIP(c->ctx, hipSetDevice(c->ctx->device));
+    rc = loss_and_grad(c, n, h, w, d_feat, d_target, loss);
+    if (rc) return rc;
+    if (host_grad) {
+        S3DG_HIP(c->ctx, hipMemcpyAsync(host_grad, c->g, ngrad * 4, hipMemcpyDeviceToHost, c->ctx->stream));
+        S3DG_HIP(c->ctx, hipStreamSynchronize(c->ctx->stream));
+    }
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_cnn_train_step_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t w, const float *d_feat,
+                                          const float *d_target, float lr, double *loss) {
+    if (!c || !d_feat || !d_target) return S3DG_ERR_INVALID;
+    int rc = check_dims(c->ctx, n, h, w);
+    if (rc) return rc;
+    S3DG_HIP(c->ctx, hipSetDevice(c->ctx->device));
+    rc = loss_and_grad(c, n, h, w, d_feat, d_target, loss);
+    if (rc) return rc;
+    c->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.0f - powf(b1, (float)c->step), c2 = 1.0f - powf(b2, (float)c->step);
+    hipLaunchKernelGGL(k_adam, dim3(grid1((long long)c->nparam)), dim3(256), 0, c->ctx->stream, c->p, c->g, c->m, c->v,
+                       (long long)c->nparam, lr, b1, b2, 1e-8f, c1, c2);
+    pack_weights(c);
+    S3DG_HIP(c->ctx, hipGetLastError());
+    return S3DG_OK;
+}

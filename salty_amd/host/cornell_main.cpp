@@ -3,7 +3,8 @@
 // materials from MaterialFactory, a Mesh from vertex triples, a quad light,
 // a pinhole camera, PathTracer::Run with a pass budget, then Capture.
 //
-// usage: s3d_cornell <width> <height> <passes> <out.ppm> [sum.bin]
+// usage: s3d_cornell <width> <height> <passes> <out.ppm> [sum.bin [geometry.bin]]
+// geometry.bin: per mesh a u32 vertex count and the vertices, then the camera fov.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -67,8 +68,9 @@ int main(int argc, char **argv) {
         scene.AddShape(s);
     }
     scene.AddQuadLight(shapes[3], 0, 2, Vector3(lo, y, lo), Vector3(0.25f, 0, 0), Vector3(0, 0, 0.25f));
-    scene.GetCamera().Update(Vector3(0.5f, 0.5f, 2.4f), Vector3(0.5f, 0.5f, 0.0f), Vector3(0, 1, 0),
-                             2.0f * std::tan(20.0f * 3.14159265358979f / 180.0f), (f32)W / (f32)H, 1.0f);
+    const f32 fov = 2.0f * std::tan(20.0f * 3.14159265358979f / 180.0f);   // 40 degrees vertical
+    scene.GetCamera().Update(Vector3(0.5f, 0.5f, 2.4f), Vector3(0.5f, 0.5f, 0.0f), Vector3(0, 1, 0), fov,
+                             (f32)W / (f32)H, 1.0f);
 
     PathTracer pt;
     PathTracer::Config cfg;
@@ -86,6 +88,19 @@ int main(int argc, char **argv) {
         FILE *f = fopen(argv[5], "wb");
         if (!f) return 1;
         fwrite(pt.GetSum().data(), sizeof(Color3), pt.GetSum().size(), f);
+        fclose(f);
+    }
+    if (argc > 6) {
+        // geometry as built here: per mesh a u32 vertex count then the vertices
+        FILE *f = fopen(argv[6], "wb");
+        if (!f) return 1;
+        const std::vector<Vertex> *meshes[4] = {&vw, &vr, &vg, &vl};
+        for (const std::vector<Vertex> *m : meshes) {
+            const u32 n = (u32)m->size();
+            fwrite(&n, sizeof(n), 1, f);
+            fwrite(m->data(), sizeof(Vertex), n, f);
+        }
+        fwrite(&fov, sizeof(fov), 1, f);
         fclose(f);
     }
     printf("passes=%d rays=%llu seconds=%.3f\n", pt.GetPassCount(), (unsigned long long)pt.GetRayCount(),

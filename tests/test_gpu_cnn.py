@@ -1,0 +1,79 @@
+"""Denoiser kernels (BF16 MFMA forward / backward-data / backward-weight, FP32
+accumulation) against the float64 NumPy restatement in oracle/unet.py."""
+import numpy as np
+import pytest
+
+from oracle import unet
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(x):
+    """Round-to-nearest-even to bfloat16, returned as float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def oracle_params(p):
+    """The parameters the GPU computes with: bf16 convolution weights, FP32
+    biases and head."""
+    q = p.astype(np.float64).copy()
+    lay, _ = unet.param_layout()
+    for name, cin, cout in unet.LAYERS:
+        wo, _, _, _ = lay[name]
+        q[wo:wo + 9 * cin * cout] = bf16_round(p[wo:wo + 9 * cin * cout])
+    return q
+
+
+def case(seed, n=1, h=32, w=32):
+    rng = np.random.default_rng(seed)
+    feat = np.concatenate([np.abs(rng.gamma(1.0, 0.6, size=(n, h, w, 3))), rng.uniform(0.2, 0.9, (n, h, w, 3)),
+                           rng.normal(size=(n, h, w, 3)), rng.uniform(0.5, 5.0, (n, h, w, 1))], -1)
+    target = np.abs(feat[..., :3] * 0.8 + rng.normal(scale=0.05, size=(n, h, w, 3)))
+    return feat.astype(np.float32), target.astype(np.float32)
+
+
+def test_gradients_match_float64_restatement(ctx):
+    import salty_amd
+    net = salty_amd.Denoiser(ctx, seed=3)
+    feat, target = case(0)
+    p = net.get_params()
+    loss, g = net.gradient(feat, target)
+    ref_loss, ref_g, _ = unet.loss_and_grad(oracle_params(p), feat.astype(np.float64), target.astype(np.float64))
+    assert abs(loss - ref_loss) <= 0.02 * ref_loss, (loss, ref_loss)
+    lay, _ = unet.param_layout()
+    for name, cin, cout in unet.LAYERS:
+        wo, bo, _, _ = lay[name]
+        for lo, hi, what in ((wo, wo + 9 * cin * cout, "W"), (bo, bo + cout, "b")):
+            a, b = g[lo:hi].astype(np.float64), ref_g[lo:hi]
+            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
+            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            assert cos > 0.98 and rel < 0.2, (name, what, cos, rel)
+    a = g[lay["head_w"]:].astype(np.float64)
+    b = ref_g[lay["head_w"]:]
+    assert np.linalg.norm(a - b) <= 0.05 * np.linalg.norm(b)
+
+
+def test_inference_matches_float64_forward(ctx):
+    import salty_amd
+    net = salty_amd.Denoiser(ctx, seed=4)
+    feat, target = case(1, n=2, h=16, w=24)
+    out = net.infer(feat)
+    _, _, pred_log = unet.loss_and_grad(oracle_params(net.get_params()), feat.astype(np.float64),
+                                        target.astype(np.float64))
+    ref = np.maximum(np.expm1(pred_log), 0)
+    err = np.abs(out - ref) / (np.abs(ref) + 0.05)
+    assert np.median(err) < 0.02 and np.percentile(err, 99) < 0.15
+
+
+def test_training_reduces_the_loss(ctx):
+    import salty_amd
+    net = salty_amd.Denoiser(ctx, seed=5)
+    feat, target = case(2, n=4, h=32, w=32)
+    d_f, d_t = ctx.upload(feat), ctx.upload(target)
+    losses = [net.train_step_device(4, 32, 32, d_f, d_t, 2e-3) for _ in range(60)]
+    ctx.free(d_f)
+    ctx.free(d_t)
+    assert np.all(np.isfinite(losses))
+    assert np.mean(losses[-5:]) < 0.5 * np.mean(losses[:3]), losses[::10]
