@@ -54,29 +54,29 @@ def cpu_baseline(desc, args):
     sc = O.OracleScene(desc)
     W, H = args.width, args.height
     mid = H // 2
-    rows = 1
-    total_t, total_rays, total_paths, sample_rows = 0.0, 0, 0, 0
-    while True:
-        r0 = mid - rows // 2
-        t = time.perf_counter()
-        _, counts = sc.render(W, H, 1, args.depth, seed=args.seed, rows=(r0, r0 + rows), threads=threads)
-        dt = time.perf_counter() - t
-        total_t += dt
-        total_rays += int(counts[0] + counts[1])
-        total_paths += int(counts[2])
-        sample_rows += rows
-        if total_t >= args.cpu_seconds or sample_rows >= H:
-            break
-        # grow the band towards the target duration
-        rows = int(min(H - sample_rows, max(1, rows * min(8.0, (args.cpu_seconds - total_t) / max(dt, 1e-3)))))
+    # probe 8 centre rows at 1 spp, then size the timed sample to the target
+    t = time.perf_counter()
+    sc.render(W, H, 1, args.depth, seed=args.seed, rows=(mid - 4, mid + 4), threads=threads)
+    per_row = max((time.perf_counter() - t) / 8, 1e-6)
+    rows_needed = args.cpu_seconds / per_row
+    if rows_needed < H:
+        rows, spp = max(8, int(rows_needed)), 1
+    else:
+        rows, spp = H, max(1, int(round(rows_needed / H)))
+    r0 = mid - rows // 2
+    t = time.perf_counter()
+    _, counts = sc.render(W, H, spp, args.depth, seed=args.seed, rows=(r0, r0 + rows), threads=threads)
+    total_t = time.perf_counter() - t
+    total_rays = int(counts[0] + counts[1])
+    total_paths = int(counts[2])
     return {
         "value": total_rays / total_t / 1e9,
         "unit": "Grays/s",
         "paths_per_s": total_paths / total_t,
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample_rows} rows around the centre of the {W}x{H} frame, 1 spp, depth {args.depth} "
-                  f"({total_paths} paths, {total_rays} rays, {total_t:.1f} s)",
+        "sample": f"rows {r0}-{r0 + rows - 1} of the {W}x{H} frame, {spp} spp, depth {args.depth} "
+                  f"({total_paths} paths, {total_rays} rays, {total_t:.1f} s on {threads} threads)",
     }
 
 

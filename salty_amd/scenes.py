@@ -98,6 +98,53 @@ def cornell_box() -> SceneDesc:
     return desc
 
 
+def cornell_variant(seed: int) -> SceneDesc:
+    """A seeded Cornell-box-style scene (the denoiser's training and held-out
+    scenes): random wall colours, box sizes, positions and rotations, light
+    size and position; Lambert materials, 36 triangles, radiance-15 quad light.
+    Seed 0 is the canonical box of cornell_box()."""
+    if seed == 0:
+        return cornell_box()
+    rng = np.random.default_rng(seed)
+    P = lambda *x: np.array(x, dtype=np.float64)  # noqa: E731
+    tris, mats = [], []
+    tris += _quad(P(0, 0, 0), P(0, 0, 1), P(1, 0, 1), P(1, 0, 0)); mats += [0] * 2
+    tris += _quad(P(0, 1, 0), P(1, 1, 0), P(1, 1, 1), P(0, 1, 1)); mats += [0] * 2
+    tris += _quad(P(0, 0, 0), P(1, 0, 0), P(1, 1, 0), P(0, 1, 0)); mats += [0] * 2
+    tris += _quad(P(0, 0, 0), P(0, 1, 0), P(0, 1, 1), P(0, 0, 1)); mats += [1] * 2
+    tris += _quad(P(1, 0, 0), P(1, 0, 1), P(1, 1, 1), P(1, 1, 0)); mats += [2] * 2
+    for k in range(2):
+        sx, sz = rng.uniform(0.15, 0.35, 2)
+        sy = rng.uniform(0.15, 0.7)
+        cx = rng.uniform(0.2, 0.8)
+        cz = rng.uniform(0.2, 0.8)
+        tris += _box_tris((sx, sy, sz), (cx, cz), rng.uniform(-45, 45))
+        mats += [3 + k] * 12
+    y = 0.999
+    half = rng.uniform(0.06, 0.2)
+    lx, lz = rng.uniform(0.25, 0.75, 2)
+    lo_x, hi_x, lo_z, hi_z = lx - half, lx + half, lz - half, lz + half
+    tris += _quad(P(lo_x, y, lo_z), P(hi_x, y, lo_z), P(hi_x, y, hi_z), P(lo_x, y, hi_z)); mats += [5] * 2
+    p = np.array(tris, dtype=np.float32)
+    col = lambda: tuple(float(c) for c in rng.uniform(0.1, 0.85, 3))  # noqa: E731
+    materials = [dict(type=MAT_LAMBERT, diffuse=tuple(float(c) for c in rng.uniform(0.6, 0.85, 1).repeat(3))),
+                 dict(type=MAT_LAMBERT, diffuse=col()), dict(type=MAT_LAMBERT, diffuse=col()),
+                 dict(type=MAT_LAMBERT, diffuse=col()), dict(type=MAT_LAMBERT, diffuse=col()),
+                 dict(type=MAT_LAMBERT, diffuse=(0.0, 0.0, 0.0), emissive=(15.0, 15.0, 15.0))]
+    desc = SceneDesc()
+    desc.meshes = [Mesh(_tri_vertices(p), triangle_material=np.array(mats, dtype=np.int32))]
+    desc.shapes = [Shape(SHAPE_MESH, mesh=0)]
+    desc.materials = materials
+    desc.lights = [Light(LIGHT_QUAD, shape=0, first_triangle=34, triangle_count=2,
+                         corner=(float(np.float32(lo_x)), float(np.float32(y)), float(np.float32(lo_z))),
+                         edge1=(float(np.float32(hi_x) - np.float32(lo_x)), 0.0, 0.0),
+                         edge2=(0.0, 0.0, float(np.float32(hi_z) - np.float32(lo_z))))]
+    j = rng.normal(scale=0.05, size=2)
+    desc.camera = Camera(position=(0.5 + j[0], 0.5 + j[1], 2.4), target=(0.5, 0.5, 0.0), upward=(0.0, 1.0, 0.0),
+                         fov=FOV_40, aspect=1.0, near_clip=1.0)
+    return desc
+
+
 # ---------------------------------------------------------------------------
 # Icospheres and value noise
 # ---------------------------------------------------------------------------
