@@ -42,7 +42,22 @@ def parse():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a plain device copy of MIB MiB (reference point for HBM byte counters) and exit")
     return p.parse_args()
+
+
+def copy_calibration(mib: int) -> None:
+    """One device-to-device copy kernel of a known size: FETCH_SIZE / WRITE_SIZE
+    of this dispatch against its mib MiB give the counters' scale."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "checksum": float(b[::4096].sum().item())}), flush=True)
 
 
 def cpu_baseline(desc, args):
@@ -82,6 +97,9 @@ def cpu_baseline(desc, args):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path shards by pixel tiles x "
               f"sample ranges with a replicated BVH, which this build does not launch.", file=sys.stderr)

@@ -183,7 +183,7 @@ typedef struct s3dg_render_config {
     int32_t  rr_start_depth;   /* dice honoured from this depth on (0 = reference) */
     int32_t  first_pass;       /* pass index of the first sample (progressive) */
     uint32_t flags;            /* S3DG_RENDER_* instrumentation bits           */
-    uint32_t reserved;
+    uint32_t passes_in_flight; /* passes traced together per wavefront (0 = auto) */
 } s3dg_render_config;
 
 /* flags: time every traversal launch with HIP events on the context stream;

@@ -230,11 +230,11 @@ class Scene:
                        "s3dg_trace_counted_device")
 
     def render(self, width: int, height: int, spp: int, max_depth: int, seed: int = 0, rr_start_depth: int = 0,
-               first_pass: int = 0, accum: np.ndarray | None = None, flags: int = 0):
+               first_pass: int = 0, accum: np.ndarray | None = None, flags: int = 0, passes_in_flight: int = 0):
         """Adds spp passes to accum (HxWx3 sum buffer) and returns (accum, stats)."""
         if accum is None:
             accum = np.zeros((height, width, 3), dtype=np.float32)
-        cfg = RenderConfig(width, height, max_depth, spp, seed, rr_start_depth, first_pass, flags, 0)
+        cfg = RenderConfig(width, height, max_depth, spp, seed, rr_start_depth, first_pass, flags, passes_in_flight)
         st = RenderStats()
         self.ctx.check(self.ctx.lib.s3dg_render(self.handle, C.byref(cfg), _ptr(accum), C.byref(st)),
                        "s3dg_render")

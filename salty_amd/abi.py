@@ -73,7 +73,7 @@ class SceneDescC(C.Structure):
 class RenderConfig(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_bounce_count", C.c_int32), ("spp", C.c_int32),
                 ("seed", C.c_uint64), ("rr_start_depth", C.c_int32), ("first_pass", C.c_int32),
-                ("flags", C.c_uint32), ("reserved", C.c_uint32)]
+                ("flags", C.c_uint32), ("passes_in_flight", C.c_uint32)]
 
 
 RENDER_TIME_KERNELS = 1

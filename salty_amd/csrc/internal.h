@@ -18,6 +18,10 @@ struct s3dg_context {
     // small reduction buffer (tonemap partial sums)
     void *small = nullptr;
     size_t small_bytes = 0;
+    // traversal kernels hand new rays to a wave once this many of its lanes are idle
+    uint32_t refill = 32;
+    // a traversal step keeps testing triangles while this many lanes have some pending
+    uint32_t tri_batch = 16;
 };
 
 namespace s3dg {
@@ -37,6 +41,7 @@ struct SceneDev {
     int32_t light_count;
     float env[3];
     int32_t stack_need;     // deepest stack the trees can require
+    uint32_t tri_batch;     // lanes with pending triangles that keep a step testing
     GCamera cam;
 };
 

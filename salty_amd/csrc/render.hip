@@ -3,11 +3,15 @@
 // kernels over device-resident ray queues.
 //
 // One pass adds one pixel-centre sample per pixel (like one iteration of the
-// reference's progressive loop).  Per bounce: extend (closest hit of the
-// queued rays), shade (emission, light sample, material sample, Russian
-// roulette, compaction into the next queue and the shadow queue) and shadow
-// (visibility of the light sample, then the NEE term).  The order in which a
-// path adds emission, NEE and miss radiance to L is the reference's.
+// reference's progressive loop).  Several passes are traced together: path
+// slot p holds pixel p % npix of pass p / npix, so the deep bounces, where
+// Russian roulette has thinned the queues, still fill the GPU.  Per bounce:
+// extend (closest hit of the queued rays), shade (emission, light sample,
+// material sample, Russian roulette, compaction into the next queue and the
+// shadow queue) and shadow (visibility of the light sample, then the NEE term).
+// The order in which a path adds emission, NEE and miss radiance to L is the
+// reference's, and the passes are added to the accumulator in pass order, so
+// the sum is the same whatever the number of passes in flight.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +29,7 @@ namespace s3dg {
 uint32_t batch_grid(uint32_t n);
 int ensure_gstack(s3dg_scene *sc, uint32_t threads);
 int check_error_flag(s3dg_scene *sc);
+int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks);
 }  // namespace s3dg
 
 namespace {
@@ -44,7 +49,8 @@ struct PathBuf {
     int32_t *slight;                        // light index
     uint32_t *q[2];                         // extension queues (ping-pong)
     uint32_t *qs;                           // shadow queue
-    uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count
+    uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count,
+                                            // [4],[5] extend / shadow work-fetch cursors
     unsigned long long *tot;                // [0] extension rays, [1] shadow rays
 };
 
@@ -78,6 +84,11 @@ PathBuf carve(void *base, uint32_t n) {
 }
 size_t carve_bytes(uint32_t n) { return path_bytes(n) + 40 * 256; }
 
+// Paths traced together when the caller leaves passes_in_flight at 0 (about
+// 2.3 GB of path state), and the most ever held at once.
+constexpr uint32_t kAutoPathsInFlight = 1u << 24;
+constexpr uint32_t kMaxPathsInFlight = 1u << 27;
+
 // Wave-aggregated queue append: one atomic per wavefront.
 __device__ __forceinline__ uint32_t queue_slot(uint32_t *counter, bool want) {
     const uint64_t mask = __ballot(want);
@@ -91,19 +102,31 @@ __device__ __forceinline__ uint32_t queue_slot(uint32_t *counter, bool want) {
     return base + rank;
 }
 
+// Persistent-kernel work fetch: lane 0 of the wave takes the next 64 queue
+// entries; the wave leaves its loop once the cursor has passed the queue end.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *cursor) {
+    uint32_t base = 0;
+    if (__lane_id() == 0) base = atomicAdd(cursor, 64u);
+    return __shfl(base, 0);
+}
+
 __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, int32_t W, int32_t H, uint64_t seed,
-                                                    uint32_t pass, uint32_t scene_id) {
-    const uint32_t n = (uint32_t)W * (uint32_t)H;
+                                                    uint32_t pass0, uint32_t npaths, uint32_t scene_id) {
+    const uint32_t npix = (uint32_t)W * (uint32_t)H;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gtid == 0) {
-        P.cnt[0] = n;
+        P.cnt[0] = npaths;
         P.cnt[1] = 0;
         P.cnt[2] = 0;
+        P.cnt[4] = 0;
+        P.cnt[5] = 0;
     }
-    for (uint32_t p = gtid; p < n; p += gridDim.x * blockDim.x) {
-        const uint32_t x = p % (uint32_t)W, y = p / (uint32_t)W;
+    for (uint32_t p = gtid; p < npaths; p += gridDim.x * blockDim.x) {
+        const uint32_t pix = p % npix;
+        const uint32_t pass = pass0 + p / npix;
+        const uint32_t x = pix % (uint32_t)W, y = pix / (uint32_t)W;
         Pcg rng;
-        rng.seed(path_seed(seed, p, pass, scene_id));
+        rng.seed(path_seed(seed, pix, pass, scene_id));
         // TracePath pixel-centre coordinates (s3d_pt.cpp:326-328)
         const float fx = (0.5f + (float)x) / (float)W - 0.5f;
         const float fy = (0.5f + (float)y) / (float)H - 0.5f;
@@ -120,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, int32_
 
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap,
-                                                    int32_t *err) {
+                                                    int32_t *err, uint32_t refill) {
     __shared__ uint2 lds[kLdsStack * kBlock];
     const uint32_t n = P.cnt[cur];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -128,29 +151,34 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_
     if (gtid == 0) {
         P.cnt[cur ^ 1] = 0;
         P.cnt[2] = 0;
+        P.cnt[5] = 0;   // the shadow launch of this bounce fetches from zero
         P.tot[0] += n;
     }
     Stack st;
-    st.lds = lds + threadIdx.x;
-    st.gst = gstack + gtid;
-    st.gstride = stride;
-    st.gcap = gcap;
-    st.err = err;
+    st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     const uint32_t *q = P.q[cur];
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const uint32_t p = q[i];
-        DRay r;
-        r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
-        r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
-        DHit h = no_hit();
-        st.sp = 0;
-        traverse<false, COUNT>(S, r, kRefHitMin, kRefHitMax, h, st, P.tot + 2);
-        P.ht[p] = h.t;
-        P.hu[p] = h.u;
-        P.hv[p] = h.v;
-        P.hprim[p] = h.prim;
-        P.hshape[p] = h.shape;
-    }
+    Tracer<false, COUNT> tr;
+    uint32_t p = 0;
+    refill_loop(
+        &P.cnt[4], n, refill,
+        [&](uint32_t i) {
+            p = q[i];
+            DRay r;
+            r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
+            r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
+            st.sp = 0;
+            tr.start(S, r, kRefHitMin, kRefHitMax);
+            return true;
+        },
+        [&]() { return tr.step(S, st); },
+        [&]() {
+            P.ht[p] = tr.best.t;
+            P.hu[p] = tr.best.u;
+            P.hv[p] = tr.best.v;
+            P.hprim[p] = tr.best.prim;
+            P.hshape[p] = tr.best.shape;
+            tr.count(P.tot + 2);
+        });
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t cur, int32_t depth,
@@ -159,6 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t *q = P.q[cur];
+    if (gtid == 0) P.cnt[4] = 0;   // the next extend launch fetches from zero
     const uint32_t trips = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < trips; ++it) {
         const uint32_t i = gtid + it * stride;
@@ -248,76 +277,96 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
 
 template <bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap,
-                                                    int32_t *err) {
+                                                    int32_t *err, uint32_t refill) {
     __shared__ uint2 lds[kLdsStack * kBlock];
     const uint32_t n = P.cnt[2];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gtid == 0) P.tot[1] += n;
     Stack st;
-    st.lds = lds + threadIdx.x;
-    st.gst = gstack + gtid;
-    st.gstride = stride;
-    st.gcap = gcap;
-    st.err = err;
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const uint32_t p = P.qs[i];
-        const GLight &lt = S.lights[P.slight[p]];
-        DRay r;
-        r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
-        r.dx = P.sx[p]; r.dy = P.sy[p]; r.dz = P.sz[p];
-        // Distance to the sampled light along the shadow ray, with the
-        // reference's own test of that shape.
-        float tl = -1.0f, lu = 0.0f, lv = 0.0f;
-        int32_t ltri = -1;
-        int32_t lmat;
-        const GInst *lsph = nullptr;
-        if (lt.type == LIGHT_SPHERE) {
-            lsph = &S.insts[lt.tri[0]];
-            tl = sphere_test(lsph->center[0], lsph->center[1], lsph->center[2], lsph->radius, r, kRefHitMin);
-            lmat = lsph->material;
-        } else {
-            float best = kFMaxF;
-            for (int k = 0; k < lt.count; ++k) {
-                float t, u, v;
-                if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) && t < best) {
-                    best = t; lu = u; lv = v; ltri = lt.tri[k];
+    st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
+    Tracer<true, COUNT> tr;
+    uint32_t p = 0;
+    float tl = 0.0f, lu = 0.0f, lv = 0.0f;
+    int32_t ltri = -1;
+    refill_loop(
+        &P.cnt[5], n, refill,
+        [&](uint32_t i) {
+            p = P.qs[i];
+            const GLight &lt = S.lights[P.slight[p]];
+            DRay r;
+            r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
+            r.dx = P.sx[p]; r.dy = P.sy[p]; r.dz = P.sz[p];
+            // Distance to the sampled light along the shadow ray, with the
+            // reference's own test of that shape.
+            tl = -1.0f;
+            ltri = -1;
+            if (lt.type == LIGHT_SPHERE) {
+                const GInst &ls = S.insts[lt.tri[0]];
+                tl = sphere_test(ls.center[0], ls.center[1], ls.center[2], ls.radius, r, kRefHitMin);
+            } else {
+                float best = kFMaxF;
+                for (int k = 0; k < lt.count; ++k) {
+                    float t, u, v;
+                    if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) && t < best) {
+                        best = t; lu = u; lv = v; ltri = lt.tri[k];
+                    }
                 }
+                if (ltri >= 0) tl = best;
             }
-            if (ltri >= 0) tl = best;
-            lmat = ltri >= 0 ? S.shade[ltri].material : 0;
-        }
-        if (!(tl >= 0.0f)) continue;
-        // "closest hit is the light" == no other hit strictly before it
-        DHit occ = no_hit();
-        st.sp = 0;
-        traverse<true, COUNT>(S, r, kRefHitMin, nextafterf(tl, 0.0f), occ, st, P.tot + 4);
-        if (occ.prim != -1) continue;
-        // NextEventEstimation contribution (s3d_pt.cpp:274-289)
-        const V3 o = v3(r.ox, r.oy, r.oz), ld = v3(r.dx, r.dy, r.dz);
-        const V3 ph = o + ld * tl;
-        const V3 ln = (lt.type == LIGHT_SPHERE) ? unit3(ph - ld3(lsph->center)) : tri_normal(S.shade[ltri], lu, lv);
-        const float cos_shadow = P.scos[p];
-        const float cos_light = fabsf(dot3(ln, -ld));
-        const float d2 = P.sd2[p];
-        const V3 fs = v3(P.sfr[p], P.sfg[p], P.sfb[p]);
-        const float G = (cos_shadow * cos_light) / d2;
-        const float brdf_pdf = fabsf(ld.z / kPi) * cos_light / d2;
-        const float mis = lt.pdf / (brdf_pdf + lt.pdf);
-        const V3 nee = ld3(S.mats[lmat].emissive) * fs * mis * (G / lt.pdf);
-        const V3 W = v3(P.swr[p], P.swg[p], P.swb[p]);
-        const V3 c = W * nee;
-        P.lr[p] = P.lr[p] + c.x;
-        P.lg[p] = P.lg[p] + c.y;
-        P.lb[p] = P.lb[p] + c.z;
-    }
+            if (!(tl >= 0.0f)) return false;
+            // "closest hit is the light" == no other hit strictly before it
+            st.sp = 0;
+            tr.start(S, r, kRefHitMin, nextafterf(tl, 0.0f));
+            return true;
+        },
+        [&]() { return tr.step(S, st); },
+        [&]() {
+            tr.count(P.tot + 4);
+            if (tr.best.prim != -1) return;
+            // NextEventEstimation contribution (s3d_pt.cpp:274-289)
+            const GLight &lt = S.lights[P.slight[p]];
+            const V3 o = v3(tr.wray.ox, tr.wray.oy, tr.wray.oz), ld = v3(tr.wray.dx, tr.wray.dy, tr.wray.dz);
+            const V3 ph = o + ld * tl;
+            V3 ln;
+            int32_t lmat;
+            if (lt.type == LIGHT_SPHERE) {
+                const GInst &ls = S.insts[lt.tri[0]];
+                ln = unit3(ph - ld3(ls.center));
+                lmat = ls.material;
+            } else {
+                ln = tri_normal(S.shade[ltri], lu, lv);
+                lmat = S.shade[ltri].material;
+            }
+            const float cos_shadow = P.scos[p];
+            const float cos_light = fabsf(dot3(ln, -ld));
+            const float d2 = P.sd2[p];
+            const V3 fs = v3(P.sfr[p], P.sfg[p], P.sfb[p]);
+            const float G = (cos_shadow * cos_light) / d2;
+            const float brdf_pdf = fabsf(ld.z / kPi) * cos_light / d2;
+            const float mis = lt.pdf / (brdf_pdf + lt.pdf);
+            const V3 nee = ld3(S.mats[lmat].emissive) * fs * mis * (G / lt.pdf);
+            const V3 W = v3(P.swr[p], P.swg[p], P.swb[p]);
+            const V3 c = W * nee;
+            P.lr[p] = P.lr[p] + c.x;
+            P.lg[p] = P.lg[p] + c.y;
+            P.lb[p] = P.lb[p] + c.z;
+        });
 }
 
-__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, float *accum, uint32_t n) {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        accum[3 * p + 0] = accum[3 * p + 0] + P.lr[p];
-        accum[3 * p + 1] = accum[3 * p + 1] + P.lg[p];
-        accum[3 * p + 2] = accum[3 * p + 2] + P.lb[p];
+// m_RenderTarget[idx] += L for each pass in turn (s3d_pt.cpp:330-332).
+__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, float *accum, uint32_t npix, uint32_t passes) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        float r = accum[3 * p + 0], g = accum[3 * p + 1], b = accum[3 * p + 2];
+        for (uint32_t s = 0; s < passes; ++s) {
+            const size_t q = (size_t)s * npix + p;
+            r = r + P.lr[q];
+            g = g + P.lg[q];
+            b = b + P.lb[q];
+        }
+        accum[3 * p + 0] = r;
+        accum[3 * p + 1] = g;
+        accum[3 * p + 2] = b;
     }
 }
 
@@ -334,11 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = lds + threadIdx.x;
-    st.gst = gstack + gtid;
-    st.gstride = stride;
-    st.gcap = gcap;
-    st.err = err;
+    st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     for (uint32_t p = gtid; p < n; p += stride) {
         const uint32_t x = p % (uint32_t)W, y = p / (uint32_t)W;
         Pcg rng;
@@ -348,9 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
         DRay r;
         r.ox = o.x; r.oy = o.y; r.oz = o.z;
         r.dx = d.x; r.dy = d.y; r.dz = d.z;
-        DHit h = no_hit();
-        st.sp = 0;
-        traverse<false, false>(S, r, kRefHitMin, kRefHitMax, h, st, nullptr);
+        const DHit h = traverse<false, false>(S, r, kRefHitMin, kRefHitMax, st, nullptr);
         V3 a = v3(0, 0, 0), nn = v3(0, 0, 0);
         float t = 0.0f;
         if (h.prim != -1) {
@@ -386,18 +429,27 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
         return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
     S3DG_HIP(ctx, hipSetDevice(ctx->device));
     const uint32_t n = (uint32_t)cfg->width * (uint32_t)cfg->height;
-    int rc = ensure_buffer(ctx, &sc->d_render, &sc->render_bytes, carve_bytes(n));
+    const uint32_t spp = (uint32_t)cfg->spp;
+    uint32_t flight = cfg->passes_in_flight;
+    if (flight == 0) flight = std::max(1u, kAutoPathsInFlight / n);
+    flight = std::max(1u, std::min({flight, std::max(spp, 1u), kMaxPathsInFlight / n}));
+    const uint32_t slots = n * flight;
+    int rc = ensure_buffer(ctx, &sc->d_render, &sc->render_bytes, carve_bytes(slots));
     if (rc) return rc;
-    PathBuf P = carve(sc->d_render, n);
-    const uint32_t grid = batch_grid(n);
-    rc = ensure_gstack(sc, grid * kBlock);
+    PathBuf P = carve(sc->d_render, slots);
+    const uint32_t grid = batch_grid(slots);
+    uint32_t tgrid = 0;
+    rc = resident_grid(ctx, reinterpret_cast<const void *>(&k_extend<false>), &tgrid);
+    if (rc) return rc;
+    rc = ensure_gstack(sc, tgrid * kBlock);
     if (rc) return rc;
     const int32_t gcap = std::max(4, sc->dev.stack_need - kLdsStack + 4);
     uint2 *gs = (uint2 *)sc->d_gstack;
 
     const bool timed = (cfg->flags & S3DG_RENDER_TIME_KERNELS) != 0;
     const bool count = (cfg->flags & S3DG_RENDER_COUNT_WORK) != 0;
-    const size_t launches = (size_t)cfg->spp * (size_t)cfg->max_bounce_count;
+    const size_t batches = (spp + flight - 1) / flight;
+    const size_t launches = batches * (size_t)cfg->max_bounce_count;
     std::vector<hipEvent_t> ev;
     if (timed) {
         ev.resize(4 * launches);
@@ -409,33 +461,34 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
     S3DG_HIP(ctx, hipEventRecord(e0, ctx->stream));
     hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
     size_t li = 0;
-    for (int32_t s = 0; s < cfg->spp; ++s) {
-        const uint32_t pass = (uint32_t)(cfg->first_pass + s);
+    for (uint32_t s0 = 0; s0 < spp; s0 += flight) {
+        const uint32_t passes = std::min(flight, spp - s0);
+        const uint32_t pass0 = (uint32_t)cfg->first_pass + s0;
         hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cfg->width, cfg->height,
-                           cfg->seed, pass, 0u);
+                           cfg->seed, pass0, n * passes, 0u);
         int32_t cur = 0;
         for (int32_t depth = 0; depth < cfg->max_bounce_count; ++depth, ++li) {
             if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li], ctx->stream));
             if (count)
-                hipLaunchKernelGGL(k_extend<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
-                                   gcap, sc->d_error);
+                hipLaunchKernelGGL(k_extend<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
+                                   gcap, sc->d_error, ctx->refill);
             else
-                hipLaunchKernelGGL(k_extend<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
-                                   gcap, sc->d_error);
+                hipLaunchKernelGGL(k_extend<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
+                                   gcap, sc->d_error, ctx->refill);
             if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 1], ctx->stream));
             hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, depth,
                                cfg->rr_start_depth);
             if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 2], ctx->stream));
             if (count)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
-                                   sc->d_error);
+                hipLaunchKernelGGL(k_shadow<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
+                                   sc->d_error, ctx->refill);
             else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
-                                   sc->d_error);
+                hipLaunchKernelGGL(k_shadow<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
+                                   sc->d_error, ctx->refill);
             if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 3], ctx->stream));
             cur ^= 1;
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_accum, n);
+        hipLaunchKernelGGL(k_accumulate, dim3(batch_grid(n)), dim3(kBlock), 0, ctx->stream, P, d_accum, n, passes);
     }
     S3DG_HIP(ctx, hipGetLastError());
     S3DG_HIP(ctx, hipEventRecord(e1, ctx->stream));

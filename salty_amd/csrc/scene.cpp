@@ -470,7 +470,10 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     }
     for (int a = 0; a < 3; ++a) dv.env[a] = d->environment[a];
     dv.light_count = (int32_t)d->light_count;
-    dv.stack_need = blas_depth + 9 * tlas_depth + 4;
+    // per BLAS level: one child group and one leaf group; per TLAS level: one
+    // child group and up to eight shapes; one restore entry
+    dv.stack_need = 2 * blas_depth + 9 * tlas_depth + 4;
+    dv.tri_batch = ctx->tri_batch;
 
     size_t total = 0;
     int rc;

@@ -2,6 +2,8 @@
 // (closest hit and any hit) behind s3dg_trace / s3dg_trace_device.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <new>
 
@@ -28,6 +30,16 @@ extern "C" int s3dg_create(s3dg_context **out, int device) {
     s3dg_context *ctx = new (std::nothrow) s3dg_context();
     if (!ctx) return S3DG_ERR_NOMEM;
     ctx->device = device;
+    // S3DG_REFILL (1..64) overrides the idle-lane count at which traversal waves take new rays
+    if (const char *env = std::getenv("S3DG_REFILL")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 1 && v <= 64) ctx->refill = (uint32_t)v;
+    }
+    // S3DG_TRI_BATCH (1..65) overrides the pending-triangle lane count that keeps a step testing
+    if (const char *env = std::getenv("S3DG_TRI_BATCH")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 1 && v <= 65) ctx->tri_batch = (uint32_t)v;
+    }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return S3DG_ERR_DEVICE;
@@ -93,38 +105,52 @@ namespace {
 template <bool ANY, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n,
                                                    s3dg_hit *__restrict__ hits, int32_t *__restrict__ shapes,
-                                                   uint2 *gstack, int32_t gcap, int32_t *err, unsigned long long *counters) {
+                                                   uint2 *gstack, int32_t gcap, int32_t *err, unsigned long long *counters,
+                                                   uint32_t *cursor, uint32_t refill) {
     __shared__ uint2 lds[kLdsStack * kBlock];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     Stack st;
-    st.lds = lds + threadIdx.x;
-    st.gst = gstack + gtid;
-    st.gstride = stride;
-    st.gcap = gcap;
-    st.err = err;
-    for (uint32_t i = gtid; i < n; i += stride) {
-        const float4 a = reinterpret_cast<const float4 *>(rays + i)[0];
-        const float4 b = reinterpret_cast<const float4 *>(rays + i)[1];
-        DRay r;
-        r.ox = a.x; r.oy = a.y; r.oz = a.z;
-        r.dx = b.x; r.dy = b.y; r.dz = b.z;
-        DHit h = no_hit();
-        st.sp = 0;
-        traverse<ANY, COUNT>(S, r, a.w, b.w, h, st, counters);
-        int32_t prim = h.prim;
-        if (prim >= 0) prim = S.shade[prim].input_index;
-        float4 o;
-        o.x = __int_as_float(prim);
-        o.y = h.t;
-        o.z = h.u;
-        o.w = h.v;
-        reinterpret_cast<float4 *>(hits)[i] = o;
-        if (shapes) shapes[i] = h.shape;
-    }
+    st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
+    Tracer<ANY, COUNT> tr;
+    uint32_t i = 0;
+    refill_loop(
+        cursor, n, refill,
+        [&](uint32_t k) {
+            i = k;
+            const float4 a = reinterpret_cast<const float4 *>(rays + i)[0];
+            const float4 b = reinterpret_cast<const float4 *>(rays + i)[1];
+            DRay r;
+            r.ox = a.x; r.oy = a.y; r.oz = a.z;
+            r.dx = b.x; r.dy = b.y; r.dz = b.z;
+            st.sp = 0;
+            tr.start(S, r, a.w, b.w);
+            return true;
+        },
+        [&]() { return tr.step(S, st); },
+        [&]() {
+            int32_t prim = tr.best.prim;
+            if (prim >= 0) prim = S.shade[prim].input_index;
+            float4 o;
+            o.x = __int_as_float(prim);
+            o.y = tr.best.t;
+            o.z = tr.best.u;
+            o.w = tr.best.v;
+            reinterpret_cast<float4 *>(hits)[i] = o;
+            if (shapes) shapes[i] = tr.best.shape;
+            tr.count(counters);
+        });
+}
+
+__global__ void k_clear_u32(uint32_t *p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
 int stack_capacity(const s3dg_scene *sc) { return std::max(4, sc->dev.stack_need - kLdsStack + 4); }
+
+// Work cursor of the batch kernel: a word of the scene's small status block
+// (the error flag lives at its start).
+uint32_t *trace_cursor(s3dg_scene *sc) { return reinterpret_cast<uint32_t *>(sc->d_error) + 16; }
 
 }  // namespace
 
@@ -134,6 +160,16 @@ namespace s3dg {
 uint32_t batch_grid(uint32_t n) {
     const uint32_t want = (n + kBlock - 1) / kBlock;
     return std::max(1u, std::min(want, 256u * 8u));
+}
+
+// Blocks of `kernel` (kBlock threads) that fit on the device at once: the
+// grid of a persistent kernel.
+int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks) {
+    int per_cu = 0, cus = 0;
+    S3DG_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0));
+    S3DG_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    *blocks = (uint32_t)std::max(1, per_cu) * (uint32_t)std::max(1, cus);
+    return S3DG_OK;
 }
 
 int ensure_gstack(s3dg_scene *sc, uint32_t threads) {
@@ -156,25 +192,31 @@ int launch_trace(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mod
                  int32_t *d_shapes, unsigned long long *d_counters) {
     s3dg_context *ctx = sc->ctx;
     if (n == 0) return S3DG_OK;
-    const uint32_t grid = batch_grid(n);
-    int rc = ensure_gstack(sc, grid * kBlock);
+    uint32_t grid = 0;
+    int rc = resident_grid(ctx, reinterpret_cast<const void *>(&k_trace<false, false>), &grid);
+    if (rc) return rc;
+    grid = std::min(grid, (n + kBlock - 1) / kBlock);
+    rc = ensure_gstack(sc, grid * kBlock);
     if (rc) return rc;
     uint2 *gs = (uint2 *)sc->d_gstack;
     const int32_t cap = stack_capacity(sc);
+    uint32_t *cursor = trace_cursor(sc);
+    const uint32_t refill = ctx->refill;
+    hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, ctx->stream, cursor);
     if (mode == S3DG_TRACE_ANY) {
         if (d_counters)
             hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
         else
             hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
     } else {
         if (d_counters)
             hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
         else
             hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters);
+                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
     }
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;

@@ -153,13 +153,24 @@ __device__ __forceinline__ RayPre precompute(const DRay &r) {
     return p;
 }
 
-enum { ENT_GROUP = 0u, ENT_INST = 1u, ENT_RESTORE = 2u };
+// Stack entries: a group of a node's hit inner children still to visit, a
+// top-level shape to enter, the return to the world ray after an instance,
+// and a group of a node's hit leaves whose triangles are still to test.
+enum { ENT_GROUP = 0u, ENT_INST = 1u, ENT_RESTORE = 2u, ENT_LEAVES = 3u };
 
 // Traversal stack: the first kLdsStack entries of a lane live in LDS (stride
-// kBlock), the rest in a global overflow area (stride gstride).
+// kBlock), the rest in a global overflow area (stride gstride).  The pointers
+// carry their address spaces so that the two halves are never merged into one
+// flat access (which would wait on both the LDS and the vector-memory counters).
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+typedef __attribute__((address_space(1))) uint64_t glb_u64;
+
+__device__ __forceinline__ uint64_t pack_entry(uint2 e) { return (uint64_t)e.x | ((uint64_t)e.y << 32); }
+__device__ __forceinline__ uint2 unpack_entry(uint64_t v) { return make_uint2((uint32_t)v, (uint32_t)(v >> 32)); }
+
 struct Stack {
-    uint2 *lds;
-    uint2 *gst;
+    lds_u64 *lds;
+    glb_u64 *gst;
     uint32_t gstride;
     int32_t gcap;
     int32_t *err;
@@ -167,9 +178,9 @@ struct Stack {
 
     __device__ __forceinline__ void push(uint2 e) {
         if (sp < kLdsStack) {
-            lds[sp * kBlock] = e;
+            lds[sp * kBlock] = pack_entry(e);
         } else if (sp - kLdsStack < gcap) {
-            gst[(size_t)(sp - kLdsStack) * gstride] = e;
+            gst[(size_t)(sp - kLdsStack) * gstride] = pack_entry(e);
         } else {
             atomicOr(err, 1);
             return;
@@ -178,86 +189,182 @@ struct Stack {
     }
     __device__ __forceinline__ uint2 top() const {
         const int32_t i = sp - 1;
-        return i < kLdsStack ? lds[i * kBlock] : gst[(size_t)(i - kLdsStack) * gstride];
+        if (i < kLdsStack) return unpack_entry(lds[i * kBlock]);
+        uint64_t v = gst[(size_t)(i - kLdsStack) * gstride];
+        // keeps the two loads apart (an empty asm that names the loaded value)
+        __asm__ volatile("" : "+v"(v));
+        return unpack_entry(v);
     }
     __device__ __forceinline__ void set_top(uint2 e) {
         const int32_t i = sp - 1;
-        if (i < kLdsStack) lds[i * kBlock] = e;
-        else gst[(size_t)(i - kLdsStack) * gstride] = e;
+        if (i < kLdsStack) lds[i * kBlock] = pack_entry(e);
+        else gst[(size_t)(i - kLdsStack) * gstride] = pack_entry(e);
+    }
+    // Lane `lane` of the block's stacks over the shared array `base`.
+    __device__ __forceinline__ void init(uint2 *base, uint32_t lane, uint2 *global_base, uint32_t global_stride,
+                                         int32_t global_cap, int32_t *error) {
+        lds = (lds_u64 *)(base + lane);
+        gst = (glb_u64 *)global_base;
+        gstride = global_stride;
+        gcap = global_cap;
+        err = error;
+        sp = 0;
     }
 };
 
-// Closest hit (ANY = false) or first hit found (ANY = true) in [tmin, tmax].
+// Incremental traversal of one ray.  start() sets the ray up; each step()
+// tests at most one pending triangle (more while most lanes of the wave have
+// triangles pending), visits at most one node and pops a stack entry when the
+// lane has neither, and returns true once the ray is finished (its result is
+// then in `best`).  Triangle tests and node visits of different lanes thus
+// share the wave's instructions instead of waiting for each other; kernels
+// call step() from persistent loops that hand finished lanes new rays.
+// ANY = false: closest hit in [tmin, tmax]; ANY = true: first hit found.
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ void traverse(const SceneDev &S, const DRay &wray, float tmin, float tmax,
-                                         DHit &best, Stack &st, unsigned long long *counters) {
-    DRay ray = wray;
-    RayPre pre = precompute(ray);
-    int32_t cur_shape = S.single_shape;
-    uint32_t node = S.root;
-    bool have_node = true;
-    const float lo_cull = cull_lo(tmin);
-    float bound = fminf(cull_hi(tmax), cull_hi(best.t));
-    uint32_t visits = 0, tests = 0;
+struct Tracer {
+    DRay wray;          // world-space ray
+    DRay ray;           // ray of the current tree (instance space inside an instance)
+    RayPre pre;
+    DHit best;
+    float tmin, tmax, lo_cull, bound;
+    uint32_t node;       // next node to visit (when have_node)
+    uint32_t tri_cur, tri_end;   // triangles of the current leaf still to test
+    uint32_t leaf_node, leaf_mask;   // node whose hit leaves (mask of slots) follow
+    int32_t cur_shape;
+    bool have_node;
+    uint32_t visits, tests;
 
-    for (;;) {
-        if (!have_node) {
-            if (st.sp == 0) break;
-            uint2 e = st.top();
-            const uint32_t kind = e.y >> 30;
-            if (kind == ENT_GROUP) {
-                const uint32_t imask = e.y & 0xffu;
-                uint32_t rem = (e.y >> 8) & 0xffu;
-                const uint32_t k0 = __builtin_ctz(rem);
-                const uint32_t s0 = k0 ^ pre.oct;
-                node = e.x + __builtin_popcount(imask & ((1u << s0) - 1u));
-                rem &= rem - 1u;
-                if (rem) st.set_top(make_uint2(e.x, (e.y & ~0xff00u) | (rem << 8)));
-                else st.sp--;
-                have_node = true;
-            } else if (kind == ENT_INST) {
-                st.sp--;
-                const GInst &in = S.insts[e.x];
-                if (in.type == INST_SPHERE) {
-                    const float d = sphere_test(in.center[0], in.center[1], in.center[2], in.radius, ray, tmin);
-                    if (d >= 0.0f && d <= tmax) {
-                        const bool take = ANY ? true
-                                              : (d < best.t || (d == best.t && (best.prim != -2 || in.shape > best.shape)));
-                        if (take) {
-                            best.t = d;
-                            best.u = 0.0f;
-                            best.v = 0.0f;
-                            best.prim = -2;
-                            best.shape = in.shape;
-                            if (ANY) return;
-                            bound = fminf(cull_hi(tmax), cull_hi(best.t));
-                        }
-                    }
-                } else {
-                    cur_shape = in.shape;
-                    if (in.type == INST_TRANSFORMED) {
-                        st.push(make_uint2(0u, ENT_RESTORE << 30));
-                        ray = to_local(in, wray);
-                        pre = precompute(ray);
-                    }
-                    node = in.blas_root;
-                    have_node = true;
-                }
-            } else {   // ENT_RESTORE
-                st.sp--;
-                ray = wray;
-                pre = precompute(ray);
+    __device__ __forceinline__ void start(const SceneDev &S, const DRay &r, float t0, float t1) {
+        wray = r;
+        ray = r;
+        pre = precompute(r);
+        best = no_hit();
+        tmin = t0;
+        tmax = t1;
+        lo_cull = cull_lo(t0);
+        bound = cull_hi(t1);
+        node = S.root;
+        cur_shape = S.single_shape;
+        have_node = true;
+        tri_cur = tri_end = 0;
+        leaf_mask = 0;
+        leaf_node = 0;
+        if (COUNT) visits = tests = 0;
+    }
+
+    __device__ __forceinline__ void take(float t, float u, float v, int32_t prim, int32_t shape) {
+        best.t = t;
+        best.u = u;
+        best.v = v;
+        best.prim = prim;
+        best.shape = shape;
+        bound = fminf(cull_hi(tmax), cull_hi(t));
+    }
+
+    // Moves to the triangles of the next pending leaf of leaf_node.
+    __device__ __forceinline__ void next_leaf(const SceneDev &S) {
+        const uint32_t s = (uint32_t)__builtin_ctz(leaf_mask);
+        leaf_mask &= leaf_mask - 1u;
+        const uint32_t *w = reinterpret_cast<const uint32_t *>(S.nodes + leaf_node);
+        const uint32_t cnts = w[5 + (s >> 2)];   // cnt bytes of slots 0-3, 4-7
+        tri_cur = w[8 + s];
+        tri_end = tri_cur + ((cnts >> ((s & 3u) * 8u)) & 0xffu);
+    }
+
+    // Tests triangle tri_cur.  Returns true when an any-hit search is over.
+    __device__ __forceinline__ bool test_triangle(const SceneDev &S) {
+        const uint32_t i = tri_cur++;
+        float t, u, v;
+        if (COUNT) ++tests;
+        if (tri_test(S.tris, i, ray, tmin, tmax, t, u, v)) {
+            if (ANY) {
+                take(t, u, v, (int32_t)i, cur_shape);
+                return true;
             }
-            continue;
+            // the first triangle in shape order, then depth-first order, wins a tie
+            const bool win = t < best.t ||
+                             (t == best.t && best.prim >= 0 &&
+                              (cur_shape < best.shape || (cur_shape == best.shape && (int32_t)i < best.prim)));
+            if (win) take(t, u, v, (int32_t)i, cur_shape);
         }
+        if (tri_cur == tri_end && leaf_mask) next_leaf(S);
+        return false;
+    }
 
-        // ---- visit one compressed node --------------------------------
+    // Pops one entry: a node group yields its next child, an instance entry
+    // enters the instance (or tests the sphere), a restore entry returns to
+    // the world ray, a leaf group becomes the pending triangles.  Returns true
+    // when an any-hit search is over.
+    __device__ __forceinline__ bool pop(const SceneDev &S, Stack &st) {
+        const uint2 e = st.top();
+        const uint32_t kind = e.y >> 30;
+        if (kind == ENT_GROUP) {
+            const uint32_t imask = e.y & 0xffu;
+            uint32_t rem = (e.y >> 8) & 0xffu;
+            const uint32_t s0 = (uint32_t)__builtin_ctz(rem) ^ pre.oct;
+            node = e.x + (uint32_t)__builtin_popcount(imask & ((1u << s0) - 1u));
+            rem &= rem - 1u;
+            if (rem) st.set_top(make_uint2(e.x, (e.y & ~0xff00u) | (rem << 8)));
+            else st.sp--;
+            have_node = true;
+        } else if (kind == ENT_LEAVES) {
+            st.sp--;
+            leaf_node = e.x;
+            leaf_mask = e.y & 0xffu;
+            next_leaf(S);
+        } else if (kind == ENT_INST) {
+            st.sp--;
+            const GInst &in = S.insts[e.x];
+            if (in.type == INST_SPHERE) {
+                const float d = sphere_test(in.center[0], in.center[1], in.center[2], in.radius, ray, tmin);
+                if (d >= 0.0f && d <= tmax) {
+                    // a sphere takes any tie (Sphere::IsHit rejects only t > best)
+                    const bool win = ANY || d < best.t || (d == best.t && (best.prim != -2 || in.shape > best.shape));
+                    if (win) {
+                        take(d, 0.0f, 0.0f, -2, in.shape);
+                        if (ANY) return true;
+                    }
+                }
+            } else {
+                cur_shape = in.shape;
+                if (in.type == INST_TRANSFORMED) {
+                    st.push(make_uint2(0u, ENT_RESTORE << 30));
+                    ray = to_local(in, wray);
+                    pre = precompute(ray);
+                }
+                node = in.blas_root;
+                have_node = true;
+            }
+        } else {   // ENT_RESTORE
+            st.sp--;
+            ray = wray;
+            pre = precompute(ray);
+        }
+        return false;
+    }
+
+    // Visits `node`: slab tests of its eight children; the hit leaves become
+    // the pending triangles (or a stacked leaf group when triangles are still
+    // pending), the nearest hit inner child is visited next and the others are
+    // pushed as one group.
+    struct NodeLines {
+        uint4 h0, h1, q0, q1, q2;
+    };
+
+    __device__ __forceinline__ NodeLines fetch(const SceneDev &S) const {
         const uint4 *np = reinterpret_cast<const uint4 *>(S.nodes + node);
-        const uint4 h0 = np[0];
-        const uint4 h1 = np[1];
-        const uint4 q0 = np[4];
-        const uint4 q1 = np[5];
-        const uint4 q2 = np[6];
+        NodeLines n;
+        n.h0 = np[0];
+        n.h1 = np[1];
+        n.q0 = np[4];
+        n.q1 = np[5];
+        n.q2 = np[6];
+        return n;
+    }
+
+    __device__ __forceinline__ void visit(const SceneDev &S, Stack &st, const NodeLines &nl) {
+        const uint4 *np = reinterpret_cast<const uint4 *>(S.nodes + node);
+        const uint4 h0 = nl.h0, h1 = nl.h1, q0 = nl.q0, q1 = nl.q1, q2 = nl.q2;
         if (COUNT) ++visits;
         const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
         const uint32_t imask = h0.w >> 24;
@@ -269,24 +376,30 @@ __device__ __forceinline__ void traverse(const SceneDev &S, const DRay &wray, fl
         const uint32_t child_base = h1.x;
         const uint32_t cnt_lo = h1.y, cnt_hi = h1.z;
         const uint32_t top = h1.w & 1u;
+        // near / far planes per axis from the direction signs: for a positive
+        // direction component the low plane is entered first
+        const bool fx = (pre.oct & 1u) != 0, fy = (pre.oct & 2u) != 0, fz = (pre.oct & 4u) != 0;
+        const uint32_t nx0 = fx ? q1.z : q0.x, nx1 = fx ? q1.w : q0.y;
+        const uint32_t ux0 = fx ? q0.x : q1.z, ux1 = fx ? q0.y : q1.w;
+        const uint32_t ny0 = fy ? q2.x : q0.z, ny1 = fy ? q2.y : q0.w;
+        const uint32_t uy0 = fy ? q0.z : q2.x, uy1 = fy ? q0.w : q2.y;
+        const uint32_t nz0 = fz ? q2.z : q1.x, nz1 = fz ? q2.w : q1.y;
+        const uint32_t uz0 = fz ? q1.x : q2.z, uz1 = fz ? q1.y : q2.w;
         uint32_t hits = 0;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int sh = (s & 3) * 8;
-            const uint32_t w0 = s < 4 ? q0.x : q0.y;   // qlo x
-            const uint32_t w1 = s < 4 ? q0.z : q0.w;   // qlo y
-            const uint32_t w2 = s < 4 ? q1.x : q1.y;   // qlo z
-            const uint32_t w3 = s < 4 ? q1.z : q1.w;   // qhi x
-            const uint32_t w4 = s < 4 ? q2.x : q2.y;   // qhi y
-            const uint32_t w5 = s < 4 ? q2.z : q2.w;   // qhi z
-            const float tlx = fmaf((float)((w0 >> sh) & 0xffu), ax, bx);
-            const float thx = fmaf((float)((w3 >> sh) & 0xffu), ax, bx);
-            const float tly = fmaf((float)((w1 >> sh) & 0xffu), ay, by);
-            const float thy = fmaf((float)((w4 >> sh) & 0xffu), ay, by);
-            const float tlz = fmaf((float)((w2 >> sh) & 0xffu), az, bz);
-            const float thz = fmaf((float)((w5 >> sh) & 0xffu), az, bz);
-            const float tn = fmaxf(fmaxf(fminf(tlx, thx), fminf(tly, thy)), fmaxf(fminf(tlz, thz), lo_cull));
-            const float tf = fminf(fminf(fmaxf(tlx, thx), fmaxf(tly, thy)), fminf(fmaxf(tlz, thz), bound));
+            const uint32_t wnx = s < 4 ? nx0 : nx1, wux = s < 4 ? ux0 : ux1;
+            const uint32_t wny = s < 4 ? ny0 : ny1, wuy = s < 4 ? uy0 : uy1;
+            const uint32_t wnz = s < 4 ? nz0 : nz1, wuz = s < 4 ? uz0 : uz1;
+            const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bx);
+            const float tux = fmaf((float)((wux >> sh) & 0xffu), ax, bx);
+            const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, by);
+            const float tuy = fmaf((float)((wuy >> sh) & 0xffu), ay, by);
+            const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bz);
+            const float tuz = fmaf((float)((wuz >> sh) & 0xffu), az, bz);
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, lo_cull));
+            const float tf = fminf(fminf(tux, tuy), fminf(tuz, bound));
             if (tn <= tf) hits |= 1u << s;
         }
         // valid slots: internal children or leaves with a non-zero count
@@ -297,65 +410,120 @@ __device__ __forceinline__ void traverse(const SceneDev &S, const DRay &wray, fl
             valid |= (c != 0u ? 1u : 0u) << s;
         }
         hits &= valid;
-        uint32_t leaves = hits & ~imask;
+        const uint32_t leaves = hits & ~imask;
         if (leaves) {
             if (top) {
-                // top-level leaves are single shapes: queued, visited in turn
+                // top-level leaves are single shapes: queued, visited in turn,
+                // far ones pushed first so that the nearest is popped first
                 uint32_t lk = to_korder(leaves, pre.oct);
-                // push far ones first so the nearest is popped first
                 while (lk) {
                     const uint32_t k = 31u - __builtin_clz(lk);
                     lk &= ~(1u << k);
                     const uint32_t s = k ^ pre.oct;
                     st.push(make_uint2(reinterpret_cast<const uint32_t *>(np)[8 + s], ENT_INST << 30));
                 }
+            } else if (tri_cur == tri_end && leaf_mask == 0) {
+                const uint32_t s = (uint32_t)__builtin_ctz(leaves);
+                leaf_node = node;
+                leaf_mask = leaves & (leaves - 1u);
+                tri_cur = reinterpret_cast<const uint32_t *>(np)[8 + s];
+                tri_end = tri_cur + (((s < 4 ? cnt_lo : cnt_hi) >> ((s & 3u) * 8u)) & 0xffu);
             } else {
-                while (leaves) {
-                    const uint32_t s = __builtin_ctz(leaves);
-                    leaves &= leaves - 1u;
-                    const uint32_t start = reinterpret_cast<const uint32_t *>(np)[8 + s];
-                    const uint32_t c = ((s < 4 ? cnt_lo : cnt_hi) >> ((s & 3) * 8)) & 0xffu;
-                    for (uint32_t i = start; i < start + c; ++i) {
-                        float t, u, v;
-                        if (COUNT) ++tests;
-                        if (!tri_test(S.tris, i, ray, tmin, tmax, t, u, v)) continue;
-                        if (ANY) {
-                            best.t = t;
-                            best.u = u;
-                            best.v = v;
-                            best.prim = (int32_t)i;
-                            best.shape = cur_shape;
-                            if (COUNT) { atomicAdd(&counters[0], (unsigned long long)visits); atomicAdd(&counters[1], (unsigned long long)tests); }
-                            return;
-                        }
-                        const bool take = t < best.t ||
-                                          (t == best.t && best.prim >= 0 &&
-                                           (cur_shape < best.shape || (cur_shape == best.shape && (int32_t)i < best.prim)));
-                        if (take) {
-                            best.t = t;
-                            best.u = u;
-                            best.v = v;
-                            best.prim = (int32_t)i;
-                            best.shape = cur_shape;
-                            bound = fminf(cull_hi(tmax), cull_hi(best.t));
-                        }
-                    }
-                }
+                st.push(make_uint2(node, (ENT_LEAVES << 30) | leaves));
             }
         }
         const uint32_t inner = to_korder(hits & imask, pre.oct);
         if (inner) {
-            const uint32_t k0 = __builtin_ctz(inner);
-            const uint32_t s0 = k0 ^ pre.oct;
+            const uint32_t s0 = (uint32_t)__builtin_ctz(inner) ^ pre.oct;
             const uint32_t rem = inner & (inner - 1u);
             if (rem) st.push(make_uint2(child_base, (ENT_GROUP << 30) | (rem << 8) | imask));
-            node = child_base + __builtin_popcount(imask & ((1u << s0) - 1u));
+            node = child_base + (uint32_t)__builtin_popcount(imask & ((1u << s0) - 1u));
             have_node = true;
         } else {
             have_node = false;
         }
     }
-    if (COUNT) { atomicAdd(&counters[0], (unsigned long long)visits); atomicAdd(&counters[1], (unsigned long long)tests); }
+
+    // One step; true when the ray is finished.  The next node's lines are
+    // requested before the pending triangles are tested, so that the two
+    // memory round trips overlap.
+    __device__ __forceinline__ bool step(const SceneDev &S, Stack &st) {
+        bool tris = tri_cur < tri_end;
+        if (!have_node && !tris) {
+            if (pop(S, st)) return true;
+            tris = tri_cur < tri_end;
+        }
+        NodeLines nl;
+        if (have_node) nl = fetch(S);
+        // triangle tests: one per step, more while most of the wave has triangles pending
+        for (;;) {
+            if (tris) {
+                if (test_triangle(S)) return true;
+                tris = tri_cur < tri_end;
+            }
+            if ((uint32_t)__popcll(__ballot(tris)) < S.tri_batch) break;
+        }
+        if (have_node) {
+            // keeps the compiler from evaluating node fields before the
+            // triangle tests (which would wait for the node lines there)
+            __asm__ volatile("" : "+v"(nl.h0.w), "+v"(nl.h1.w));
+            visit(S, st, nl);
+        }
+        return !have_node && tri_cur >= tri_end && leaf_mask == 0 && st.sp == 0;
+    }
+
+    __device__ __forceinline__ void count(unsigned long long *counters) const {
+        if (COUNT) {
+            atomicAdd(&counters[0], (unsigned long long)visits);
+            atomicAdd(&counters[1], (unsigned long long)tests);
+        }
+    }
+};
+
+// Whole traversal of one ray (for kernels without lane refill).
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ DHit traverse(const SceneDev &S, const DRay &r, float tmin, float tmax, Stack &st,
+                                         unsigned long long *counters) {
+    Tracer<ANY, COUNT> tr;
+    st.sp = 0;
+    tr.start(S, r, tmin, tmax);
+    while (!tr.step(S, st)) {
+    }
+    tr.count(counters);
+    return tr.best;
+}
+
+// Persistent-kernel work distribution with lane refill: lanes that have no
+// ray take new ones from `cursor` whenever at least `refill` lanes of the
+// wave are idle.  `begin(i)` sets lane state up for queue entry i and returns
+// false when the entry needs no traversal; `finish()` consumes the result.
+template <class Begin, class Step, class Finish>
+__device__ __forceinline__ void refill_loop(uint32_t *cursor, uint32_t n, uint32_t refill, Begin begin, Step step,
+                                            Finish finish) {
+    bool active = false, drained = false;
+    const uint64_t below = (1ull << __lane_id()) - 1ull;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        const uint32_t want = (uint32_t)__popcll(idle);
+        if (!drained && want >= refill) {
+            uint32_t base = 0;
+            if (__lane_id() == 0) base = atomicAdd(cursor, want);
+            base = __shfl(base, 0);
+            if (base + want >= n) drained = true;
+            if (!active) {
+                const uint32_t i = base + (uint32_t)__popcll(idle & below);
+                if (i < n) active = begin(i);
+            }
+        }
+        if (!active) {
+            if (drained && __ballot(active) == 0) break;
+            continue;
+        }
+        if (step()) {
+            finish();
+            active = false;
+        }
+    }
 }
 
 }  // namespace s3dg

@@ -28,3 +28,18 @@ def test_cornell_matches_oracle(ctx):
     assert abs(int(st.extension_rays) - int(counts[0])) <= 1e-3 * counts[0]
     assert abs(int(st.shadow_rays) - int(counts[1])) <= 1e-3 * counts[1]
     assert st.paths == counts[2] == w * h * 4
+
+
+def test_passes_in_flight_do_not_change_the_sum(ctx):
+    """Tracing several passes per wavefront changes only the schedule: the
+    accumulated sum and the ray counts are identical bit for bit."""
+    import salty_amd
+    sc = salty_amd.Scene(ctx, scenes.cornell_box())
+    w, h = 80, 48
+    outs = []
+    for flight in (1, 3, 0):
+        img, st = sc.render(w, h, spp=7, max_depth=6, seed=5, first_pass=2, passes_in_flight=flight)
+        outs.append((img, st.extension_rays, st.shadow_rays))
+    for img, ne, ns in outs[1:]:
+        assert np.array_equal(img, outs[0][0])
+        assert (ne, ns) == outs[0][1:]
