@@ -42,6 +42,11 @@ def parse():
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--workload", choices=("mesh", "soup", "unet"), default="mesh",
+                   help="mesh: the path-traced mesh scene (default); soup: closest/any-hit rays against a "
+                        "triangle soup; unet: denoiser inference at 1920x1080 and training steps")
+    p.add_argument("--soup-tris", type=int, default=4 << 20)
+    p.add_argument("--soup-rays", type=int, default=16 << 20)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a plain device copy of MIB MiB (reference point for HBM byte counters) and exit")
     return p.parse_args()
@@ -95,10 +100,201 @@ def cpu_baseline(desc, args):
     }
 
 
+def soup_cpu_baseline(desc, rays, seconds):
+    """The oracle's reference traversal (no clipping, exact slab tests) on a
+    prefix of the ray batch sized to about `seconds`, on the host's cores."""
+    from oracle import oracle as O
+    threads = max(1, min(16, os.cpu_count() or 1))
+    mesh = O.OracleMesh(desc.meshes[0].vertices, desc.meshes[0].bvh_kind)
+    t = time.perf_counter()
+    mesh.trace(rays[:256], threads=threads)
+    per_ray = max((time.perf_counter() - t) / 256, 1e-9)
+    m = int(min(len(rays), max(1024, seconds / per_ray)))
+    t = time.perf_counter()
+    mesh.trace(rays[:m], threads=threads)
+    el = time.perf_counter() - t
+    return {"value": m / el / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
+            "sample": f"first {m} rays of the batch, closest hit ({el:.1f} s on {threads} threads)"}
+
+
+def soup_main(args):
+    """Closest-hit and any-hit throughput of the batch traversal kernel on a
+    triangle soup (v0 ~ U[0,1]^3, edges of length N(0.01, 0.003) in uniform
+    directions) with rays from U[-0.1, 1.1]^3 in uniform directions,
+    tmin 1e-4, tmax infinity."""
+    import salty_amd
+    from salty_amd import abi, scenes
+    t = time.perf_counter()
+    desc = scenes.triangle_soup(args.soup_tris, seed=args.seed)
+    rays = scenes.soup_rays(args.soup_rays, seed=args.seed + 1)
+    gen_s = time.perf_counter() - t
+    ctx = salty_amd.Context(0)
+    t = time.perf_counter()
+    scene = salty_amd.Scene(ctx, desc)
+    build_s = time.perf_counter() - t
+    n = len(rays)
+    d_rays = ctx.upload(rays)
+    d_hits = ctx.malloc(n * 16)
+    d_cnt = ctx.upload(np.zeros(2, np.uint64))
+    scene.trace_counted_device(d_rays, n, abi.TRACE_CLOSEST, d_hits, d_cnt)
+    cnt = ctx.download(d_cnt, (2,), np.uint64)
+    nodes, tris = float(cnt[0]) / n, float(cnt[1]) / n
+    res = {}
+    for mode, name in ((abi.TRACE_CLOSEST, "closest"), (abi.TRACE_ANY, "any")):
+        for _ in range(args.warmup):
+            scene.trace_device(d_rays, n, mode, d_hits)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            scene.trace_device(d_rays, n, mode, d_hits)
+        res[name] = (time.perf_counter() - t) / args.steps
+    bytes_per_ray = nodes * NODE_BYTES + tris * TRI_BYTES + 32 + 16
+    achieved = bytes_per_ray * n / res["closest"] / 1e9
+    line = {
+        "metric": "Grays/s, closest-hit batch traversal (any-hit beside it)",
+        "value": n / res["closest"] / 1e9,
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["closest"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded triangle soup and rays)",
+        "config": {"workload": f"triangle soup, {args.soup_tris} triangles, {n} rays, tmin 1e-4, tmax inf",
+                   "any_hit_grays_per_s": n / res["any"] / 1e9, "any_hit_ms": res["any"] * 1e3,
+                   "scene_build_s": build_s, "generate_s": gen_s,
+                   "device_scene_bytes": scene.stats().device_bytes},
+        "roofline": {"kernel": "k_trace<closest> (batch traversal of the compressed BVH8)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_ray": bytes_per_ray, "nodes_per_ray": nodes,
+                     "tris_per_ray": tris},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = soup_cpu_baseline(desc, rays, args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+    for p in (d_rays, d_hits, d_cnt):
+        ctx.free(p)
+
+
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
+
+
+def unet_cpu_baseline(seconds):
+    """The float64 NumPy U-Net forward of the oracle on a 64x64 crop, repeated
+    to about `seconds`, in FLOP/s."""
+    from oracle import unet
+    from salty_amd import denoise
+    rng = np.random.default_rng(0)
+    _, nparam = unet.param_layout()
+    p = rng.normal(scale=0.05, size=nparam)
+    f = np.abs(rng.normal(size=(1, 64, 64, 10)))
+    t = np.abs(rng.normal(size=(1, 64, 64, 3)))
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        unet.loss_and_grad(p, f, t)   # forward + backward; counted as 3 forwards
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    flops = reps * 3 * denoise.forward_flops(1, 64, 64)
+    return {"value": flops / el / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} forward+backward passes of a 64x64 crop in float64 NumPy ({el:.1f} s)"}
+
+
+def unet_main(args):
+    """Denoiser inference on one 1920x1080 10-channel frame, training steps of
+    16 x 256^2 crops, and NLM (search radius 7, patch radius 2) on the frame."""
+    import salty_amd
+    from salty_amd import denoise
+    ctx = salty_amd.Context(0)
+    net = salty_amd.Denoiser(ctx, seed=0)
+    denoise.load_weights(net)
+    rng = np.random.default_rng(args.seed)
+    W, H = args.width, args.height
+    feat = np.concatenate([rng.gamma(1.0, 0.5, (1, H, W, 3)), rng.uniform(0, 1, (1, H, W, 3)),
+                           rng.normal(size=(1, H, W, 3)), rng.uniform(1, 5, (1, H, W, 1))], -1).astype(np.float32)
+    d_f = ctx.upload(feat)
+    d_o = ctx.malloc(H * W * 3 * 4)
+    for _ in range(args.warmup):
+        net.infer_device(1, H, W, d_f, d_o)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        net.infer_device(1, H, W, d_f, d_o)
+    ctx.synchronize()
+    inf_s = (time.perf_counter() - t) / args.steps
+    # NLM on the same frame (RGBA)
+    rgba = np.concatenate([feat[0, ..., :3], np.ones((H, W, 1), np.float32)], -1)
+    d_in = ctx.upload(rgba)
+    d_nl = ctx.malloc(rgba.nbytes)
+    ctx.nlm_device(W, H, 0.2, 7, 2, d_in, d_nl)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.nlm_device(W, H, 0.2, 7, 2, d_in, d_nl)
+    ctx.synchronize()
+    nlm_s = (time.perf_counter() - t) / args.steps
+    # training steps at 16 x 256^2
+    B, S = 16, 256
+    tf = np.ascontiguousarray(feat[0, :S, :S][None].repeat(B, 0))
+    d_tf = ctx.upload(tf)
+    d_tt = ctx.upload(np.ascontiguousarray(tf[..., :3]))
+    net.train_step_device(B, S, S, d_tf, d_tt, 1e-6)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        net.train_step_device(B, S, S, d_tf, d_tt, 1e-6)
+    ctx.synchronize()
+    train_s = (time.perf_counter() - t) / args.steps
+    fl = denoise.forward_flops(1, H, W)
+    achieved = fl / inf_s / 1e12
+    line = {
+        "metric": "U-Net denoiser inference TFLOP/s at 1920x1080 (training steps and NLM beside it)",
+        "value": achieved,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": inf_s * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic (random 10-channel frame)",
+        "config": {"workload": f"4-level U-Net 32/64/96/128, {W}x{H} 10-channel input, BF16 MFMA, FP32 accumulation",
+                   "inference_gflop": fl / 1e9,
+                   "train_step_ms": train_s * 1e3, "train_step_gflop": 3 * denoise.forward_flops(B, S, S) / 1e9,
+                   "train_tflops": 3 * denoise.forward_flops(B, S, S) / train_s / 1e12,
+                   "nlm_ms": nlm_s * 1e3, "nlm_radii": [7, 2]},
+        "roofline": {"kernel": "whole inference (3x3 convolutions dominate)", "bound": "mfma",
+                     "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = unet_cpu_baseline(min(args.cpu_seconds, 10.0))
+    print(json.dumps(line), flush=True)
+    for p in (d_f, d_o, d_in, d_nl, d_tf, d_tt):
+        ctx.free(p)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "soup":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        soup_main(args)
+        return
+    if args.workload == "unet":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        unet_main(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path shards by pixel tiles x "

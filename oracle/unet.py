@@ -100,8 +100,19 @@ def up_bwd(dy):
     return dy.reshape(n, h // 2, 2, w // 2, 2, c).sum(axis=(2, 4))
 
 
-def loss_and_grad(params, features, target):
-    """Returns (mean L1 loss, gradient vector, prediction in log space)."""
+REL_EPS = 1e-2
+
+
+def loss_and_grad(params, features, target, den=None):
+    """Returns (mean loss, gradient vector, head output in log space).
+
+    The prediction is expm1 of the head output.  The loss per value is the
+    relative squared error (pred - target)^2 / (den + REL_EPS) with den =
+    pred^2 held constant (not differentiated), or the given array `den`
+    (fixing it makes the loss a smooth function of the parameters, which is
+    how the gradient is checked by finite differences).  Pixels whose depth
+    guide is 0 (camera ray missed) are excluded from the loss: the network's
+    output is replaced there by the input radiance."""
     P = unpack(np.asarray(params, np.float64))
     lay, nparam = param_layout()
     cache = {}
@@ -123,12 +134,16 @@ def loss_and_grad(params, features, target):
     d0 = layer("d0b", layer("d0a", np.concatenate([up(d1), e0], -1)))
     Wh, bh = P["head"]
     out = d0 @ Wh.T + bh
-    t = np.log1p(np.maximum(np.asarray(target, np.float64), 0))
-    diff = out - t
+    t = np.maximum(np.asarray(target, np.float64), 0)
+    pred = np.expm1(out)
+    d = pred * pred if den is None else np.asarray(den, np.float64)
+    # pixels whose camera ray missed (depth guide 0) are passed through, not learned
+    hit = (np.asarray(features)[..., 9] != 0)[..., None]
+    diff = np.where(hit, pred - t, 0.0)
     count = diff.size
-    loss = np.abs(diff).sum() / count
+    loss = (diff * diff / (d + REL_EPS)).sum() / count
     g = np.zeros(nparam)
-    dout = np.sign(diff) / count
+    dout = 2.0 * diff / (d + REL_EPS) * np.exp(out) / count
     g[lay["head_w"]:lay["head_w"] + OUT * 32] = np.einsum("nhwk,nhwc->kc", dout, d0).reshape(-1)
     g[lay["head_b"]:lay["head_b"] + OUT] = dout.sum(axis=(0, 1, 2))
     dd0 = dout @ Wh
@@ -159,5 +174,5 @@ def loss_and_grad(params, features, target):
     return loss, g, out
 
 
-def loss_only(params, features, target):
-    return loss_and_grad(params, features, target)[0]
+def loss_only(params, features, target, den=None):
+    return loss_and_grad(params, features, target, den)[0]

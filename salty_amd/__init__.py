@@ -175,6 +175,11 @@ class Context:
                                      _ptr(out)), "s3dg_nlm")
         return out
 
+    def nlm_device(self, w: int, h: int, coeff: float, search_radius: int, patch_radius: int, d_in: int,
+                   d_out: int):
+        self.check(self.lib.s3dg_nlm_device(self.handle, w, h, coeff, search_radius, patch_radius,
+                                            C.c_void_p(d_in), C.c_void_p(d_out)), "s3dg_nlm_device")
+
 
 class Scene:
     """A scene uploaded to the device (two-level compressed BVH + shading data)."""

@@ -24,6 +24,7 @@ using namespace s3dg;
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 namespace {
 
@@ -71,127 +72,210 @@ __device__ __forceinline__ float load1(const ConvIn &I, int n, int y, int x, int
 }
 
 // ---------------------------------------------------------------------------
-// 3x3 convolution (stride 1, zero padding) as an implicit GEMM:
-// M = pixels, N = output channels, K = 9 taps x Cin.  A workgroup of four
-// waves covers 64 consecutive pixels of one row; each wave 16 pixels x 16*NT
-// output channels.  Used for the forward pass (weights [tap][cout][cin]) and
-// for the data gradient (flipped transposed weights, dY as input).
+// 3x3 convolution (stride 1, zero padding) as an implicit GEMM: M = pixels,
+// N = output channels, K = 9 taps x Cin, on v_mfma_f32_32x32x16_bf16.  A
+// workgroup (four waves) computes a 16 x 16 pixel tile for COT output
+// channels.  Per 32-channel slice of the input it stages the 18 x 18 halo
+// tile and the 9 x COT x 32 weights in LDS (rows padded from 64 to 80 bytes,
+// so that the 16-byte reads of eight consecutive lanes hit distinct banks),
+// then each wave runs 9 taps x 2 K steps over its 4 pixel rows (two 32-pixel
+// A tiles) and COT / 32 B tiles.  Used for the forward pass (weights
+// [tap][cout][cin]) and for the data gradient (flipped transposed weights,
+// dY as input).
 // ---------------------------------------------------------------------------
-template <int NT>
-__global__ __launch_bounds__(256) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
-                                                  const bf16 *__restrict__ wt, const float *__restrict__ bias,
-                                                  int32_t Cout, int32_t relu, bf16 *__restrict__ out) {
+constexpr int kTile = 16;                 // output tile edge (pixels)
+constexpr int kHalo = kTile + 2;
+constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 padding
+
+template <int COT>
+__global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
+                                                     const bf16 *__restrict__ wt, const float *__restrict__ bias,
+                                                     int32_t Cout, int32_t relu, bf16 *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) bf16 s_in[kHalo * kHalo * kRow];
+    __shared__ __attribute__((aligned(16))) bf16 s_w[9 * COT * kRow];
+    constexpr int NT = COT / 32;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int x0 = blockIdx.x * 64 + wave * 16;
-    const int y = blockIdx.y;
-    const int n = blockIdx.z / ((Cout + 16 * NT - 1) / (16 * NT));
-    const int co0 = (blockIdx.z % ((Cout + 16 * NT - 1) / (16 * NT))) * 16 * NT;
+    const int tiles_x = (W + kTile - 1) / kTile;
+    const int x0 = (blockIdx.x % tiles_x) * kTile, y0 = (blockIdx.x / tiles_x) * kTile;
+    const int co_blocks = (Cout + COT - 1) / COT;
+    const int n = blockIdx.y / co_blocks;
+    const int co0 = (blockIdx.y % co_blocks) * COT;
     const int Cin = I.C1 + I.C2;
-    const int px = x0 + (lane & 15);         // A row of this lane
-    const int kg = (lane >> 4) * 8;          // K sub-block of this lane
-    f32x4 acc[NT];
-    for (int t = 0; t < NT; ++t)
-        for (int i = 0; i < 4; ++i) acc[t][i] = 0.0f;
-    if (x0 < W) {
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-            const bf16 *wtap = wt + (size_t)tap * Cout * Cin;
-            for (int c0 = 0; c0 < Cin; c0 += 32) {
-                const bf16x8 a = load8(I, n, y + dy, px + dx, H, W, c0 + kg);
+    f32x16 acc[2][NT];
 #pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const int co = co0 + 16 * t + (lane & 15);
-                    bf16x8 b;
-                    if (co < Cout) b = *reinterpret_cast<const bf16x8 *>(wtap + (size_t)co * Cin + c0 + kg);
-                    else
-                        for (int j = 0; j < 8; ++j) b[j] = f2b(0.0f);
-                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[m][t][i] = 0.0f;
+    bf16x8 zero;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) zero[j] = f2b(0.0f);
+    const int koff = 8 * (lane >> 5);
+    const int p = lane & 31;
+    for (int c0 = 0; c0 < Cin; c0 += 32) {
+        for (int e = threadIdx.x; e < kHalo * kHalo * 4; e += 256) {
+            const int pix = e >> 2, q = e & 3;
+            const int y = y0 + pix / kHalo - 1, x = x0 + pix % kHalo - 1;
+            *reinterpret_cast<bf16x8 *>(&s_in[pix * kRow + q * 8]) = load8(I, n, y, x, H, W, c0 + q * 8);
+        }
+        for (int e = threadIdx.x; e < 9 * COT * 4; e += 256) {
+            const int row = e >> 2, q = e & 3;
+            const int tap = row / COT, co = co0 + row % COT;
+            bf16x8 v = zero;
+            if (co < Cout) v = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + co) * Cin + c0 + q * 8);
+            *reinterpret_cast<bf16x8 *>(&s_w[row * kRow + q * 8]) = v;
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+            for (int kk = 0; kk < 32; kk += 16) {
+                bf16x8 a[2], b[NT];
+#pragma unroll
+                for (int m = 0; m < 2; ++m) {
+                    const int r = wave * 4 + m * 2 + (p >> 4);
+                    a[m] = *reinterpret_cast<const bf16x8 *>(&s_in[((r + dy) * kHalo + (p & 15) + dx) * kRow + kk + koff]);
                 }
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    b[t] = *reinterpret_cast<const bf16x8 *>(&s_w[(tap * COT + 32 * t + p) * kRow + kk + koff]);
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int t = 0; t < NT; ++t)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
             }
         }
+        __syncthreads();
     }
-    // D layout: row (pixel) = 4*(lane/16) + i, column (channel) = lane % 16
+    // D layout: row (pixel of the 32-pixel tile) = 8*(i/4) + 4*(lane/32) + i%4,
+    // column (channel) = lane % 32
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int co = co0 + 16 * t + (lane & 15);
+        const int co = co0 + 32 * t + p;
         if (co >= Cout) continue;
         const float bv = bias ? bias[co] : 0.0f;
-        for (int i = 0; i < 4; ++i) {
-            const int x = x0 + 4 * (lane >> 4) + i;
-            if (x >= W) continue;
-            float v = acc[t][i] + bv;
-            if (relu) v = fmaxf(v, 0.0f);
-            out[(((size_t)n * H + y) * W + x) * Cout + co] = f2b(v);
-        }
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int row = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+                const int y = y0 + wave * 4 + m * 2 + (row >> 4), x = x0 + (row & 15);
+                if (y >= H || x >= W) continue;
+                float v = acc[m][t][i] + bv;
+                if (relu) v = fmaxf(v, 0.0f);
+                out[(((size_t)n * H + y) * W + x) * Cout + co] = f2b(v);
+            }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[tap][co][ci] += sum_p dPre[p][co] * X[p + off(tap)][ci]
-// as a GEMM with M = co, N = ci, K = pixels.  Grid: x = tap * (Cin/32) +
-// ci-chunk, y = co-chunk of 64, z = pixel slice.  Each wave accumulates a
-// 64 x 32 tile over every fourth 32-pixel run of the slice; the four waves are
-// reduced through LDS and added to dW with one FP32 atomic per element.
+// Weight gradient: dW[tap][co][ci] += sum_p dPre[p][co] * X[p + off(tap)][ci],
+// a GEMM per tap with M = co, N = ci, K = pixels, on
+// v_mfma_f32_16x16x32_bf16.  A workgroup owns COT output channels x 32 input
+// channels x all 9 taps and walks a strided set of 8 x 16 pixel tiles.  Per
+// tile it stages dPre transposed ([co][pixel]) and the 10 x 18 input halo
+// transposed in three copies shifted by 0, 1 and 2 columns ([shift][ci][row]
+// [16 columns]), so that every MFMA operand is one aligned 16-byte LDS read
+// (8 consecutive pixels of one channel) whatever the tap.  Wave w accumulates
+// 16 output channels x CT 16-channel input tiles x 9 taps in registers and
+// adds them to dW with FP32 atomics at the end.
 // ---------------------------------------------------------------------------
+constexpr int kDwRows = 8;                       // pixel tile: 8 rows x 16 columns
+constexpr int kDwPix = kDwRows * kTile;          // 128
+constexpr int kDwPitch = kDwPix + 8;             // dPre row (one channel), padded
+constexpr int kDwCi = 200;                       // one channel of a halo copy: 10 rows x 16 + padding
+
+template <int COT>
 __global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H, int32_t W,
-                                                  const bf16 *__restrict__ dpre, int32_t Cout,
-                                                  int32_t runs_per_slice, float *__restrict__ dw) {
-    __shared__ float red[4][64 * 32];
+                                                  const bf16 *__restrict__ dpre, int32_t Cout, float *__restrict__ dw) {
+    constexpr int CT = COT == 64 ? 2 : 1;        // input-channel tiles of 16 per wave
+    __shared__ __attribute__((aligned(16))) bf16 s_dy[COT * kDwPitch];
+    __shared__ __attribute__((aligned(16))) bf16 s_x[3 * 32 * kDwCi];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int Cin = I.C1 + I.C2;
     const int nchunk = Cin / 32;
-    const int tap = blockIdx.x / nchunk;
     const int ci0 = (blockIdx.x % nchunk) * 32;
-    const int co0 = blockIdx.y * 64;
-    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
-    const long long npix = (long long)N * H * W;
-    const long long nruns = (npix + 31) / 32;
-    const long long r0 = (long long)blockIdx.z * runs_per_slice;
-    f32x4 acc[4][2];
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 2; ++b)
-            for (int i = 0; i < 4; ++i) acc[a][b][i] = 0.0f;
-    const int kg = (lane >> 4) * 8;
-    for (long long r = r0 + wave; r < r0 + runs_per_slice && r < nruns; r += 4) {
-        const long long pbase = r * 32 + kg;
-        bf16x8 A[4], B[2];
-        int pn[8], py[8], pxx[8];
-        bool pv[8];
-        for (int j = 0; j < 8; ++j) {
-            const long long p = pbase + j;
-            pv[j] = p < npix;
-            const long long pp = pv[j] ? p : 0;
-            pxx[j] = (int)(pp % W);
-            py[j] = (int)((pp / W) % H);
-            pn[j] = (int)(pp / ((long long)W * H));
+    const int co0 = (blockIdx.x / nchunk) * COT;
+    const int cot = COT == 64 ? wave : (wave & 1);   // this wave's 16-channel output tile
+    const int ct0 = COT == 64 ? 0 : (wave >> 1);     // first input-channel tile
+    const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kDwRows - 1) / kDwRows;
+    const long long ntiles = (long long)N * tiles_x * tiles_y;
+    f32x4 acc[9][CT];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int c = 0; c < CT; ++c)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[t][c][i] = 0.0f;
+    const bf16 z = f2b(0.0f);
+    for (long long tile = blockIdx.y; tile < ntiles; tile += gridDim.y) {
+        const int n = (int)(tile / ((long long)tiles_x * tiles_y));
+        const int rem = (int)(tile % ((long long)tiles_x * tiles_y));
+        const int y0 = (rem / tiles_x) * kDwRows, x0 = (rem % tiles_x) * kTile;
+        // dPre tile, transposed
+        for (int e = threadIdx.x; e < kDwPix * (COT / 8); e += 256) {
+            const int px = e / (COT / 8), q = e % (COT / 8);
+            const int y = y0 + px / kTile, x = x0 + px % kTile;
+            const int co = co0 + q * 8;
+            bf16x8 v;
+            if (y < H && x < W && co < Cout) {
+                v = *reinterpret_cast<const bf16x8 *>(dpre + (((size_t)n * H + y) * W + x) * Cout + co);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) v[j] = z;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s_dy[(q * 8 + j) * kDwPitch + px] = v[j];
         }
-        for (int m = 0; m < 4; ++m) {
-            const int co = co0 + 16 * m + (lane & 15);
-            for (int j = 0; j < 8; ++j) {
-                float v = 0.0f;
-                if (pv[j] && co < Cout) v = b2f(dpre[(((size_t)pn[j] * H + py[j]) * W + pxx[j]) * Cout + co]);
-                A[m][j] = f2b(v);
+        // input halo (rows y0-1 .. y0+8, columns x0-1 .. x0+16), transposed, three shifted copies
+        for (int e = threadIdx.x; e < (kDwRows + 2) * (kTile + 2) * 4; e += 256) {
+            const int hp = e >> 2, q = e & 3;
+            const int hy = hp / (kTile + 2), hx = hp % (kTile + 2);
+            const bf16x8 v = load8(I, n, y0 + hy - 1, x0 + hx - 1, H, W, ci0 + q * 8);
+#pragma unroll
+            for (int sh = 0; sh < 3; ++sh) {
+                const int j = hx - sh;
+                if (j < 0 || j >= kTile) continue;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) s_x[(sh * 32 + q * 8 + k) * kDwCi + hy * kTile + j] = v[k];
             }
         }
-        for (int t = 0; t < 2; ++t) {
-            const int ci = ci0 + 16 * t + (lane & 15);
-            for (int j = 0; j < 8; ++j)
-                B[t][j] = f2b(pv[j] ? load1(I, pn[j], py[j] + dy, pxx[j] + dx, H, W, ci) : 0.0f);
+        __syncthreads();
+#pragma unroll 1
+        for (int ks = 0; ks < kDwPix / 32; ++ks) {
+            const int g = lane >> 4;                    // 8-pixel group of this lane
+            const int px = 32 * ks + 8 * g;
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&s_dy[(16 * cot + (lane & 15)) * kDwPitch + px]);
+            const int r = 2 * ks + (g >> 1), c = 8 * (g & 1);
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int dy = tap / 3, sh = tap % 3;   // halo row offset, column shift
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct) {
+                    const int ci = 16 * (ct0 + ct) + (lane & 15);
+                    const bf16x8 b =
+                        *reinterpret_cast<const bf16x8 *>(&s_x[(sh * 32 + ci) * kDwCi + (r + dy) * kTile + c]);
+                    acc[tap][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tap][ct], 0, 0, 0);
+                }
+            }
         }
-        for (int m = 0; m < 4; ++m)
-            for (int t = 0; t < 2; ++t) acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[m], B[t], acc[m][t], 0, 0, 0);
+        __syncthreads();
     }
-    // stash: element (co_local = 16m + 4*(lane/16) + i, ci_local = 16t + lane%16)
-    for (int m = 0; m < 4; ++m)
-        for (int t = 0; t < 2; ++t)
-            for (int i = 0; i < 4; ++i)
-                red[wave][(16 * m + 4 * (lane >> 4) + i) * 32 + 16 * t + (lane & 15)] = acc[m][t][i];
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-        const float s = red[0][e] + red[1][e] + red[2][e] + red[3][e];
-        const int co = co0 + e / 32, ci = ci0 + e % 32;
-        if (co < Cout && s != 0.0f) atomicAdd(&dw[((size_t)tap * Cout + co) * Cin + ci], s);
-    }
+    // D layout: row (output channel) = 4*(lane/16) + i, column (input channel) = lane % 16
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int co = co0 + 16 * cot + 4 * (lane >> 4) + i;
+                const int ci = ci0 + 16 * (ct0 + ct) + (lane & 15);
+                const float v = acc[tap][ct][i];
+                if (co < Cout && v != 0.0f) atomicAdd(&dw[((size_t)tap * Cout + co) * Cin + ci], v);
+            }
 }
 
 __global__ void k_bias_grad(const bf16 *__restrict__ dpre, long long npix, int32_t C, float *__restrict__ db) {
@@ -287,11 +371,19 @@ __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *
     }
 }
 
-// 1x1 head (32 -> 3, FP32) with the L1 loss on log radiance.  Inference
-// writes expm1(prediction) clamped at 0; training writes dL/dX (bf16) and
-// accumulates the head gradients and the loss.
+// 1x1 head (32 -> 3, FP32).  The prediction is expm1 of the head output;
+// inference writes it clamped at 0.  Pixels whose camera ray missed the scene
+// (depth guide exactly 0) carry the exact environment radiance in every
+// sample, so their input radiance is passed through and they take no part in
+// the loss.  Training uses the relative squared error
+// (pred - target)^2 / (pred^2 + kRelEps) with the denominator held constant
+// (unbiased under noisy targets), writes dL/dX (bf16) and accumulates the head
+// gradients and the loss.
+constexpr float kRelEps = 1e-2f;
+
 __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long long npix, const float *__restrict__ wh,
-                                               const float *__restrict__ bh, const float *__restrict__ target,
+                                               const float *__restrict__ bh, const float *__restrict__ feat,
+                                               const float *__restrict__ target,
                                                float *__restrict__ out, bf16 *__restrict__ dx, float *__restrict__ dwh,
                                                float *__restrict__ dbh, double *__restrict__ loss, float inv_count) {
     __shared__ float sw[kOut * 32 + kOut];
@@ -312,17 +404,29 @@ __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long l
             for (int c = 0; c < 32; ++c) s += sw[k * 32 + c] * xi[c];
             o[k] = s;
         }
+        const bool miss = feat[p * kInChannels + 9] == 0.0f;
         if (!target) {
-            for (int k = 0; k < kOut; ++k) out[p * kOut + k] = fmaxf(expm1f(o[k]), 0.0f);
+            for (int k = 0; k < kOut; ++k)
+                out[p * kOut + k] = miss ? feat[p * kInChannels + k] : fmaxf(expm1f(o[k]), 0.0f);
+            continue;
+        }
+        if (miss) {
+            for (int k = 0; k < kOut; ++k)
+                if (out) out[p * kOut + k] = feat[p * kInChannels + k];
+            for (int c = 0; c < 32; ++c) dx[p * 32 + c] = f2b(0.0f);
             continue;
         }
         float g[kOut];
         for (int k = 0; k < kOut; ++k) {
-            const float t = log1pf(fmaxf(target[p * kOut + k], 0.0f));
-            const float d = o[k] - t;
-            l += fabs((double)d);
-            g[k] = (d > 0.0f ? inv_count : (d < 0.0f ? -inv_count : 0.0f));
-            if (out) out[p * kOut + k] = fmaxf(expm1f(o[k]), 0.0f);
+            const float t = fmaxf(target[p * kOut + k], 0.0f);
+            // outputs beyond e^30 are treated as e^30 (keeps pred^2 finite)
+            const float oc = fminf(o[k], 30.0f);
+            const float pred = expm1f(oc);
+            const float d = pred - t;
+            const float den = pred * pred + kRelEps;
+            l += (double)(d * d / den);
+            g[k] = 2.0f * d / den * expf(oc) * inv_count;
+            if (out) out[p * kOut + k] = fmaxf(pred, 0.0f);
         }
         for (int c = 0; c < 32; ++c) {
             float s = 0.0f;
@@ -468,23 +572,28 @@ Acts carve_acts(void *base, long long n0) {
 void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
           bf16 *out) {
     hipStream_t s = c->ctx->stream;
-    const int nt = (Cout % 64 == 0) ? 4 : 2;
-    const int chunks = (Cout + 16 * nt - 1) / (16 * nt);
-    dim3 grid((W + 63) / 64, H, N * chunks);
-    if (nt == 4) hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
-    else hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+    const unsigned tiles = (unsigned)(((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile));
+    if (Cout % 64 == 0) {
+        dim3 grid(tiles, N * (Cout / 64));
+        hipLaunchKernelGGL(k_conv3x3<64>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+    } else {
+        dim3 grid(tiles, N * ((Cout + 31) / 32));
+        hipLaunchKernelGGL(k_conv3x3<32>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+    }
 }
 
 void conv_dw(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *dpre, int Cout, float *dw) {
     const int Cin = I.C1 + I.C2;
-    const long long npix = (long long)N * H * W;
-    const long long nruns = (npix + 31) / 32;
-    const int xy = 9 * (Cin / 32) * ((Cout + 63) / 64);
-    long long slices = std::max<long long>(1, std::min<long long>(4096 / xy + 1, nruns / 4 + 1));
-    const int runs = (int)((nruns + slices - 1) / slices);
-    slices = (nruns + runs - 1) / runs;
-    dim3 grid(9 * (Cin / 32), (Cout + 63) / 64, (unsigned)slices);
-    hipLaunchKernelGGL(k_conv_dw, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, runs, dw);
+    const long long ntiles = (long long)N * ((W + kTile - 1) / kTile) * ((H + kDwRows - 1) / kDwRows);
+    const int cot = (Cout % 64 == 0) ? 64 : 32;
+    const int blocks_xy = (Cin / 32) * ((Cout + cot - 1) / cot);
+    // about 1024 workgroups, each walking at least a few tiles
+    const long long slices = std::max<long long>(1, std::min<long long>(ntiles, 1024 / blocks_xy + 1));
+    dim3 grid((unsigned)blocks_xy, (unsigned)slices);
+    if (cot == 64)
+        hipLaunchKernelGGL(k_conv_dw<64>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw);
+    else
+        hipLaunchKernelGGL(k_conv_dw<32>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw);
 }
 
 void bias_grad(s3dg_cnn *c, const bf16 *dpre, long long npix, int C, float *db) {
@@ -609,7 +718,7 @@ int loss_and_grad(s3dg_cnn *c, int n, int h, int w, const float *d_feat, const f
     forward(c, a, d_feat, n, h, w);
     const float inv = 1.0f / (float)(n0 * kOut);
     hipLaunchKernelGGL(k_head, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 2048)), dim3(256), 0, s, a.d0b, n0,
-                       c->p + c->head_w, c->p + c->head_b, d_target, nullptr, a.g_a, c->g + c->head_w,
+                       c->p + c->head_w, c->p + c->head_b, d_feat, d_target, nullptr, a.g_a, c->g + c->head_w,
                        c->g + c->head_b, c->d_loss, inv);
     backward(c, a, n, h, w);
     S3DG_HIP(ctx, hipGetLastError());
@@ -725,7 +834,8 @@ extern "C" int s3dg_cnn_infer_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t 
     if (rc) return rc;
     forward(c, a, d_feat, n, h, w);
     hipLaunchKernelGGL(k_head, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 2048)), dim3(256), 0,
-                       c->ctx->stream, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, nullptr, d_out, nullptr, nullptr,
+                       c->ctx->stream, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, d_feat, nullptr, d_out, nullptr,
+                       nullptr,
                        nullptr, nullptr, 0.0f);
     S3DG_HIP(c->ctx, hipGetLastError());
     return S3DG_OK;

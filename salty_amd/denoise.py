@@ -50,6 +50,20 @@ class FrameRenderer:
         img = self.ctx.download(self.d_acc, (self.h, self.w, 3), np.float32) / np.float32(spp)
         return img, st.milliseconds
 
+    def ladder(self, levels, first_pass=0):
+        """Means after each of the increasing sample counts `levels` of one
+        progressive render (passes first_pass, first_pass + 1, ...):
+        [(spp, image, cumulative device ms)]."""
+        self.ctx.copy_to_device(self.d_acc, self.zero)
+        out, done, ms = [], 0, 0.0
+        for spp in levels:
+            st = self.scene.render_device(self.cfg(spp - done, first_pass + done), self.d_acc)
+            ms += st.milliseconds
+            done = spp
+            img = self.ctx.download(self.d_acc, (self.h, self.w, 3), np.float32) / np.float32(spp)
+            out.append((spp, img, ms))
+        return out
+
     def guides(self):
         t = time.perf_counter()
         self.scene.guides_device(self.cfg(1), self.d_alb, self.d_nrm, self.d_dep)
@@ -65,46 +79,99 @@ def features(mean_rgb, albedo, normal, depth) -> np.ndarray:
     return np.concatenate([mean_rgb, albedo, normal, depth], -1).astype(np.float32)
 
 
-def make_dataset(ctx, seeds, width=256, height=256, spp_in=4, spp_ref=1024, max_depth=8, log=None):
-    """Noisy/reference pairs of Cornell-style scenes: features (N,H,W,10), targets (N,H,W,3)."""
-    feats, tgts = [], []
+# (name, input channels with real data, output channels, resolution level) of
+# the 3x3 convolutions; the first layer's 10 input channels are padded to 32.
+CONV_LAYERS = [("e0a", 10, 32, 0), ("e0b", 32, 32, 0), ("e1a", 32, 64, 1), ("e1b", 64, 64, 1), ("e2a", 64, 96, 2),
+               ("e2b", 96, 96, 2), ("b0", 96, 128, 3), ("b1", 128, 128, 3), ("d2a", 224, 96, 2),
+               ("d2b", 96, 96, 2), ("d1a", 160, 64, 1), ("d1b", 64, 64, 1), ("d0a", 96, 32, 0), ("d0b", 32, 32, 0)]
+
+
+def forward_flops(n: int, h: int, w: int) -> float:
+    """Multiply-adds x 2 of one forward pass: the 3x3 convolutions at their
+    resolution levels plus the 1x1 head (32 -> 3)."""
+    px = [n * (h >> l) * (w >> l) for l in range(4)]
+    conv = sum(2.0 * 9 * ci * co * px[l] for _, ci, co, l in CONV_LAYERS)
+    return conv + 2.0 * 32 * 3 * px[0]
+
+
+INPUT_LEVELS = (1, 2, 4, 8, 16, 32, 64, 128, 256)
+
+
+class Dataset:
+    """Noisy inputs at several sample counts, guides and reference of each scene:
+    noisy (N, L, H, W, 3), guides (N, H, W, 7), target (N, H, W, 3)."""
+
+    def __init__(self, noisy, guides, target, levels):
+        self.noisy, self.guides, self.target, self.levels = noisy, guides, target, tuple(levels)
+
+    def __len__(self):
+        return len(self.target)
+
+
+def make_dataset(ctx, seeds, width=256, height=256, levels=INPUT_LEVELS, spp_ref=1024, max_depth=8, log=None):
+    """Training pairs of Cornell-style scenes.  The reference uses passes
+    disjoint from the inputs', so its noise is independent of theirs."""
+    noisy, guides, tgts = [], [], []
     for i, s in enumerate(seeds):
         fr = FrameRenderer(ctx, scenes.cornell_variant(int(s)), width, height, max_depth, seed=int(s))
-        noisy, _ = fr.mean(spp_in)
+        noisy.append(np.stack([img for _, img, _ in fr.ladder(levels)]))
         ref, _ = fr.mean(spp_ref, first_pass=1 << 20)
         a, n, d, _ = fr.guides()
         fr.close()
-        feats.append(features(noisy, a, n, d))
+        guides.append(np.concatenate([a, n, d], -1))
         tgts.append(ref)
-        if log and (i + 1) % 32 == 0:
+        if log and (i + 1) % 64 == 0:
             log(f"dataset {i + 1}/{len(seeds)}")
-    return np.stack(feats), np.stack(tgts)
+    return Dataset(np.stack(noisy), np.stack(guides), np.stack(tgts), levels)
 
 
 def augment(f, t, rng):
+    """Random mirror in x and in y (the normal's component changes sign)."""
     if rng.random() < 0.5:
         f, t = f[:, :, ::-1], t[:, :, ::-1]
         f = f.copy()
-        f[..., 6] = -f[..., 6]          # mirror the normal's x
+        f[..., 6] = -f[..., 6]
+    if rng.random() < 0.5:
+        f, t = f[:, ::-1], t[:, ::-1]
+        f = f.copy()
+        f[..., 7] = -f[..., 7]
     return np.ascontiguousarray(f), np.ascontiguousarray(t)
 
 
-def train(net, ctx, feats, tgts, steps=1500, batch=16, lr=1e-3, seed=0, log=None):
+def sample_batch(data: Dataset, batch, rng, crop=None):
+    """A batch of (features, target) with a random input level per element and
+    an optional random square crop."""
+    n, L, h, w, _ = data.noisy.shape
+    idx = rng.integers(n, size=batch)
+    lev = rng.integers(L, size=batch)
+    c = crop or h
+    y0 = rng.integers(h - c + 1, size=batch) if c < h else np.zeros(batch, int)
+    x0 = rng.integers(w - c + 1, size=batch) if c < w else np.zeros(batch, int)
+    f = np.empty((batch, c, c, 10), np.float32)
+    t = np.empty((batch, c, c, 3), np.float32)
+    for b in range(batch):
+        ys, xs = slice(y0[b], y0[b] + c), slice(x0[b], x0[b] + c)
+        f[b, ..., :3] = data.noisy[idx[b], lev[b], ys, xs]
+        f[b, ..., 3:] = data.guides[idx[b], ys, xs]
+        t[b] = data.target[idx[b], ys, xs]
+    return augment(f, t, rng)
+
+
+def train(net, ctx, data: Dataset, steps=3000, batch=16, lr=1e-3, crop=128, seed=0, log=None):
+    """Adam with a cosine learning-rate schedule down to 10 %."""
     rng = np.random.default_rng(seed)
-    n, h, w, _ = feats.shape
-    d_f = ctx.malloc(batch * h * w * 10 * 4)
-    d_t = ctx.malloc(batch * h * w * 3 * 4)
+    d_f = ctx.malloc(batch * crop * crop * 10 * 4)
+    d_t = ctx.malloc(batch * crop * crop * 3 * 4)
     losses = []
     t0 = time.perf_counter()
     for s in range(steps):
-        idx = rng.choice(n, size=batch, replace=n < batch)
-        f, t = augment(feats[idx], tgts[idx], rng)
+        f, t = sample_batch(data, batch, rng, crop)
         ctx.copy_to_device(d_f, f)
         ctx.copy_to_device(d_t, t)
         cur_lr = lr * (0.5 * (1 + np.cos(np.pi * s / steps)) * 0.9 + 0.1)
-        losses.append(net.train_step_device(batch, h, w, d_f, d_t, float(cur_lr)))
-        if log and (s + 1) % 100 == 0:
-            log(f"step {s + 1} loss {np.mean(losses[-100:]):.4f} ({time.perf_counter() - t0:.1f}s)")
+        losses.append(net.train_step_device(batch, crop, crop, d_f, d_t, float(cur_lr)))
+        if log and (s + 1) % 200 == 0:
+            log(f"step {s + 1} loss {np.mean(losses[-200:]):.4f} ({time.perf_counter() - t0:.1f}s)")
     ctx.synchronize()
     secs = time.perf_counter() - t0
     ctx.free(d_f)
@@ -125,45 +192,63 @@ def load_weights(net, path=WEIGHTS) -> bool:
     return True
 
 
-def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=4096, max_depth=8, seed=1234,
-                   nlm=True):
-    """Wall-clock (device) time for plain path tracing to reach relMSE <= target
-    against the gt_spp render, and for path tracing + denoiser."""
+def crossing(points, target):
+    """Interpolated cost at which relMSE reaches `target` from [(cost, relMSE)]
+    measured at increasing cost: log-log between the bracketing points, or a
+    1/cost extrapolation past the last point (None if never below and not
+    decreasing)."""
+    pts = [(c, r) for c, r in points if c > 0 and r > 0]
+    for (c0, r0), (c1, r1) in zip(pts, pts[1:]):
+        if r0 > target >= r1:
+            k = np.log(target / r0) / np.log(r1 / r0)
+            return float(np.exp(np.log(c0) + k * (np.log(c1) - np.log(c0))))
+    if pts and pts[0][1] <= target:
+        return float(pts[0][0])
+    if len(pts) >= 2 and pts[-1][1] < pts[-2][1]:
+        c, r = pts[-1]
+        return float(c * r / target)
+    return None
+
+
+def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65536, max_depth=8, seed=1234,
+                   plain_max=32768, den_levels=INPUT_LEVELS, nlm=True):
+    """Device time for plain path tracing to reach relMSE <= target against
+    a gt_spp render (passes disjoint from the others), and for path tracing +
+    guides + U-Net inference.  Times are interpolated between the measured
+    sample counts (relMSE of plain path tracing falls as 1/spp)."""
     fr = FrameRenderer(ctx, desc, width, height, max_depth, seed=seed)
-    gt, gt_ms = fr.mean(gt_spp, first_pass=1 << 22)
+    gt, gt_ms = fr.mean(gt_spp, first_pass=1 << 24)
     a, n, d, guide_ms = fr.guides()
-    # plain: progressive spp ladder, time of the render alone
     plain = []
     spp = 1
-    plain_spp = None
-    while spp <= gt_spp // 2:
-        img, ms = fr.mean(spp)
+    levels = []
+    while spp <= plain_max:
+        levels.append(spp)
+        spp *= 2
+    for spp, img, ms in fr.ladder(levels):
         r = rel_mse(img, gt)
         plain.append((spp, ms, r))
-        if r <= target and plain_spp is None:
-            plain_spp = (spp, ms)
+        if r < target / 2:
             break
-        spp *= 2
-    # refine the crossing with a linear-in-1/spp fit of the last two points
-    den = []
+    plain_ms = crossing([(ms, r) for _, ms, r in plain], target)
     d_f = ctx.malloc(width * height * 10 * 4)
     d_o = ctx.malloc(width * height * 3 * 4)
-    best = None
-    for spp in (1, 2, 4, 8, 16, 32, 64):
-        img, ms = fr.mean(spp)
-        f = features(img, a, n, d)[None]
-        ctx.copy_to_device(d_f, f)
+    den = []
+    for spp, img, ms in fr.ladder(den_levels):
+        ctx.copy_to_device(d_f, features(img, a, n, d)[None])
         ctx.synchronize()
         t = time.perf_counter()
         net.infer_device(1, height, width, d_f, d_o)
         ctx.synchronize()
         inf_ms = (time.perf_counter() - t) * 1e3
         out = ctx.download(d_o, (height, width, 3), np.float32)
-        r = rel_mse(out, gt)
-        total = ms + guide_ms + inf_ms
-        den.append((spp, ms, inf_ms, guide_ms, r))
-        if r <= target and best is None:
-            best = (spp, total)
+        den.append((spp, ms, inf_ms, rel_mse(out, gt)))
+    best = [(ms + guide_ms + inf, r) for spp, ms, inf, r in den]
+    den_ms = None
+    for tot, r in sorted(best):
+        if r <= target:
+            den_ms = tot
+            break
     nl = None
     if nlm:
         img, ms = fr.mean(16)
@@ -177,5 +262,5 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=40
     ctx.free(d_f)
     ctx.free(d_o)
     fr.close()
-    return dict(gt_spp=gt_spp, gt_ms=gt_ms, plain=plain, plain_hit=plain_spp, denoised=den, denoised_hit=best,
-                guide_ms=guide_ms, nlm=nl)
+    return dict(gt_spp=gt_spp, gt_ms=gt_ms, plain=plain, plain_ms=plain_ms, denoised=den, denoised_ms=den_ms,
+                guide_ms=guide_ms, best_denoised_relmse=min(r for *_, r in den), nlm=nl)

@@ -77,3 +77,16 @@ def test_training_reduces_the_loss(ctx):
     ctx.free(d_t)
     assert np.all(np.isfinite(losses))
     assert np.mean(losses[-5:]) < 0.5 * np.mean(losses[:3]), losses[::10]
+
+
+def test_miss_pixels_pass_through(ctx):
+    """Where the camera ray missed (depth guide 0) the output is the input
+    radiance, bit for bit."""
+    import salty_amd
+    net = salty_amd.Denoiser(ctx, seed=4)
+    feat, _ = case(7, 1, 32, 32)
+    feat[0, 5:9, 10:20, 9] = 0.0
+    out = net.infer(feat)
+    m = feat[..., 9] == 0
+    assert np.array_equal(out[m], feat[..., :3][m])
+    assert not np.array_equal(out[~m], feat[..., :3][~m])

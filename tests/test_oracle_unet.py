@@ -25,8 +25,13 @@ def make_case(seed=0, h=16, w=16):
 
 def test_unet_gradient_matches_finite_differences():
     p, feat, target = make_case()
-    loss, g, _ = unet.loss_and_grad(p, feat, target)
+    loss, g, out = unet.loss_and_grad(p, feat, target)
     assert np.isfinite(loss) and loss > 0
+    # the loss's denominator is held constant in the gradient: differentiate
+    # the loss with that denominator fixed at this point
+    den = np.expm1(out) ** 2
+    loss_f, g_f, _ = unet.loss_and_grad(p, feat, target, den)
+    assert loss_f == loss and np.array_equal(g_f, g)
     lay, n = unet.param_layout()
     rng = np.random.default_rng(1)
     idx = []
@@ -43,15 +48,25 @@ def test_unet_gradient_matches_finite_differences():
     for i in idx:
         q = p.copy()
         q[i] += h
-        lp = unet.loss_only(q, feat, target)
+        lp = unet.loss_only(q, feat, target, den)
         q[i] -= 2 * h
-        lm = unet.loss_only(q, feat, target)
+        lm = unet.loss_only(q, feat, target, den)
         fd = (lp - lm) / (2 * h)
         if abs(fd) < 1e-9 and abs(g[i]) < 1e-9:
             continue
         assert abs(fd - g[i]) <= 1e-4 * max(abs(fd), abs(g[i])) + 1e-9, (i, fd, g[i])
         checked += 1
     assert checked >= 25
+
+
+def test_loss_is_relative_squared_error():
+    """Closed form: with the head's output fixed, the loss is the mean of
+    (expm1(out) - target)^2 / (expm1(out)^2 + 1e-2)."""
+    p, feat, target = make_case(3, 8, 8)
+    loss, _, out = unet.loss_and_grad(p, feat, target)
+    pred = np.expm1(out)
+    ref = np.mean((pred - target) ** 2 / (pred ** 2 + 1e-2))
+    assert abs(loss - ref) <= 1e-12 * ref
 
 
 def test_padded_input_channels_have_zero_gradient():
@@ -61,3 +76,20 @@ def test_padded_input_channels_have_zero_gradient():
     wo, _, cin, cout = lay["e0a"]
     gw = g[wo:wo + 9 * cout * cin].reshape(9, cout, cin)
     assert np.all(gw[:, :, unet.IN_CH:] == 0) and np.any(gw[:, :, :unet.IN_CH] != 0)
+
+
+def test_miss_pixels_do_not_enter_the_loss():
+    """Pixels with depth guide 0 take no part: changing their target changes
+    neither the loss nor the gradient, and the loss of the rest is unchanged
+    by their presence."""
+    p, feat, target = make_case(4, 8, 8)
+    feat[0, 2:4, 3:6, 9] = 0.0
+    l0, g0, _ = unet.loss_and_grad(p, feat, target)
+    t2 = target.copy()
+    t2[0, 2:4, 3:6] += 5.0
+    l1, g1, _ = unet.loss_and_grad(p, feat, t2)
+    assert l0 == l1 and np.array_equal(g0, g1)
+    t3 = target.copy()
+    t3[0, 0, 0] += 5.0
+    l2, _, _ = unet.loss_and_grad(p, feat, t3)
+    assert l2 != l0
