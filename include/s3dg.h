@@ -301,6 +301,35 @@ int s3dg_render_guides_device(s3dg_scene *scene, const s3dg_render_config *cfg,
                               float *d_albedo, float *d_normal, float *d_depth);
 
 /* ---------------------------------------------------------------------------
+ * Scene sets and batched rendering (no reference counterpart: the reference
+ * renders one scene per process).  A set merges existing scenes of one
+ * context into one set of device arrays (the scenes stay valid and
+ * independent); s3dg_render_batch_device renders a list of frames of its
+ * members, each with its own resolution, passes, seed and sum buffer, with the
+ * paths of all frames sharing every launch.  A frame's sum equals that of
+ * s3dg_render_device on the member scene with the same settings.  From
+ * `common` only max_bounce_count, rr_start_depth, flags and passes_in_flight
+ * (the path capacity in units of the largest frame; 0 = automatic) are used.
+ * ------------------------------------------------------------------------- */
+typedef struct s3dg_scene_set s3dg_scene_set;
+
+typedef struct s3dg_frame_job {
+    uint32_t scene;            /* member index in the set                      */
+    int32_t  width;
+    int32_t  height;
+    int32_t  spp;
+    uint64_t seed;
+    int32_t  first_pass;
+    int32_t  reserved;
+    float   *d_accum_rgb;      /* device sum buffer, width*height*3 floats     */
+} s3dg_frame_job;
+
+int s3dg_scene_set_create(s3dg_context *ctx, uint32_t count, s3dg_scene *const *scenes, s3dg_scene_set **out);
+int s3dg_scene_set_destroy(s3dg_scene_set *set);
+int s3dg_render_batch_device(s3dg_scene_set *set, uint32_t frame_count, const s3dg_frame_job *frames,
+                             const s3dg_render_config *common, s3dg_render_stats *stats);
+
+/* ---------------------------------------------------------------------------
  * Post: replaces ToneMapper::Map (src/src/s3d_tonemapper.cpp:206-232) and the
  * sRGB/8-bit conversion of PathTracer::Capture (src/src/s3d_pt.cpp:137-171).
  * out_rgb (optional) receives the tonemapped float image, out_rgb8 (optional)

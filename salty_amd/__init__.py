@@ -42,6 +42,10 @@ _SIGNATURES = {
     "s3dg_render": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.POINTER(RenderStats)],
     "s3dg_render_device": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.POINTER(RenderStats)],
     "s3dg_render_guides_device": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_scene_set_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
+    "s3dg_scene_set_destroy": [C.c_void_p],
+    "s3dg_render_batch_device": [C.c_void_p, C.c_uint32, C.POINTER(abi.FrameJob), C.POINTER(RenderConfig),
+                                 C.POINTER(RenderStats)],
     "s3dg_tonemap": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],
     "s3dg_tonemap_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],
     "s3dg_nlm": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
@@ -255,6 +259,42 @@ class Scene:
         self.ctx.check(self.ctx.lib.s3dg_render_guides_device(self.handle, C.byref(cfg), C.c_void_p(d_albedo),
                                                               C.c_void_p(d_normal), C.c_void_p(d_depth)),
                        "s3dg_render_guides_device")
+
+
+class SceneSet:
+    """Scenes of one context merged for batched rendering (the scenes stay usable)."""
+
+    def __init__(self, ctx: Context, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[sc.handle.value for sc in self.scenes])
+        h = C.c_void_p()
+        ctx.check(ctx.lib.s3dg_scene_set_create(ctx.handle, len(self.scenes), arr, C.byref(h)),
+                  "s3dg_scene_set_create")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.ctx.lib.s3dg_scene_set_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render_batch_device(self, frames, max_depth: int, rr_start_depth: int = 0, flags: int = 0,
+                            passes_in_flight: int = 0) -> RenderStats:
+        """frames: sequence of (scene, width, height, spp, seed, first_pass, d_accum)."""
+        jobs = (abi.FrameJob * len(frames))()
+        for j, (scene, w, h, spp, seed, first_pass, d_acc) in enumerate(frames):
+            jobs[j] = abi.FrameJob(scene, w, h, spp, seed, first_pass, 0, d_acc)
+        cfg = RenderConfig(0, 0, max_depth, 0, 0, rr_start_depth, 0, flags, passes_in_flight)
+        st = RenderStats()
+        self.ctx.check(self.ctx.lib.s3dg_render_batch_device(self.handle, len(frames), jobs, C.byref(cfg),
+                                                             C.byref(st)), "s3dg_render_batch_device")
+        return st
 
 
 class Denoiser:

@@ -26,7 +26,20 @@ struct s3dg_context {
 
 namespace s3dg {
 
-// Device-side view of a scene, passed by value to kernels.
+// Per-scene part of the device view: where traversal starts, the lights and
+// camera.  A scene has one; a scene set (several scenes merged into one set of
+// arrays) has one per member, indexed by the scene number each path carries.
+struct SceneRec {
+    uint32_t root;          // top-level root node, or the mesh root of a single untransformed mesh
+    int32_t single_shape;   // shape index of that mesh (-1 with a top-level tree)
+    int32_t light_base;     // first light of the scene in the light array
+    int32_t light_count;
+    float env[3];
+    int32_t pad;
+    GCamera cam;
+};
+
+// Device-side view of a scene or a scene set, passed by value to kernels.
 struct SceneDev {
     const GNode *nodes;
     const GTri *tris;
@@ -35,14 +48,23 @@ struct SceneDev {
     const GMat *mats;
     const GLight *lights;
     const int32_t *inst_of_shape;   // top-level shape index -> instance record
-    uint32_t root;          // top-level root node, or the mesh root when single_mesh
-    int32_t single_mesh;    // 1: one untransformed mesh, traversal starts at its root
-    int32_t single_shape;   // shape index of that mesh
-    int32_t light_count;
-    float env[3];
+    const SceneRec *recs;
     int32_t stack_need;     // deepest stack the trees can require
     uint32_t tri_batch;     // lanes with pending triangles that keep a step testing
-    GCamera cam;
+};
+
+// Device buffers of the traversal and render kernels, owned by a scene or a
+// scene set and sized for the largest launch so far.
+struct Workspace {
+    void *d_gstack = nullptr;       // traversal stack overflow area (global memory)
+    size_t gstack_bytes = 0;
+    int32_t *d_error = nullptr;     // word 0: error flag; word 16: batch traversal work cursor
+    void *d_render = nullptr;       // path state
+    size_t render_bytes = 0;
+    void *d_batch = nullptr;        // batch descriptor and segment table of the render
+    size_t batch_bytes = 0;
+    hipGraphExec_t graph = nullptr; // captured bounce loop
+    uint64_t graph_key[8] = {};     // launch parameters the graph was captured with
 };
 
 struct SceneMeshInfo {
@@ -57,29 +79,52 @@ struct SceneMeshInfo {
 struct s3dg_scene {
     s3dg_context *ctx = nullptr;
     s3dg::SceneDev dev{};
+    s3dg::SceneRec rec{};
+    void *d_rec = nullptr;
     std::vector<s3dg::SceneMeshInfo> meshes;
     s3dg_scene_stats stats{};
     // device allocations
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
     void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr;
-    // traversal stack overflow area (global memory), sized per launch
-    void *d_gstack = nullptr;
-    size_t gstack_bytes = 0;
-    int32_t *d_error = nullptr;
-    // render state, sized for the largest frame rendered so far
-    void *d_render = nullptr;
-    size_t render_bytes = 0;
+    struct {
+        uint32_t nodes, tris, insts, mats, lights, shapes;   // element counts of the device arrays
+    } counts{};
+    s3dg::Workspace ws;
 };
 
 namespace s3dg {
 
+// One frame of a render: scene (index into SceneDev::recs), resolution,
+// passes, RNG key, first pass index and its RGB sum buffer in device memory.
+struct RenderJob {
+    uint32_t scene;
+    int32_t width, height, spp;
+    uint64_t seed;
+    int32_t first_pass;
+    float *d_accum;
+};
+
+int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const RenderJob *jobs, size_t njobs,
+                int32_t max_bounce, int32_t rr_start, uint32_t flags, uint32_t passes_in_flight,
+                s3dg_render_stats *stats);
 int set_error(s3dg_context *ctx, int code, const std::string &msg);
+int workspace_init(s3dg_context *ctx, Workspace &ws);
+void workspace_free(Workspace &ws);
+int ensure_gstack(s3dg_context *ctx, Workspace &ws, uint32_t threads, int32_t stack_need);
+int check_error_flag(s3dg_context *ctx, Workspace &ws);
+int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks);
+uint32_t batch_grid(uint32_t n);
 int hip_check(s3dg_context *ctx, hipError_t e, const char *what);
 int ensure_buffer(s3dg_context *ctx, void **ptr, size_t *have, size_t need);
 
 // Stack entries held in LDS per lane before spilling to global memory.
 constexpr int kLdsStack = 16;
 constexpr int kBlock = 256;
+
+// Global-memory stack entries per lane beyond the LDS ones.
+inline int32_t stack_capacity(int32_t stack_need) {
+    return stack_need - kLdsStack + 4 > 4 ? stack_need - kLdsStack + 4 : 4;
+}
 
 }  // namespace s3dg
 

@@ -25,12 +25,6 @@
 
 using namespace s3dg;
 
-namespace s3dg {
-uint32_t batch_grid(uint32_t n);
-int ensure_gstack(s3dg_scene *sc, uint32_t threads);
-int check_error_flag(s3dg_scene *sc);
-int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks);
-}  // namespace s3dg
 
 namespace {
 
@@ -46,7 +40,8 @@ struct PathBuf {
     float *swr, *swg, *swb;                 // W at the shading point
     float *sfr, *sfg, *sfb;                 // fs = base colour / pi
     float *scos, *sd2;                      // |n.l|, squared light distance
-    int32_t *slight;                        // light index
+    int32_t *slight;                        // light index (in the scene set's light array)
+    uint32_t *scene;                        // scene of the path (index into SceneDev::recs)
     uint32_t *q[2];                         // extension queues (ping-pong)
     uint32_t *qs;                           // shadow queue
     uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count,
@@ -55,8 +50,8 @@ struct PathBuf {
 };
 
 size_t path_bytes(uint32_t n) {
-    // 29 four-byte fields + 1 eight-byte field per path, three queues, counters
-    return (size_t)n * (29 * 4 + 8 + 3 * 4) + 4096;
+    // 30 four-byte fields + 1 eight-byte field per path, three queues, counters
+    return (size_t)n * (30 * 4 + 8 + 3 * 4) + 4096;
 }
 
 PathBuf carve(void *base, uint32_t n) {
@@ -77,12 +72,45 @@ PathBuf carve(void *base, uint32_t n) {
     b.hprim = (int32_t *)take((size_t)n * 4);
     b.hshape = (int32_t *)take((size_t)n * 4);
     b.slight = (int32_t *)take((size_t)n * 4);
+    b.scene = (uint32_t *)take((size_t)n * 4);
     b.q[0] = (uint32_t *)take((size_t)n * 4);
     b.q[1] = (uint32_t *)take((size_t)n * 4);
     b.qs = (uint32_t *)take((size_t)n * 4);
     return b;
 }
 size_t carve_bytes(uint32_t n) { return path_bytes(n) + 40 * 256; }
+
+// A batch of paths is a list of segments: consecutive passes of one frame
+// (a scene, its resolution, seed and accumulation buffer).  Path slot p of
+// segment g is pixel (p - path0) % npix of pass pass0 + (p - path0) / npix.
+struct Seg {
+    uint32_t path0;     // first path slot
+    uint32_t npix;      // width * height
+    uint32_t pix0;      // first accumulation thread (prefix of npix over the batch)
+    uint32_t scene;     // index into SceneDev::recs
+    int32_t width, height;
+    uint32_t pass0, passes;
+    uint64_t seed;
+    float *accum;       // RGB sum buffer of the frame
+};
+
+struct BatchDesc {
+    uint32_t nseg, paths, pixels, pad;
+};
+
+// Segment holding path slot (or accumulation thread) i: binary search on
+// path0 (or pix0); segments are sorted by both.
+template <bool BY_PIX>
+__device__ __forceinline__ const Seg &find_seg(const Seg *__restrict__ segs, uint32_t nseg, uint32_t i) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t start = BY_PIX ? segs[mid].pix0 : segs[mid].path0;
+        if (start <= i) lo = mid;
+        else hi = mid;
+    }
+    return segs[lo];
+}
 
 // Paths traced together when the caller leaves passes_in_flight at 0 (about
 // 2.3 GB of path state), and the most ever held at once.
@@ -110,9 +138,9 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *cursor) {
     return __shfl(base, 0);
 }
 
-__global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, int32_t W, int32_t H, uint64_t seed,
-                                                    uint32_t pass0, uint32_t npaths, uint32_t scene_id) {
-    const uint32_t npix = (uint32_t)W * (uint32_t)H;
+__global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const BatchDesc *__restrict__ bd,
+                                                    const Seg *__restrict__ segs) {
+    const uint32_t npaths = bd->paths, nseg = bd->nseg;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gtid == 0) {
         P.cnt[0] = npaths;
@@ -122,16 +150,19 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, int32_
         P.cnt[5] = 0;
     }
     for (uint32_t p = gtid; p < npaths; p += gridDim.x * blockDim.x) {
-        const uint32_t pix = p % npix;
-        const uint32_t pass = pass0 + p / npix;
-        const uint32_t x = pix % (uint32_t)W, y = pix / (uint32_t)W;
+        const Seg &g = find_seg<false>(segs, nseg, p);
+        const uint32_t local = p - g.path0;
+        const uint32_t pix = local % g.npix;
+        const uint32_t pass = g.pass0 + local / g.npix;
+        const uint32_t x = pix % (uint32_t)g.width, y = pix / (uint32_t)g.width;
         Pcg rng;
-        rng.seed(path_seed(seed, pix, pass, scene_id));
+        rng.seed(path_seed(g.seed, pix, pass, 0u));
         // TracePath pixel-centre coordinates (s3d_pt.cpp:326-328)
-        const float fx = (0.5f + (float)x) / (float)W - 0.5f;
-        const float fy = (0.5f + (float)y) / (float)H - 0.5f;
+        const float fx = (0.5f + (float)x) / (float)g.width - 0.5f;
+        const float fy = (0.5f + (float)y) / (float)g.height - 0.5f;
         V3 o, d;
-        camera_ray(S.cam, fx, fy, rng, o, d);
+        camera_ray(S.recs[g.scene].cam, fx, fy, rng, o, d);
+        P.scene[p] = g.scene;
         P.ox[p] = o.x; P.oy[p] = o.y; P.oz[p] = o.z;
         P.dx[p] = d.x; P.dy[p] = d.y; P.dz[p] = d.z;
         P.wr[p] = 1.0f; P.wg[p] = 1.0f; P.wb[p] = 1.0f;
@@ -167,7 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_
             r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
             r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
             st.sp = 0;
-            tr.start(S, r, kRefHitMin, kRefHitMax);
+            tr.start(S.recs[P.scene[p]], r, kRefHitMin, kRefHitMax);
             return true;
         },
         [&]() { return tr.step(S, st); },
@@ -201,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
             const int32_t prim = P.hprim[p];
             if (prim == -1) {
                 // miss: L += W * environment (SampleIBL(dir)*10 in the reference)
-                L = L + W * ld3(S.env);
+                L = L + W * ld3(S.recs[P.scene[p]].env);
             } else {
                 const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
                 const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
@@ -224,9 +255,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                 L = L + W * ld3(m.emissive);
                 Pcg rng;
                 rng.state = P.rng[p];
-                if (!m.delta && S.light_count > 0) {
+                const SceneRec &rec = S.recs[P.scene[p]];
+                if (!m.delta && rec.light_count > 0) {
                     // NextEventEstimation sampling half (s3d_pt.cpp:258-268)
-                    const uint32_t li = rng.u32() % (uint32_t)S.light_count;
+                    const uint32_t li = (uint32_t)rec.light_base + rng.u32() % (uint32_t)rec.light_count;
                     const GLight &lt = S.lights[li];
                     V3 lpos;
                     if (lt.type == LIGHT_SPHERE) {
@@ -317,7 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 
             if (!(tl >= 0.0f)) return false;
             // "closest hit is the light" == no other hit strictly before it
             st.sp = 0;
-            tr.start(S, r, kRefHitMin, nextafterf(tl, 0.0f));
+            tr.start(S.recs[P.scene[p]], r, kRefHitMin, nextafterf(tl, 0.0f));
             return true;
         },
         [&]() { return tr.step(S, st); },
@@ -355,18 +387,23 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 
 }
 
 // m_RenderTarget[idx] += L for each pass in turn (s3d_pt.cpp:330-332).
-__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, float *accum, uint32_t npix, uint32_t passes) {
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
-        float r = accum[3 * p + 0], g = accum[3 * p + 1], b = accum[3 * p + 2];
-        for (uint32_t s = 0; s < passes; ++s) {
-            const size_t q = (size_t)s * npix + p;
+__global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, const BatchDesc *__restrict__ bd,
+                                                        const Seg *__restrict__ segs) {
+    const uint32_t npixels = bd->pixels, nseg = bd->nseg;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < npixels; t += gridDim.x * blockDim.x) {
+        const Seg &g = find_seg<true>(segs, nseg, t);
+        const uint32_t pix = t - g.pix0;
+        float *acc = g.accum + 3 * (size_t)pix;
+        float r = acc[0], gg = acc[1], b = acc[2];
+        for (uint32_t s = 0; s < g.passes; ++s) {
+            const size_t q = (size_t)g.path0 + (size_t)s * g.npix + pix;
             r = r + P.lr[q];
-            g = g + P.lg[q];
+            gg = gg + P.lg[q];
             b = b + P.lb[q];
         }
-        accum[3 * p + 0] = r;
-        accum[3 * p + 1] = g;
-        accum[3 * p + 2] = b;
+        acc[0] = r;
+        acc[1] = gg;
+        acc[2] = b;
     }
 }
 
@@ -389,11 +426,12 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
         Pcg rng;
         rng.seed(path_seed(seed, p, pass, 0u));
         V3 o, d;
-        camera_ray(S.cam, (0.5f + (float)x) / (float)W - 0.5f, (0.5f + (float)y) / (float)H - 0.5f, rng, o, d);
+        camera_ray(S.recs[0].cam, (0.5f + (float)x) / (float)W - 0.5f, (0.5f + (float)y) / (float)H - 0.5f, rng, o,
+                   d);
         DRay r;
         r.ox = o.x; r.oy = o.y; r.oz = o.z;
         r.dx = d.x; r.dy = d.y; r.dz = d.z;
-        const DHit h = traverse<false, false>(S, r, kRefHitMin, kRefHitMax, st, nullptr);
+        const DHit h = traverse<false, false>(S, S.recs[0], r, kRefHitMin, kRefHitMax, st, nullptr);
         V3 a = v3(0, 0, 0), nn = v3(0, 0, 0);
         float t = 0.0f;
         if (h.prim != -1) {
@@ -420,36 +458,120 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
 
 }  // namespace
 
-extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg, float *d_accum,
-                                  s3dg_render_stats *stats) {
-    if (!sc || !cfg || !d_accum) return S3DG_ERR_INVALID;
-    s3dg_context *ctx = sc->ctx;
-    if (cfg->width <= 0 || cfg->height <= 0 || cfg->spp < 0 || cfg->max_bounce_count < 0 ||
-        (int64_t)cfg->width * cfg->height > (1ll << 30))
-        return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
-    S3DG_HIP(ctx, hipSetDevice(ctx->device));
-    const uint32_t n = (uint32_t)cfg->width * (uint32_t)cfg->height;
-    const uint32_t spp = (uint32_t)cfg->spp;
-    uint32_t flight = cfg->passes_in_flight;
-    if (flight == 0) flight = std::max(1u, kAutoPathsInFlight / n);
-    flight = std::max(1u, std::min({flight, std::max(spp, 1u), kMaxPathsInFlight / n}));
-    const uint32_t slots = n * flight;
-    int rc = ensure_buffer(ctx, &sc->d_render, &sc->render_bytes, carve_bytes(slots));
+namespace s3dg {
+
+// The render of a list of frames (RenderJob: scene, resolution, samples, seed,
+// accumulation buffer) as batches of path slots: each batch holds consecutive
+// passes of one or more frames, up to the path capacity; every batch runs
+// raygen, max_bounce x (extend, shade, shadow) and the accumulation.  Passes
+// of a frame are accumulated in pass order whatever the batching, so a frame's
+// sum does not depend on what else shares its batches.
+int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const RenderJob *jobs, size_t njobs,
+                int32_t max_bounce, int32_t rr_start, uint32_t flags, uint32_t passes_in_flight,
+                s3dg_render_stats *stats) {
+    uint64_t total_paths = 0;
+    uint32_t max_npix = 1;
+    for (size_t j = 0; j < njobs; ++j) {
+        const RenderJob &J = jobs[j];
+        if (J.width <= 0 || J.height <= 0 || J.spp < 0 || !J.d_accum || (int64_t)J.width * J.height > (1ll << 27))
+            return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
+        const uint32_t npix = (uint32_t)J.width * (uint32_t)J.height;
+        max_npix = std::max(max_npix, npix);
+        total_paths += (uint64_t)npix * (uint32_t)J.spp;
+    }
+    if (max_bounce < 0) return set_error(ctx, S3DG_ERR_INVALID, "bad bounce count");
+    uint64_t cap = passes_in_flight ? (uint64_t)passes_in_flight * max_npix : (uint64_t)kAutoPathsInFlight;
+    cap = std::min<uint64_t>(cap, kMaxPathsInFlight);
+    cap = std::min<uint64_t>(cap, std::max<uint64_t>(total_paths, 1));
+    cap = std::max<uint64_t>(cap, max_npix);
+    const uint32_t slots = (uint32_t)cap;
+
+    // ---- batches of segments ----------------------------------------------
+    std::vector<std::vector<Seg>> batches;
+    {
+        std::vector<Seg> cur;
+        uint32_t used = 0, pixels = 0;
+        auto flush = [&]() {
+            if (!cur.empty()) batches.push_back(cur);
+            cur.clear();
+            used = pixels = 0;
+        };
+        for (size_t j = 0; j < njobs; ++j) {
+            const RenderJob &J = jobs[j];
+            const uint32_t npix = (uint32_t)J.width * (uint32_t)J.height;
+            uint32_t done = 0;
+            while (done < (uint32_t)J.spp) {
+                const uint32_t fit = (slots - used) / npix;
+                if (fit == 0) {
+                    flush();
+                    continue;
+                }
+                const uint32_t k = std::min(fit, (uint32_t)J.spp - done);
+                Seg g{};
+                g.path0 = used;
+                g.npix = npix;
+                g.pix0 = pixels;
+                g.scene = J.scene;
+                g.width = J.width;
+                g.height = J.height;
+                g.pass0 = (uint32_t)J.first_pass + done;
+                g.passes = k;
+                g.seed = J.seed;
+                g.accum = J.d_accum;
+                cur.push_back(g);
+                used += k * npix;
+                pixels += npix;
+                done += k;
+            }
+        }
+        flush();
+    }
+    size_t max_seg = 1;
+    for (auto &b : batches) max_seg = std::max(max_seg, b.size());
+
+    // ---- buffers -------------------------------------------------------------
+    int rc = ensure_buffer(ctx, &ws.d_render, &ws.render_bytes, carve_bytes(slots));
     if (rc) return rc;
-    PathBuf P = carve(sc->d_render, slots);
+    PathBuf P = carve(ws.d_render, slots);
+    // device tables: the slot the kernels read (descriptor + segments), then
+    // every batch's descriptor and segments, copied into the slot in turn
+    std::vector<BatchDesc> descs(batches.size());
+    std::vector<Seg> all_segs;
+    std::vector<size_t> seg_off(batches.size());
+    for (size_t b = 0; b < batches.size(); ++b) {
+        const std::vector<Seg> &bs = batches[b];
+        descs[b].nseg = (uint32_t)bs.size();
+        descs[b].paths = bs.back().path0 + bs.back().passes * bs.back().npix;
+        descs[b].pixels = bs.back().pix0 + bs.back().npix;
+        seg_off[b] = all_segs.size();
+        all_segs.insert(all_segs.end(), bs.begin(), bs.end());
+    }
+    const size_t slot_bytes = 256 + ((max_seg * sizeof(Seg) + 255) & ~(size_t)255);
+    const size_t desc_bytes = (descs.size() * sizeof(BatchDesc) + 255) & ~(size_t)255;
+    rc = ensure_buffer(ctx, &ws.d_batch, &ws.batch_bytes,
+                       slot_bytes + desc_bytes + std::max<size_t>(all_segs.size(), 1) * sizeof(Seg));
+    if (rc) return rc;
+    BatchDesc *d_desc = (BatchDesc *)ws.d_batch;
+    Seg *d_segs = (Seg *)((char *)ws.d_batch + 256);
+    BatchDesc *d_all_desc = (BatchDesc *)((char *)ws.d_batch + slot_bytes);
+    Seg *d_all_segs = (Seg *)((char *)ws.d_batch + slot_bytes + desc_bytes);
+    S3DG_HIP(ctx, hipMemcpyAsync(d_all_desc, descs.data(), descs.size() * sizeof(BatchDesc), hipMemcpyHostToDevice,
+                                 ctx->stream));
+    if (!all_segs.empty())
+        S3DG_HIP(ctx, hipMemcpyAsync(d_all_segs, all_segs.data(), all_segs.size() * sizeof(Seg),
+                                     hipMemcpyHostToDevice, ctx->stream));
     const uint32_t grid = batch_grid(slots);
     uint32_t tgrid = 0;
     rc = resident_grid(ctx, reinterpret_cast<const void *>(&k_extend<false>), &tgrid);
     if (rc) return rc;
-    rc = ensure_gstack(sc, tgrid * kBlock);
+    rc = ensure_gstack(ctx, ws, tgrid * kBlock, S.stack_need);
     if (rc) return rc;
-    const int32_t gcap = std::max(4, sc->dev.stack_need - kLdsStack + 4);
-    uint2 *gs = (uint2 *)sc->d_gstack;
+    const int32_t gcap = stack_capacity(S.stack_need);
+    uint2 *gs = (uint2 *)ws.d_gstack;
 
-    const bool timed = (cfg->flags & S3DG_RENDER_TIME_KERNELS) != 0;
-    const bool count = (cfg->flags & S3DG_RENDER_COUNT_WORK) != 0;
-    const size_t batches = (spp + flight - 1) / flight;
-    const size_t launches = batches * (size_t)cfg->max_bounce_count;
+    const bool timed = (flags & S3DG_RENDER_TIME_KERNELS) != 0;
+    const bool count = (flags & S3DG_RENDER_COUNT_WORK) != 0;
+    const size_t launches = batches.size() * (size_t)max_bounce;
     std::vector<hipEvent_t> ev;
     if (timed) {
         ev.resize(4 * launches);
@@ -460,35 +582,65 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
     S3DG_HIP(ctx, hipEventCreate(&e1));
     S3DG_HIP(ctx, hipEventRecord(e0, ctx->stream));
     hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
-    size_t li = 0;
-    for (uint32_t s0 = 0; s0 < spp; s0 += flight) {
-        const uint32_t passes = std::min(flight, spp - s0);
-        const uint32_t pass0 = (uint32_t)cfg->first_pass + s0;
-        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cfg->width, cfg->height,
-                           cfg->seed, pass0, n * passes, 0u);
+
+    // one batch's kernels; li = index of its first bounce launch (timing)
+    auto launch_batch = [&](size_t li) {
+        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, S, P, d_desc, d_segs);
         int32_t cur = 0;
-        for (int32_t depth = 0; depth < cfg->max_bounce_count; ++depth, ++li) {
-            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li], ctx->stream));
+        for (int32_t depth = 0; depth < max_bounce; ++depth, ++li) {
+            if (timed) (void)hipEventRecord(ev[4 * li], ctx->stream);
             if (count)
-                hipLaunchKernelGGL(k_extend<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
-                                   gcap, sc->d_error, ctx->refill);
+                hipLaunchKernelGGL(k_extend<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, cur, gs, gcap,
+                                   ws.d_error, ctx->refill);
             else
-                hipLaunchKernelGGL(k_extend<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, gs,
-                                   gcap, sc->d_error, ctx->refill);
-            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 1], ctx->stream));
-            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, P, cur, depth,
-                               cfg->rr_start_depth);
-            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 2], ctx->stream));
+                hipLaunchKernelGGL(k_extend<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, cur, gs, gcap,
+                                   ws.d_error, ctx->refill);
+            if (timed) (void)hipEventRecord(ev[4 * li + 1], ctx->stream);
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, S, P, cur, depth, rr_start);
+            if (timed) (void)hipEventRecord(ev[4 * li + 2], ctx->stream);
             if (count)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
-                                   sc->d_error, ctx->refill);
+                hipLaunchKernelGGL(k_shadow<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, gs, gcap,
+                                   ws.d_error, ctx->refill);
             else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, sc->dev, P, gs, gcap,
-                                   sc->d_error, ctx->refill);
-            if (timed) S3DG_HIP(ctx, hipEventRecord(ev[4 * li + 3], ctx->stream));
+                hipLaunchKernelGGL(k_shadow<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, gs, gcap,
+                                   ws.d_error, ctx->refill);
+            if (timed) (void)hipEventRecord(ev[4 * li + 3], ctx->stream);
             cur ^= 1;
         }
-        hipLaunchKernelGGL(k_accumulate, dim3(batch_grid(n)), dim3(kBlock), 0, ctx->stream, P, d_accum, n, passes);
+        hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_desc, d_segs);
+    };
+
+    // Without instrumentation the batch's kernels are replayed from a HIP
+    // graph captured once for these launch parameters (the batch tables they
+    // read are rewritten before each replay).
+    const bool use_graph = !timed && !count;
+    if (use_graph) {
+        const uint64_t key[8] = {(uint64_t)(uintptr_t)ws.d_render, slots, (uint64_t)max_bounce,
+                                 (uint64_t)(uint32_t)rr_start, tgrid, (uint64_t)(uintptr_t)S.nodes,
+                                 (uint64_t)(uintptr_t)S.recs, ((uint64_t)ctx->refill << 32) | S.tri_batch};
+        if (!ws.graph || memcmp(key, ws.graph_key, sizeof(key)) != 0) {
+            if (ws.graph) {
+                (void)hipGraphExecDestroy(ws.graph);
+                ws.graph = nullptr;
+            }
+            hipGraph_t g = nullptr;
+            S3DG_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            launch_batch(0);
+            S3DG_HIP(ctx, hipStreamEndCapture(ctx->stream, &g));
+            const hipError_t ie = hipGraphInstantiate(&ws.graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            S3DG_HIP(ctx, ie);
+            memcpy(ws.graph_key, key, sizeof(key));
+        }
+    }
+    size_t li = 0;
+    for (size_t b = 0; b < batches.size(); ++b) {
+        S3DG_HIP(ctx, hipMemcpyAsync(d_desc, d_all_desc + b, sizeof(BatchDesc), hipMemcpyDeviceToDevice, ctx->stream));
+        S3DG_HIP(ctx, hipMemcpyAsync(d_segs, d_all_segs + seg_off[b], batches[b].size() * sizeof(Seg),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+        if (use_graph) S3DG_HIP(ctx, hipGraphLaunch(ws.graph, ctx->stream));
+        else launch_batch(li);
+        li += (size_t)max_bounce;
     }
     S3DG_HIP(ctx, hipGetLastError());
     S3DG_HIP(ctx, hipEventRecord(e1, ctx->stream));
@@ -514,7 +666,7 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
         memset(stats, 0, sizeof(*stats));
         stats->extension_rays = tot[0];
         stats->shadow_rays = tot[1];
-        stats->paths = (uint64_t)n * (uint64_t)cfg->spp;
+        stats->paths = total_paths;
         stats->milliseconds = ms;
         stats->extend_ms = ext_ms;
         stats->shadow_ms = sh_ms;
@@ -525,7 +677,29 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
         stats->shadow_nodes = tot[4];
         stats->shadow_tris = tot[5];
     }
-    return check_error_flag(sc);
+    return check_error_flag(ctx, ws);
+}
+
+}  // namespace s3dg
+
+extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg, float *d_accum,
+                                  s3dg_render_stats *stats) {
+    if (!sc || !cfg || !d_accum) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = sc->ctx;
+    if (cfg->width <= 0 || cfg->height <= 0 || cfg->spp < 0 || cfg->max_bounce_count < 0 ||
+        (int64_t)cfg->width * cfg->height > (1ll << 27))
+        return set_error(ctx, S3DG_ERR_INVALID, "bad render size");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    RenderJob job{};
+    job.scene = 0;
+    job.width = cfg->width;
+    job.height = cfg->height;
+    job.spp = cfg->spp;
+    job.seed = cfg->seed;
+    job.first_pass = cfg->first_pass;
+    job.d_accum = d_accum;
+    return render_jobs(ctx, sc->dev, sc->ws, &job, 1, cfg->max_bounce_count, cfg->rr_start_depth, cfg->flags,
+                       cfg->passes_in_flight, stats);
 }
 
 extern "C" int s3dg_render(s3dg_scene *sc, const s3dg_render_config *cfg, float *accum, s3dg_render_stats *stats) {
@@ -552,12 +726,12 @@ extern "C" int s3dg_render_guides_device(s3dg_scene *sc, const s3dg_render_confi
     S3DG_HIP(ctx, hipSetDevice(ctx->device));
     const uint32_t n = (uint32_t)cfg->width * (uint32_t)cfg->height;
     const uint32_t grid = batch_grid(n);
-    int rc = ensure_gstack(sc, grid * kBlock);
+    int rc = ensure_gstack(ctx, sc->ws, grid * kBlock, sc->dev.stack_need);
     if (rc) return rc;
-    const int32_t gcap = std::max(4, sc->dev.stack_need - kLdsStack + 4);
+    const int32_t gcap = stack_capacity(sc->dev.stack_need);
     hipLaunchKernelGGL(k_guides, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, cfg->width, cfg->height,
-                       cfg->seed, (uint32_t)cfg->first_pass, d_albedo, d_normal, d_depth, (uint2 *)sc->d_gstack,
-                       gcap, sc->d_error);
+                       cfg->seed, (uint32_t)cfg->first_pass, d_albedo, d_normal, d_depth, (uint2 *)sc->ws.d_gstack,
+                       gcap, sc->ws.d_error);
     S3DG_HIP(ctx, hipGetLastError());
-    return check_error_flag(sc);
+    return check_error_flag(ctx, sc->ws);
 }

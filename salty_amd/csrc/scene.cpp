@@ -376,9 +376,8 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     const bool single = d->shape_count == 1 && d->shapes[0].type == S3DG_SHAPE_MESH;
     std::vector<int32_t> inst_pos(d->shape_count, 0);
     if (single) {
-        dv.single_mesh = 1;
-        dv.single_shape = 0;
-        dv.root = inst[0].blas_root;
+        sc->rec.single_shape = 0;
+        sc->rec.root = inst[0].blas_root;
         inst_dev = inst;
         inst_pos[0] = 0;
     } else {
@@ -391,10 +390,9 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         }
         Aabb all = top.nodes[top.root].box;
         comp.max_depth = 0;
-        dv.root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp);
+        sc->rec.root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp);
         tlas_depth = comp.max_depth;
-        dv.single_mesh = 0;
-        dv.single_shape = -1;
+        sc->rec.single_shape = -1;
         sc->stats.tlas_nodes = 0;
     }
 
@@ -452,7 +450,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     // ---- camera: PinholeCamera::Update / ThinLensCamera::Update ----------
     {
         const s3dg_camera_desc &c = d->camera;
-        GCamera &g = dv.cam;
+        GCamera &g = sc->rec.cam;
         memset(&g, 0, sizeof(g));
         const F3 pos = f3(c.position), target = f3(c.target), up = f3(c.upward);
         const F3 dir = unit(sub(target, pos));
@@ -468,31 +466,33 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         const F3 tp = sub(target, pos);
         g.focal_distance = sqrtf((tp.x * tp.x) + (tp.y * tp.y) + (tp.z * tp.z));
     }
-    for (int a = 0; a < 3; ++a) dv.env[a] = d->environment[a];
-    dv.light_count = (int32_t)d->light_count;
+    for (int a = 0; a < 3; ++a) sc->rec.env[a] = d->environment[a];
+    sc->rec.light_count = (int32_t)d->light_count;
+    sc->rec.light_base = 0;
     // per BLAS level: one child group and one leaf group; per TLAS level: one
     // child group and up to eight shapes; one restore entry
     dv.stack_need = 2 * blas_depth + 9 * tlas_depth + 4;
     dv.tri_batch = ctx->tri_batch;
 
+    sc->counts.nodes = (uint32_t)comp.nodes.size();
+    sc->counts.tris = (uint32_t)tris.size();
+    sc->counts.insts = (uint32_t)inst_dev.size();
+    sc->counts.mats = (uint32_t)mats.size();
+    sc->counts.lights = (uint32_t)lights.size();
+    sc->counts.shapes = (uint32_t)inst_pos.size();
     size_t total = 0;
     int rc;
     if ((rc = upload(ctx, &sc->d_nodes, comp.nodes, total)) || (rc = upload(ctx, &sc->d_tris, tris, total)) ||
         (rc = upload(ctx, &sc->d_shade, shade, total)) || (rc = upload(ctx, &sc->d_insts, inst_dev, total)) ||
         (rc = upload(ctx, &sc->d_mats, mats, total)) || (rc = upload(ctx, &sc->d_lights, lights, total)) ||
-        (rc = upload(ctx, &sc->d_inst_of_shape, inst_pos, total))) {
+        (rc = upload(ctx, &sc->d_inst_of_shape, inst_pos, total)) ||
+        (rc = upload(ctx, &sc->d_rec, std::vector<SceneRec>{sc->rec}, total))) {
         s3dg_scene_destroy(sc);
         return rc;
     }
-    he = hipMalloc((void **)&sc->d_error, 256);
-    if (he != hipSuccess) {
+    if ((rc = workspace_init(ctx, sc->ws))) {
         s3dg_scene_destroy(sc);
-        return hip_check(ctx, he, "hipMalloc(error flag)");
-    }
-    he = hipMemset(sc->d_error, 0, 256);
-    if (he != hipSuccess) {
-        s3dg_scene_destroy(sc);
-        return hip_check(ctx, he, "hipMemset(error flag)");
+        return rc;
     }
     dv.nodes = (const GNode *)sc->d_nodes;
     dv.tris = (const GTri *)sc->d_tris;
@@ -501,6 +501,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     dv.mats = (const GMat *)sc->d_mats;
     dv.lights = (const GLight *)sc->d_lights;
     dv.inst_of_shape = (const int32_t *)sc->d_inst_of_shape;
+    dv.recs = (const SceneRec *)sc->d_rec;
 
     sc->stats.triangles = tris.size();
     sc->stats.blas_nodes = comp.nodes.size();
@@ -514,10 +515,11 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
 
 extern "C" int s3dg_scene_destroy(s3dg_scene *sc) {
     if (!sc) return S3DG_ERR_INVALID;
-    void *ptrs[] = {sc->d_nodes, sc->d_tris, sc->d_shade, sc->d_insts, sc->d_mats,
-                    sc->d_lights, sc->d_inst_of_shape, sc->d_gstack, sc->d_error, sc->d_render};
+    void *ptrs[] = {sc->d_nodes, sc->d_tris, sc->d_shade, sc->d_insts, sc->d_mats, sc->d_lights,
+                    sc->d_inst_of_shape, sc->d_rec};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    workspace_free(sc->ws);
     delete sc;
     return S3DG_OK;
 }

@@ -1,0 +1,237 @@
+// sceneset.hip -- several scenes merged into one set of device arrays, and
+// the batched render over a list of frames of those scenes.
+//
+// No counterpart in the reference (it renders one scene per process).  The
+// member scenes' nodes, triangles, shading records, top-level records,
+// materials and lights are copied one after another into merged arrays; the
+// indices stored inside them (child and triangle ranges of the nodes, tree
+// roots, shape numbers, materials, light triangles) are moved by the member's
+// offsets on the device.  Each member keeps a SceneRec (root, lights, camera,
+// environment); a path carries its member's number, so one traversal launch
+// serves rays of every member.  Within a member the order of shapes and
+// triangles is unchanged, so ties resolve as in the member's own render and a
+// frame rendered in a batch has the same sum as rendered alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "internal.h"
+
+using namespace s3dg;
+
+struct s3dg_scene_set {
+    s3dg_context *ctx = nullptr;
+    SceneDev dev{};
+    std::vector<SceneRec> recs;
+    void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
+    void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr, *d_recs = nullptr;
+    Workspace ws;
+};
+
+namespace {
+
+__global__ void k_rebase_nodes(GNode *__restrict__ nodes, uint32_t n, uint32_t node_off, uint32_t tri_off,
+                               uint32_t inst_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        GNode &g = nodes[i];
+        if (g.imask) g.child_base += node_off;
+        const uint32_t off = (g.flags & 1u) ? inst_off : tri_off;
+        for (int s = 0; s < 8; ++s)
+            if (g.cnt[s] && !((g.imask >> s) & 1u)) g.leaf_start[s] += off;
+    }
+}
+
+__global__ void k_rebase_insts(GInst *__restrict__ insts, uint32_t n, uint32_t node_off, int32_t shape_off,
+                               int32_t mat_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        GInst &g = insts[i];
+        if (g.type != INST_SPHERE) g.blas_root += node_off;
+        g.shape += shape_off;
+        if (g.material >= 0) g.material += mat_off;
+    }
+}
+
+__global__ void k_rebase_shade(GTriShade *__restrict__ shade, uint32_t n, int32_t mat_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        shade[i].material += mat_off;
+}
+
+__global__ void k_rebase_lights(GLight *__restrict__ lights, uint32_t n, int32_t tri_off, int32_t inst_off,
+                                int32_t shape_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        GLight &g = lights[i];
+        if (g.type == LIGHT_SPHERE) {
+            g.tri[0] += inst_off;
+        } else {
+            for (int k = 0; k < g.count; ++k) g.tri[k] += tri_off;
+        }
+        g.shape += shape_off;
+    }
+}
+
+__global__ void k_add_i32(int32_t *__restrict__ a, uint32_t n, int32_t off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] += off;
+}
+
+unsigned grid_for(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 4096u)); }
+
+void free_set(s3dg_scene_set *set) {
+    void *ptrs[] = {set->d_nodes, set->d_tris, set->d_shade, set->d_insts,
+                    set->d_mats, set->d_lights, set->d_inst_of_shape, set->d_recs};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    workspace_free(set->ws);
+    delete set;
+}
+
+}  // namespace
+
+extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *const *scenes,
+                                     s3dg_scene_set **out) {
+    if (!ctx || !scenes || !out || n == 0) return S3DG_ERR_INVALID;
+    *out = nullptr;
+    for (uint32_t k = 0; k < n; ++k)
+        if (!scenes[k] || scenes[k]->ctx != ctx) return set_error(ctx, S3DG_ERR_INVALID, "scene of another context");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    s3dg_scene_set *set = new (std::nothrow) s3dg_scene_set();
+    if (!set) return set_error(ctx, S3DG_ERR_NOMEM, "host allocation");
+    set->ctx = ctx;
+    // element offsets of every member
+    struct Off {
+        uint32_t nodes, tris, insts, mats, lights, shapes;
+    };
+    std::vector<Off> off(n);
+    Off tot{0, 0, 0, 0, 0, 0};
+    int32_t stack_need = 4;
+    for (uint32_t k = 0; k < n; ++k) {
+        const auto &c = scenes[k]->counts;
+        off[k] = tot;
+        tot.nodes += c.nodes;
+        tot.tris += c.tris;
+        tot.insts += c.insts;
+        tot.mats += c.mats;
+        tot.lights += c.lights;
+        tot.shapes += c.shapes;
+        stack_need = std::max(stack_need, scenes[k]->dev.stack_need);
+    }
+    auto alloc = [&](void **p, size_t bytes) -> hipError_t { return hipMalloc(p, std::max<size_t>(bytes, 256)); };
+    hipError_t e = hipSuccess;
+    if ((e = alloc(&set->d_nodes, (size_t)tot.nodes * sizeof(GNode))) != hipSuccess ||
+        (e = alloc(&set->d_tris, (size_t)tot.tris * sizeof(GTri))) != hipSuccess ||
+        (e = alloc(&set->d_shade, (size_t)tot.tris * sizeof(GTriShade))) != hipSuccess ||
+        (e = alloc(&set->d_insts, (size_t)tot.insts * sizeof(GInst))) != hipSuccess ||
+        (e = alloc(&set->d_mats, (size_t)tot.mats * sizeof(GMat))) != hipSuccess ||
+        (e = alloc(&set->d_lights, (size_t)tot.lights * sizeof(GLight))) != hipSuccess ||
+        (e = alloc(&set->d_inst_of_shape, (size_t)tot.shapes * sizeof(int32_t))) != hipSuccess ||
+        (e = alloc(&set->d_recs, (size_t)n * sizeof(SceneRec))) != hipSuccess) {
+        free_set(set);
+        return hip_check(ctx, e, "hipMalloc(scene set)");
+    }
+    int rc = workspace_init(ctx, set->ws);
+    if (rc) {
+        free_set(set);
+        return rc;
+    }
+    hipStream_t s = ctx->stream;
+    set->recs.resize(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        const s3dg_scene *sc = scenes[k];
+        const auto &c = sc->counts;
+        const Off &o = off[k];
+        GNode *nodes = (GNode *)set->d_nodes + o.nodes;
+        GTriShade *shade = (GTriShade *)set->d_shade + o.tris;
+        GInst *insts = (GInst *)set->d_insts + o.insts;
+        GLight *lights = (GLight *)set->d_lights + o.lights;
+        int32_t *ios = (int32_t *)set->d_inst_of_shape + o.shapes;
+        struct {
+            void *dst;
+            const void *src;
+            size_t bytes;
+        } copies[] = {{nodes, sc->d_nodes, (size_t)c.nodes * sizeof(GNode)},
+                      {(GTri *)set->d_tris + o.tris, sc->d_tris, (size_t)c.tris * sizeof(GTri)},
+                      {shade, sc->d_shade, (size_t)c.tris * sizeof(GTriShade)},
+                      {insts, sc->d_insts, (size_t)c.insts * sizeof(GInst)},
+                      {(GMat *)set->d_mats + o.mats, sc->d_mats, (size_t)c.mats * sizeof(GMat)},
+                      {lights, sc->d_lights, (size_t)c.lights * sizeof(GLight)},
+                      {ios, sc->d_inst_of_shape, (size_t)c.shapes * sizeof(int32_t)}};
+        for (auto &cp : copies)
+            if (cp.bytes) {
+                e = hipMemcpyAsync(cp.dst, cp.src, cp.bytes, hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) {
+                    free_set(set);
+                    return hip_check(ctx, e, "hipMemcpyAsync(scene set)");
+                }
+            }
+        if (c.nodes)
+            hipLaunchKernelGGL(k_rebase_nodes, dim3(grid_for(c.nodes)), dim3(256), 0, s, nodes, c.nodes, o.nodes,
+                               o.tris, o.insts);
+        if (c.insts)
+            hipLaunchKernelGGL(k_rebase_insts, dim3(grid_for(c.insts)), dim3(256), 0, s, insts, c.insts, o.nodes,
+                               (int32_t)o.shapes, (int32_t)o.mats);
+        if (c.tris)
+            hipLaunchKernelGGL(k_rebase_shade, dim3(grid_for(c.tris)), dim3(256), 0, s, shade, c.tris,
+                               (int32_t)o.mats);
+        if (c.lights)
+            hipLaunchKernelGGL(k_rebase_lights, dim3(grid_for(c.lights)), dim3(256), 0, s, lights, c.lights,
+                               (int32_t)o.tris, (int32_t)o.insts, (int32_t)o.shapes);
+        if (c.shapes)
+            hipLaunchKernelGGL(k_add_i32, dim3(grid_for(c.shapes)), dim3(256), 0, s, ios, c.shapes,
+                               (int32_t)o.insts);
+        SceneRec r = sc->rec;
+        r.root += o.nodes;
+        if (r.single_shape >= 0) r.single_shape += (int32_t)o.shapes;
+        r.light_base = (int32_t)o.lights;
+        set->recs[k] = r;
+    }
+    e = hipMemcpyAsync(set->d_recs, set->recs.data(), n * sizeof(SceneRec), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        free_set(set);
+        return hip_check(ctx, e, "scene set merge");
+    }
+    SceneDev &dv = set->dev;
+    dv.nodes = (const GNode *)set->d_nodes;
+    dv.tris = (const GTri *)set->d_tris;
+    dv.shade = (const GTriShade *)set->d_shade;
+    dv.insts = (const GInst *)set->d_insts;
+    dv.mats = (const GMat *)set->d_mats;
+    dv.lights = (const GLight *)set->d_lights;
+    dv.inst_of_shape = (const int32_t *)set->d_inst_of_shape;
+    dv.recs = (const SceneRec *)set->d_recs;
+    dv.stack_need = stack_need;
+    dv.tri_batch = ctx->tri_batch;
+    *out = set;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_scene_set_destroy(s3dg_scene_set *set) {
+    if (!set) return S3DG_ERR_INVALID;
+    (void)hipSetDevice(set->ctx->device);
+    free_set(set);
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_render_batch_device(s3dg_scene_set *set, uint32_t njobs, const s3dg_frame_job *jobs,
+                                        const s3dg_render_config *common, s3dg_render_stats *stats) {
+    if (!set || !common || (njobs && !jobs)) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = set->ctx;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    std::vector<RenderJob> rj(njobs);
+    for (uint32_t j = 0; j < njobs; ++j) {
+        const s3dg_frame_job &f = jobs[j];
+        if (f.scene >= set->recs.size()) return set_error(ctx, S3DG_ERR_INVALID, "frame names no scene of the set");
+        rj[j].scene = f.scene;
+        rj[j].width = f.width;
+        rj[j].height = f.height;
+        rj[j].spp = f.spp;
+        rj[j].seed = f.seed;
+        rj[j].first_pass = f.first_pass;
+        rj[j].d_accum = f.d_accum_rgb;
+    }
+    return render_jobs(ctx, set->dev, set->ws, rj.data(), rj.size(), common->max_bounce_count,
+                       common->rr_start_depth, common->flags, common->passes_in_flight, stats);
+}

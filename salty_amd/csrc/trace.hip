@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const s3dg_ray *__
             r.ox = a.x; r.oy = a.y; r.oz = a.z;
             r.dx = b.x; r.dy = b.y; r.dz = b.z;
             st.sp = 0;
-            tr.start(S, r, a.w, b.w);
+            tr.start(S.recs[0], r, a.w, b.w);
             return true;
         },
         [&]() { return tr.step(S, st); },
@@ -146,11 +146,9 @@ __global__ void k_clear_u32(uint32_t *p) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
-int stack_capacity(const s3dg_scene *sc) { return std::max(4, sc->dev.stack_need - kLdsStack + 4); }
-
-// Work cursor of the batch kernel: a word of the scene's small status block
-// (the error flag lives at its start).
-uint32_t *trace_cursor(s3dg_scene *sc) { return reinterpret_cast<uint32_t *>(sc->d_error) + 16; }
+// Work cursor of the batch kernel: a word of the workspace's small status
+// block (the error flag lives at its start).
+uint32_t *trace_cursor(Workspace &ws) { return reinterpret_cast<uint32_t *>(ws.d_error) + 16; }
 
 }  // namespace
 
@@ -172,18 +170,32 @@ int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks) {
     return S3DG_OK;
 }
 
-int ensure_gstack(s3dg_scene *sc, uint32_t threads) {
-    const size_t need = (size_t)threads * stack_capacity(sc) * sizeof(uint2);
-    return ensure_buffer(sc->ctx, &sc->d_gstack, &sc->gstack_bytes, need);
+int workspace_init(s3dg_context *ctx, Workspace &ws) {
+    S3DG_HIP(ctx, hipMalloc((void **)&ws.d_error, 256));
+    S3DG_HIP(ctx, hipMemset(ws.d_error, 0, 256));
+    return S3DG_OK;
 }
 
-int check_error_flag(s3dg_scene *sc) {
+void workspace_free(Workspace &ws) {
+    if (ws.graph) (void)hipGraphExecDestroy(ws.graph);
+    void *ptrs[] = {ws.d_gstack, ws.d_error, ws.d_render, ws.d_batch};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    ws = Workspace{};
+}
+
+int ensure_gstack(s3dg_context *ctx, Workspace &ws, uint32_t threads, int32_t stack_need) {
+    const size_t need = (size_t)threads * stack_capacity(stack_need) * sizeof(uint2);
+    return ensure_buffer(ctx, &ws.d_gstack, &ws.gstack_bytes, need);
+}
+
+int check_error_flag(s3dg_context *ctx, Workspace &ws) {
     int32_t flag = 0;
-    S3DG_HIP(sc->ctx, hipMemcpyAsync(&flag, sc->d_error, sizeof(flag), hipMemcpyDeviceToHost, sc->ctx->stream));
-    S3DG_HIP(sc->ctx, hipStreamSynchronize(sc->ctx->stream));
+    S3DG_HIP(ctx, hipMemcpyAsync(&flag, ws.d_error, sizeof(flag), hipMemcpyDeviceToHost, ctx->stream));
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (flag) {
-        (void)hipMemsetAsync(sc->d_error, 0, sizeof(int32_t), sc->ctx->stream);
-        return set_error(sc->ctx, S3DG_ERR_UNSUPPORTED, "traversal stack overflow");
+        (void)hipMemsetAsync(ws.d_error, 0, sizeof(int32_t), ctx->stream);
+        return set_error(ctx, S3DG_ERR_UNSUPPORTED, "traversal stack overflow");
     }
     return S3DG_OK;
 }
@@ -196,27 +208,27 @@ int launch_trace(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mod
     int rc = resident_grid(ctx, reinterpret_cast<const void *>(&k_trace<false, false>), &grid);
     if (rc) return rc;
     grid = std::min(grid, (n + kBlock - 1) / kBlock);
-    rc = ensure_gstack(sc, grid * kBlock);
+    rc = ensure_gstack(ctx, sc->ws, grid * kBlock, sc->dev.stack_need);
     if (rc) return rc;
-    uint2 *gs = (uint2 *)sc->d_gstack;
-    const int32_t cap = stack_capacity(sc);
-    uint32_t *cursor = trace_cursor(sc);
+    uint2 *gs = (uint2 *)sc->ws.d_gstack;
+    const int32_t cap = stack_capacity(sc->dev.stack_need);
+    uint32_t *cursor = trace_cursor(sc->ws);
     const uint32_t refill = ctx->refill;
     hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, ctx->stream, cursor);
     if (mode == S3DG_TRACE_ANY) {
         if (d_counters)
             hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
+                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
         else
             hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
+                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
     } else {
         if (d_counters)
             hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
+                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
         else
             hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->d_error, d_counters, cursor, refill);
+                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
     }
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
@@ -231,7 +243,7 @@ extern "C" int s3dg_trace_device(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_
     S3DG_HIP(sc->ctx, hipSetDevice(sc->ctx->device));
     int rc = launch_trace(sc, d_rays, n, mode, d_hits, d_shapes, nullptr);
     if (rc) return rc;
-    return check_error_flag(sc);
+    return check_error_flag(sc->ctx, sc->ws);
 }
 
 // Variant that also counts node visits and triangle tests (counters[0], [1]),
@@ -242,7 +254,7 @@ extern "C" int s3dg_trace_counted_device(s3dg_scene *sc, const s3dg_ray *d_rays,
     S3DG_HIP(sc->ctx, hipSetDevice(sc->ctx->device));
     int rc = launch_trace(sc, d_rays, n, mode, d_hits, nullptr, (unsigned long long *)d_counters);
     if (rc) return rc;
-    return check_error_flag(sc);
+    return check_error_flag(sc->ctx, sc->ws);
 }
 
 extern "C" int s3dg_trace(s3dg_scene *sc, const s3dg_ray *rays, uint32_t n, int32_t mode, s3dg_hit *hits,

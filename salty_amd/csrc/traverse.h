@@ -234,7 +234,8 @@ struct Tracer {
     bool have_node;
     uint32_t visits, tests;
 
-    __device__ __forceinline__ void start(const SceneDev &S, const DRay &r, float t0, float t1) {
+    // Starts a ray at the root of scene `rec`.
+    __device__ __forceinline__ void start(const SceneRec &rec, const DRay &r, float t0, float t1) {
         wray = r;
         ray = r;
         pre = precompute(r);
@@ -243,8 +244,8 @@ struct Tracer {
         tmax = t1;
         lo_cull = cull_lo(t0);
         bound = cull_hi(t1);
-        node = S.root;
-        cur_shape = S.single_shape;
+        node = rec.root;
+        cur_shape = rec.single_shape;
         have_node = true;
         tri_cur = tri_end = 0;
         leaf_mask = 0;
@@ -482,11 +483,11 @@ struct Tracer {
 
 // Whole traversal of one ray (for kernels without lane refill).
 template <bool ANY, bool COUNT>
-__device__ __forceinline__ DHit traverse(const SceneDev &S, const DRay &r, float tmin, float tmax, Stack &st,
-                                         unsigned long long *counters) {
+__device__ __forceinline__ DHit traverse(const SceneDev &S, const SceneRec &rec, const DRay &r, float tmin,
+                                         float tmax, Stack &st, unsigned long long *counters) {
     Tracer<ANY, COUNT> tr;
     st.sp = 0;
-    tr.start(S, r, tmin, tmax);
+    tr.start(rec, r, tmin, tmax);
     while (!tr.step(S, st)) {
     }
     tr.count(counters);

@@ -125,57 +125,51 @@ def make_dataset(ctx, seeds, width=256, height=256, levels=INPUT_LEVELS, spp_ref
     return Dataset(np.stack(noisy), np.stack(guides), np.stack(tgts), levels)
 
 
-def augment(f, t, rng):
-    """Random mirror in x and in y (the normal's component changes sign)."""
-    if rng.random() < 0.5:
-        f, t = f[:, :, ::-1], t[:, :, ::-1]
-        f = f.copy()
-        f[..., 6] = -f[..., 6]
-    if rng.random() < 0.5:
-        f, t = f[:, ::-1], t[:, ::-1]
-        f = f.copy()
-        f[..., 7] = -f[..., 7]
-    return np.ascontiguousarray(f), np.ascontiguousarray(t)
-
-
-def sample_batch(data: Dataset, batch, rng, crop=None):
-    """A batch of (features, target) with a random input level per element and
-    an optional random square crop."""
+def train(net, ctx, data: Dataset, steps=3000, batch=16, lr=1e-3, crop=None, seed=0, log=None):
+    """Adam with a cosine learning-rate schedule down to 10 %.  The dataset is
+    held in device memory (through torch, which only moves and indexes it);
+    each step takes `batch` random (scene, input level) pairs, a random crop of
+    side `crop` (whole frames when None) and random mirrors in x and y (the
+    normal's component changes sign)."""
+    import torch
+    dev = torch.device("cuda")
+    noisy = torch.from_numpy(data.noisy).to(dev)
+    guides = torch.from_numpy(data.guides).to(dev)
+    target = torch.from_numpy(data.target).to(dev)
     n, L, h, w, _ = data.noisy.shape
-    idx = rng.integers(n, size=batch)
-    lev = rng.integers(L, size=batch)
     c = crop or h
-    y0 = rng.integers(h - c + 1, size=batch) if c < h else np.zeros(batch, int)
-    x0 = rng.integers(w - c + 1, size=batch) if c < w else np.zeros(batch, int)
-    f = np.empty((batch, c, c, 10), np.float32)
-    t = np.empty((batch, c, c, 3), np.float32)
-    for b in range(batch):
-        ys, xs = slice(y0[b], y0[b] + c), slice(x0[b], x0[b] + c)
-        f[b, ..., :3] = data.noisy[idx[b], lev[b], ys, xs]
-        f[b, ..., 3:] = data.guides[idx[b], ys, xs]
-        t[b] = data.target[idx[b], ys, xs]
-    return augment(f, t, rng)
-
-
-def train(net, ctx, data: Dataset, steps=3000, batch=16, lr=1e-3, crop=128, seed=0, log=None):
-    """Adam with a cosine learning-rate schedule down to 10 %."""
-    rng = np.random.default_rng(seed)
-    d_f = ctx.malloc(batch * crop * crop * 10 * 4)
-    d_t = ctx.malloc(batch * crop * crop * 3 * 4)
+    gen = torch.Generator().manual_seed(seed)
+    sign_x = torch.ones(7, device=dev)
+    sign_x[3] = -1.0
+    sign_y = torch.ones(7, device=dev)
+    sign_y[4] = -1.0
     losses = []
     t0 = time.perf_counter()
     for s in range(steps):
-        f, t = sample_batch(data, batch, rng, crop)
-        ctx.copy_to_device(d_f, f)
-        ctx.copy_to_device(d_t, t)
+        idx = torch.randint(n, (batch,), generator=gen).to(dev)
+        lev = torch.randint(L, (batch,), generator=gen).to(dev)
+        y0 = int(torch.randint(h - c + 1, (1,), generator=gen)) if c < h else 0
+        x0 = int(torch.randint(w - c + 1, (1,), generator=gen)) if c < w else 0
+        f_rgb = noisy[idx, lev, y0:y0 + c, x0:x0 + c]
+        f_g = guides[idx, y0:y0 + c, x0:x0 + c]
+        t = target[idx, y0:y0 + c, x0:x0 + c]
+        fx = (torch.rand(batch, generator=gen) < 0.5).to(dev).view(-1, 1, 1, 1)
+        fy = (torch.rand(batch, generator=gen) < 0.5).to(dev).view(-1, 1, 1, 1)
+        f_rgb = torch.where(fx, f_rgb.flip(2), f_rgb)
+        f_g = torch.where(fx, f_g.flip(2) * sign_x, f_g)
+        t = torch.where(fx, t.flip(2), t)
+        f_rgb = torch.where(fy, f_rgb.flip(1), f_rgb)
+        f_g = torch.where(fy, f_g.flip(1) * sign_y, f_g)
+        t = torch.where(fy, t.flip(1), t)
+        f = torch.cat([f_rgb, f_g], -1).contiguous()
+        t = t.contiguous()
+        torch.cuda.synchronize()
         cur_lr = lr * (0.5 * (1 + np.cos(np.pi * s / steps)) * 0.9 + 0.1)
-        losses.append(net.train_step_device(batch, crop, crop, d_f, d_t, float(cur_lr)))
-        if log and (s + 1) % 200 == 0:
-            log(f"step {s + 1} loss {np.mean(losses[-200:]):.4f} ({time.perf_counter() - t0:.1f}s)")
+        losses.append(net.train_step_device(batch, c, c, f.data_ptr(), t.data_ptr(), float(cur_lr)))
+        if log and (s + 1) % 500 == 0:
+            log(f"step {s + 1} loss {np.mean(losses[-500:]):.4f} ({time.perf_counter() - t0:.1f}s)")
     ctx.synchronize()
     secs = time.perf_counter() - t0
-    ctx.free(d_f)
-    ctx.free(d_t)
     return losses, secs
 
 

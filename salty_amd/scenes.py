@@ -394,3 +394,23 @@ def batch_scene(seed: int, tris: int) -> SceneDesc:
                          near_clip=1.0, type=CAMERA_PINHOLE)
     desc.environment = (0.15, 0.18, 0.22)
     return desc
+
+
+BATCH_RESOLUTIONS = ((320, 240), (640, 480), (1280, 720), (1920, 1080))
+BATCH_SPP = (4, 16, 64)
+
+
+def batch_config(seed: int = 0, count: int = 64, tri_range=(1_000, 500_000)):
+    """The uneven batch: `count` scenes with triangle counts log-uniform in
+    tri_range, resolutions drawn from BATCH_RESOLUTIONS and samples per pixel
+    from BATCH_SPP.  Returns [(scene description, width, height, spp)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        tris = int(math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1]))))
+        w, h = BATCH_RESOLUTIONS[rng.integers(len(BATCH_RESOLUTIONS))]
+        spp = int(BATCH_SPP[rng.integers(len(BATCH_SPP))])
+        desc = batch_scene(seed * 1000 + k, tris)
+        desc.camera.aspect = w / h
+        out.append((desc, w, h, spp))
+    return out

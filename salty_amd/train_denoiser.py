@@ -21,18 +21,22 @@ import numpy as np
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-scenes", type=int, default=384)
+    ap.add_argument("--train-scenes", type=int, default=1024)
     ap.add_argument("--held-out", type=int, default=4)
     ap.add_argument("--spp-ref", type=int, default=1024)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--crop", type=int, default=128)
+    ap.add_argument("--crop", type=int, default=0, help="random crop side (0: whole frames)")
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--gt-spp", type=int, default=65536)
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
 
+    # the training loop indexes the device-resident dataset with torch; torch's
+    # HIP runtime has to be initialised before the library's context
+    import torch
+    torch.cuda.init()
     import salty_amd
     from salty_amd import denoise, scenes
 
@@ -47,7 +51,7 @@ def main():
     log(f"dataset: {len(data)} scenes in {data_s:.1f}s")
     net = salty_amd.Denoiser(ctx, seed=0)
     losses, train_s = denoise.train(net, ctx, data, steps=args.steps, batch=args.batch, lr=args.lr,
-                                    crop=args.crop, log=log)
+                                    crop=args.crop or None, log=log)
     out = args.out or denoise.WEIGHTS
     denoise.save_weights(net, out)
     log(f"trained {args.steps} steps in {train_s:.1f}s; final loss {np.mean(losses[-100:]):.4f}; weights {out}")
