@@ -41,10 +41,15 @@ def parse():
     p.add_argument("--depth", type=int, default=12)
     p.add_argument("--seed", type=int, default=0)
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU baseline sample")
+    p.add_argument("--passes-in-flight", type=int, default=0,
+                   help="passes traced together per wavefront (0: the library's automatic choice)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--workload", choices=("mesh", "soup", "unet"), default="mesh",
+    p.add_argument("--no-denoise", action="store_true",
+                   help="skip the time-to-relMSE comparison (learned denoiser vs plain path tracing)")
+    p.add_argument("--workload", choices=("mesh", "soup", "unet", "batch"), default="mesh",
                    help="mesh: the path-traced mesh scene (default); soup: closest/any-hit rays against a "
-                        "triangle soup; unet: denoiser inference at 1920x1080 and training steps")
+                        "triangle soup; unet: denoiser inference at 1920x1080 and training steps; batch: 64 "
+                        "uneven scenes rendered together")
     p.add_argument("--soup-tris", type=int, default=4 << 20)
     p.add_argument("--soup-rays", type=int, default=16 << 20)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -279,6 +284,102 @@ def unet_main(args):
         ctx.free(p)
 
 
+def time_to_relmse_leg(ctx):
+    """Device time to relMSE <= 2e-3 against a 65,536-spp ground truth on the
+    36-triangle Cornell box (256x256, depth 8): plain path tracing against path
+    tracing + guides + the trained U-Net (weights in salty_amd/weights)."""
+    import salty_amd
+    from salty_amd import denoise, scenes
+    net = salty_amd.Denoiser(ctx, seed=0)
+    if not denoise.load_weights(net):
+        net.close()
+        return {"error": "no trained weights"}
+    r = denoise.time_to_relmse(ctx, net, scenes.cornell_box(), target=2e-3, width=256, height=256,
+                               gt_spp=65536, max_depth=8, nlm=False)
+    net.close()
+    pick = None
+    for spp, ms, inf, rel in r["denoised"]:
+        if rel <= 2e-3 and (pick is None or ms + inf < pick[1] + pick[2]):
+            pick = (spp, ms, inf, rel)
+    out = {"scene": "Cornell box, 36 triangles, 256x256, depth 8", "target_relmse": 2e-3,
+           "ground_truth_spp": r["gt_spp"], "plain_ms": r["plain_ms"], "denoised_ms": r["denoised_ms"],
+           "best_denoised_relmse": r["best_denoised_relmse"]}
+    if pick:
+        out.update(denoised_input_spp=pick[0], render_ms=pick[1], inference_ms=pick[2], guides_ms=r["guide_ms"],
+                   denoised_relmse=pick[3])
+    if r["plain_ms"] and r["denoised_ms"]:
+        out["speedup"] = r["plain_ms"] / r["denoised_ms"]
+    return out
+
+
+def batch_main(args):
+    """64 seeded scenes (1k-500k triangles, log-uniform), each at a resolution
+    from {320x240, 640x480, 1280x720, 1920x1080} and 4, 16 or 64 spp, depth 8,
+    rendered as one batch: all frames' paths share every launch."""
+    import salty_amd
+    from salty_amd import abi, scenes
+    t = time.perf_counter()
+    cfg = scenes.batch_config(args.seed)
+    gen_s = time.perf_counter() - t
+    ctx = salty_amd.Context(0)
+    t = time.perf_counter()
+    scs = [salty_amd.Scene(ctx, d) for d, _, _, _ in cfg]
+    sset = salty_amd.SceneSet(ctx, scs)
+    build_s = time.perf_counter() - t
+    tris = sum(sc.stats().triangles for sc in scs)
+    bufs = [ctx.malloc(w * h * 12) for _, w, h, _ in cfg]
+    frames = [(k, w, h, spp, k, 0, bufs[k]) for k, (_, w, h, spp) in enumerate(cfg)]
+    depth = 8
+    # work counts at 1 spp, outside the timed region
+    cst = sset.render_batch_device([(k, w, h, 1, k, 1000, d) for k, w, h, _, _, _, d in frames], depth,
+                                   flags=abi.RENDER_COUNT_WORK)
+    ext_nodes = cst.extend_nodes / max(cst.extension_rays, 1)
+    ext_tris = cst.extend_tris / max(cst.extension_rays, 1)
+    for _ in range(args.warmup):
+        sset.render_batch_device(frames, depth)
+    ctx.synchronize()
+    ext_ms = 0.0
+    launches = rays_ext = rays_sh = paths = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = sset.render_batch_device(frames, depth, flags=abi.RENDER_TIME_KERNELS)
+        ext_ms += st.extend_ms
+        launches += st.extend_launches
+        rays_ext += st.extension_rays
+        rays_sh += st.shadow_rays
+        paths += st.paths
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t0
+    avg_ext_ms = ext_ms / max(launches, 1)
+    bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + EXT_IO_BYTES
+    achieved = bytes_per_ray * rays_ext / max(launches, 1) / (avg_ext_ms * 1e-3) / 1e9 if avg_ext_ms else 0.0
+    line = {
+        "metric": "Grays/s + paths/s, 64 uneven scenes rendered as one batch, 1 GPU",
+        "value": (rays_ext + rays_sh) / elapsed / 1e9,
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": f"{len(cfg)} scenes, {tris} triangles in all, resolutions 320x240..1920x1080, "
+                               f"4/16/64 spp, depth 8, one scene set",
+                   "paths_per_s": paths / elapsed, "paths_per_step": paths // max(args.steps, 1),
+                   "scene_build_s": build_s, "scene_generate_s": gen_s},
+        "roofline": {"kernel": "k_extend (closest-hit traversal of the compressed BVH8)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ext_ms, "launches": launches,
+                     "bytes_per_ray": bytes_per_ray, "nodes_per_ray": ext_nodes, "tris_per_ray": ext_tris},
+    }
+    print(json.dumps(line), flush=True)
+    for d in bufs:
+        ctx.free(d)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -289,6 +390,12 @@ def main():
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         soup_main(args)
+        return
+    if args.workload == "batch":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        batch_main(args)
         return
     if args.workload == "unet":
         if args.gpus != 1:
@@ -318,7 +425,7 @@ def main():
     ctx.copy_to_device(d_accum, zero)
 
     def cfg(spp, flags=0, first_pass=0):
-        return abi.RenderConfig(W, H, args.depth, spp, args.seed, 0, first_pass, flags, 0)
+        return abi.RenderConfig(W, H, args.depth, spp, args.seed, 0, first_pass, flags, args.passes_in_flight)
 
     # work counts (nodes visited / triangles tested per ray), outside the timed region
     cst = scene.render_device(cfg(2, abi.RENDER_COUNT_WORK, first_pass=1000), d_accum)
@@ -390,6 +497,8 @@ def main():
             "shadow_kernel_ms_per_launch": sh_ms / max(launches, 1),
         },
     }
+    if not args.no_denoise:
+        line["config"]["time_to_relmse"] = time_to_relmse_leg(ctx)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(desc, args)
     print(json.dumps(line), flush=True)

@@ -49,10 +49,10 @@ struct PathBuf {
     unsigned long long *tot;                // [0] extension rays, [1] shadow rays
 };
 
-size_t path_bytes(uint32_t n) {
-    // 30 four-byte fields + 1 eight-byte field per path, three queues, counters
-    return (size_t)n * (30 * 4 + 8 + 3 * 4) + 4096;
-}
+// 30 four-byte fields + 1 eight-byte field per path and three queues
+constexpr size_t kPathBytes = 30 * 4 + 8 + 3 * 4;
+
+size_t path_bytes(uint32_t n) { return (size_t)n * kPathBytes + 4096; }
 
 PathBuf carve(void *base, uint32_t n) {
     char *p = (char *)base;
@@ -113,9 +113,11 @@ __device__ __forceinline__ const Seg &find_seg(const Seg *__restrict__ segs, uin
 }
 
 // Paths traced together when the caller leaves passes_in_flight at 0 (about
-// 2.3 GB of path state), and the most ever held at once.
-constexpr uint32_t kAutoPathsInFlight = 1u << 24;
-constexpr uint32_t kMaxPathsInFlight = 1u << 27;
+// 19 GB of path state: all 64 passes of a 1080p frame at once, which keeps the
+// deep-bounce launches full), and the most ever held at once.  The automatic
+// choice also stays below a quarter of the free device memory.
+constexpr uint32_t kAutoPathsInFlight = 1u << 27;
+constexpr uint32_t kMaxPathsInFlight = 1u << 28;
 
 // Wave-aggregated queue append: one atomic per wavefront.
 __device__ __forceinline__ uint32_t queue_slot(uint32_t *counter, bool want) {
@@ -481,6 +483,11 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     }
     if (max_bounce < 0) return set_error(ctx, S3DG_ERR_INVALID, "bad bounce count");
     uint64_t cap = passes_in_flight ? (uint64_t)passes_in_flight * max_npix : (uint64_t)kAutoPathsInFlight;
+    if (!passes_in_flight) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+            cap = std::min<uint64_t>(cap, (free_b / 4 + ws.render_bytes) / kPathBytes);
+    }
     cap = std::min<uint64_t>(cap, kMaxPathsInFlight);
     cap = std::min<uint64_t>(cap, std::max<uint64_t>(total_paths, 1));
     cap = std::max<uint64_t>(cap, max_npix);
