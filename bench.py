@@ -43,6 +43,7 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target duration of the CPU baseline sample")
     p.add_argument("--passes-in-flight", type=int, default=0,
                    help="passes traced together per wavefront (0: the library's automatic choice)")
+    p.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel settings")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-denoise", action="store_true",
                    help="skip the time-to-relMSE comparison (learned denoiser vs plain path tracing)")
@@ -432,6 +433,10 @@ def main():
     ext_nodes = cst.extend_nodes / max(cst.extension_rays, 1)
     ext_tris = cst.extend_tris / max(cst.extension_rays, 1)
 
+    tuning = None
+    if not args.no_autotune:
+        t, _ = scene.autotune(cfg(args.spp))
+        tuning = {"traversal_variant": t.traversal_variant, "refill": t.refill, "tri_batch": t.tri_batch}
     for w in range(args.warmup):
         scene.render_device(cfg(args.spp, first_pass=0), d_accum)
     ctx.synchronize()
@@ -480,6 +485,7 @@ def main():
             "scene_build_s": build_s,
             "scene_generate_s": gen_s,
             "device_scene_bytes": sstats.device_bytes,
+            "tuning": tuning,
         },
         "roofline": {
             "kernel": "k_extend (closest-hit traversal of the compressed BVH8)",

@@ -301,6 +301,23 @@ int s3dg_render_guides_device(s3dg_scene *scene, const s3dg_render_config *cfg,
                               float *d_albedo, float *d_normal, float *d_depth);
 
 /* ---------------------------------------------------------------------------
+ * Tuning of the traversal and render kernels (no reference counterpart), per
+ * context.  s3dg_autotune times trial renders of `cfg` (at most 16 passes of
+ * it) on the scene for the candidate settings and keeps the fastest in the
+ * context; results do not depend on the tuning, only the speed does.
+ * ------------------------------------------------------------------------- */
+typedef struct s3dg_tuning {
+    uint32_t traversal_variant;  /* 0: 16 LDS stack entries, 5 waves/SIMD; 1: 12, 6; 2: 8, 6 */
+    uint32_t refill;             /* idle lanes of a wave that trigger new rays (1..64)       */
+    uint32_t tri_batch;          /* lanes with pending triangles that keep a step testing (1..65) */
+    uint32_t passes_in_flight;   /* path capacity used when a render passes 0 (0 = automatic) */
+} s3dg_tuning;
+
+int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out);
+int s3dg_set_tuning(s3dg_context *ctx, const s3dg_tuning *tuning);
+int s3dg_autotune(s3dg_scene *scene, const s3dg_render_config *cfg, s3dg_tuning *best, double *best_ms);
+
+/* ---------------------------------------------------------------------------
  * Scene sets and batched rendering (no reference counterpart: the reference
  * renders one scene per process).  A set merges existing scenes of one
  * context into one set of device arrays (the scenes stay valid and

@@ -16,7 +16,9 @@ from .abi import (HIT_DTYPE, RAY_DTYPE, VERTEX_DTYPE, RenderConfig, RenderStats,
                   SceneStats)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libs3dg.so")
+# S3DG_LIBRARY names another build of the library (for example one compiled
+# with different kernel parameters); the default is the in-tree build.
+LIB_PATH = os.environ.get("S3DG_LIBRARY") or os.path.join(_HERE, "libs3dg.so")
 
 _lib = None
 
@@ -42,6 +44,9 @@ _SIGNATURES = {
     "s3dg_render": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.POINTER(RenderStats)],
     "s3dg_render_device": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.POINTER(RenderStats)],
     "s3dg_render_guides_device": [C.c_void_p, C.POINTER(RenderConfig), C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_get_tuning": [C.c_void_p, C.POINTER(abi.Tuning)],
+    "s3dg_set_tuning": [C.c_void_p, C.POINTER(abi.Tuning)],
+    "s3dg_autotune": [C.c_void_p, C.POINTER(RenderConfig), C.POINTER(abi.Tuning), C.POINTER(C.c_double)],
     "s3dg_scene_set_create": [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)],
     "s3dg_scene_set_destroy": [C.c_void_p],
     "s3dg_render_batch_device": [C.c_void_p, C.c_uint32, C.POINTER(abi.FrameJob), C.POINTER(RenderConfig),
@@ -137,6 +142,14 @@ class Context:
         return self.lib.s3dg_stream(self.handle)
 
     # -- device buffers -------------------------------------------------
+    def get_tuning(self) -> abi.Tuning:
+        t = abi.Tuning()
+        self.check(self.lib.s3dg_get_tuning(self.handle, C.byref(t)), "s3dg_get_tuning")
+        return t
+
+    def set_tuning(self, t: abi.Tuning):
+        self.check(self.lib.s3dg_set_tuning(self.handle, C.byref(t)), "s3dg_set_tuning")
+
     def malloc(self, nbytes: int) -> int:
         p = C.c_void_p()
         self.check(self.lib.s3dg_malloc(self.handle, nbytes, C.byref(p)), "s3dg_malloc")
@@ -254,6 +267,15 @@ class Scene:
         self.ctx.check(self.ctx.lib.s3dg_render_device(self.handle, C.byref(cfg), C.c_void_p(d_accum),
                                                        C.byref(st)), "s3dg_render_device")
         return st
+
+    def autotune(self, cfg: RenderConfig):
+        """Times trial renders of cfg for the candidate kernel settings and keeps
+        the fastest in the context; returns (tuning, device ms of the trial)."""
+        t = abi.Tuning()
+        ms = C.c_double()
+        self.ctx.check(self.ctx.lib.s3dg_autotune(self.handle, C.byref(cfg), C.byref(t), C.byref(ms)),
+                       "s3dg_autotune")
+        return t, ms.value
 
     def guides_device(self, cfg: RenderConfig, d_albedo: int, d_normal: int, d_depth: int):
         self.ctx.check(self.ctx.lib.s3dg_render_guides_device(self.handle, C.byref(cfg), C.c_void_p(d_albedo),

@@ -76,6 +76,11 @@ class RenderConfig(C.Structure):
                 ("flags", C.c_uint32), ("passes_in_flight", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("traversal_variant", C.c_uint32), ("refill", C.c_uint32), ("tri_batch", C.c_uint32),
+                ("passes_in_flight", C.c_uint32)]
+
+
 class FrameJob(C.Structure):
     _fields_ = [("scene", C.c_uint32), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
                 ("seed", C.c_uint64), ("first_pass", C.c_int32), ("reserved", C.c_int32),

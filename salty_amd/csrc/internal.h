@@ -22,6 +22,10 @@ struct s3dg_context {
     uint32_t refill = 32;
     // a traversal step keeps testing triangles while this many lanes have some pending
     uint32_t tri_batch = 16;
+    // traversal kernel variant (index into kVariants)
+    uint32_t variant = 1;
+    // passes traced together when a render leaves it to the library (0: automatic)
+    uint32_t passes_in_flight = 0;
 };
 
 namespace s3dg {
@@ -110,20 +114,31 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
 int set_error(s3dg_context *ctx, int code, const std::string &msg);
 int workspace_init(s3dg_context *ctx, Workspace &ws);
 void workspace_free(Workspace &ws);
-int ensure_gstack(s3dg_context *ctx, Workspace &ws, uint32_t threads, int32_t stack_need);
+int ensure_gstack(s3dg_context *ctx, Workspace &ws, uint32_t threads, int32_t stack_need, int lds);
 int check_error_flag(s3dg_context *ctx, Workspace &ws);
 int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks);
 uint32_t batch_grid(uint32_t n);
 int hip_check(s3dg_context *ctx, hipError_t e, const char *what);
 int ensure_buffer(s3dg_context *ctx, void **ptr, size_t *have, size_t need);
 
-// Stack entries held in LDS per lane before spilling to global memory.
-constexpr int kLdsStack = 16;
 constexpr int kBlock = 256;
 
-// Global-memory stack entries per lane beyond the LDS ones.
-inline int32_t stack_capacity(int32_t stack_need) {
-    return stack_need - kLdsStack + 4 > 4 ? stack_need - kLdsStack + 4 : 4;
+// Traversal kernel variants: stack entries held in LDS per lane before
+// spilling to global memory, and the waves per SIMD the register allocation
+// aims at.  Fewer LDS entries and registers let more waves hide memory
+// latency; the autotuner picks one per context.
+struct TraversalVariant {
+    int lds, waves;
+};
+constexpr TraversalVariant kVariants[] = {{16, 5}, {12, 6}, {8, 6}};
+constexpr int kVariantCount = 3;
+constexpr int kDefaultVariant = 1;
+// Stack entries kernels without a variant (guides) keep in LDS.
+constexpr int kLdsStack = 16;
+
+// Global-memory stack entries per lane beyond the `lds` ones.
+inline int32_t stack_capacity(int32_t stack_need, int lds = kLdsStack) {
+    return stack_need - lds + 4 > 4 ? stack_need - lds + 4 : 4;
 }
 
 }  // namespace s3dg

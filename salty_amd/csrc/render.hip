@@ -174,10 +174,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const 
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap,
-                                                    int32_t *err, uint32_t refill) {
-    __shared__ uint2 lds[kLdsStack * kBlock];
+template <bool COUNT, int LDS, int WAVES>
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
+k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_t *err, uint32_t refill) {
+    __shared__ uint2 lds[LDS * kBlock];
     const uint32_t n = P.cnt[cur];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -187,10 +187,11 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_
         P.cnt[5] = 0;   // the shadow launch of this bounce fetches from zero
         P.tot[0] += n;
     }
-    Stack st;
+    Stack<LDS> st;
     st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     const uint32_t *q = P.q[cur];
     Tracer<false, COUNT> tr;
+    tr.reset_counts();
     uint32_t p = 0;
     refill_loop(
         &P.cnt[4], n, refill,
@@ -210,8 +211,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneDev S, PathBuf P, int32_
             P.hv[p] = tr.best.v;
             P.hprim[p] = tr.best.prim;
             P.hshape[p] = tr.best.shape;
-            tr.count(P.tot + 2);
         });
+    tr.flush(P.tot + 2);
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t cur, int32_t depth,
@@ -309,17 +310,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
     }
 }
 
-template <bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap,
-                                                    int32_t *err, uint32_t refill) {
-    __shared__ uint2 lds[kLdsStack * kBlock];
+template <bool COUNT, int LDS, int WAVES>
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
+k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, uint32_t refill) {
+    __shared__ uint2 lds[LDS * kBlock];
     const uint32_t n = P.cnt[2];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gtid == 0) P.tot[1] += n;
-    Stack st;
+    Stack<LDS> st;
     st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     Tracer<true, COUNT> tr;
+    tr.reset_counts();
     uint32_t p = 0;
     float tl = 0.0f, lu = 0.0f, lv = 0.0f;
     int32_t ltri = -1;
@@ -356,7 +358,6 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 
         },
         [&]() { return tr.step(S, st); },
         [&]() {
-            tr.count(P.tot + 4);
             if (tr.best.prim != -1) return;
             // NextEventEstimation contribution (s3d_pt.cpp:274-289)
             const GLight &lt = S.lights[P.slight[p]];
@@ -386,6 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneDev S, PathBuf P, uint2 
             P.lg[p] = P.lg[p] + c.y;
             P.lb[p] = P.lb[p] + c.z;
         });
+    tr.flush(P.tot + 4);
 }
 
 // m_RenderTarget[idx] += L for each pass in turn (s3d_pt.cpp:330-332).
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
     const uint32_t n = (uint32_t)W * (uint32_t)H;
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st;
+    Stack<kLdsStack> st;
     st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     for (uint32_t p = gtid; p < n; p += stride) {
         const uint32_t x = p % (uint32_t)W, y = p / (uint32_t)W;
@@ -458,6 +460,32 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
     }
 }
 
+// Extension and shadow kernels of one traversal variant ([0] plain, [1] counting).
+struct TraceKernels {
+    const void *ext[2];
+    const void *sh[2];
+    int lds;
+};
+
+template <int V>
+TraceKernels trace_kernels_of() {
+    constexpr int L = kVariants[V].lds, W = kVariants[V].waves;
+    return TraceKernels{{reinterpret_cast<const void *>(&k_extend<false, L, W>),
+                         reinterpret_cast<const void *>(&k_extend<true, L, W>)},
+                        {reinterpret_cast<const void *>(&k_shadow<false, L, W>),
+                         reinterpret_cast<const void *>(&k_shadow<true, L, W>)},
+                        L};
+}
+
+TraceKernels trace_kernels(uint32_t v) {
+    static_assert(kVariantCount == 3, "one case per variant");
+    switch (v) {
+        case 0: return trace_kernels_of<0>();
+        case 2: return trace_kernels_of<2>();
+        default: return trace_kernels_of<1>();
+    }
+}
+
 }  // namespace
 
 namespace s3dg {
@@ -482,6 +510,7 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
         total_paths += (uint64_t)npix * (uint32_t)J.spp;
     }
     if (max_bounce < 0) return set_error(ctx, S3DG_ERR_INVALID, "bad bounce count");
+    if (!passes_in_flight) passes_in_flight = ctx->passes_in_flight;
     uint64_t cap = passes_in_flight ? (uint64_t)passes_in_flight * max_npix : (uint64_t)kAutoPathsInFlight;
     if (!passes_in_flight) {
         size_t free_b = 0, total_b = 0;
@@ -568,13 +597,18 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
         S3DG_HIP(ctx, hipMemcpyAsync(d_all_segs, all_segs.data(), all_segs.size() * sizeof(Seg),
                                      hipMemcpyHostToDevice, ctx->stream));
     const uint32_t grid = batch_grid(slots);
+    const TraceKernels K = trace_kernels(ctx->variant);
     uint32_t tgrid = 0;
-    rc = resident_grid(ctx, reinterpret_cast<const void *>(&k_extend<false>), &tgrid);
+    rc = resident_grid(ctx, K.ext[0], &tgrid);
     if (rc) return rc;
-    rc = ensure_gstack(ctx, ws, tgrid * kBlock, S.stack_need);
+    rc = ensure_gstack(ctx, ws, tgrid * kBlock, S.stack_need, K.lds);
     if (rc) return rc;
-    const int32_t gcap = stack_capacity(S.stack_need);
+    int32_t gcap = stack_capacity(S.stack_need, K.lds);
     uint2 *gs = (uint2 *)ws.d_gstack;
+    int32_t *d_err = ws.d_error;
+    uint32_t refill = ctx->refill;
+    SceneDev Sx = S;
+    Sx.tri_batch = ctx->tri_batch;
 
     const bool timed = (flags & S3DG_RENDER_TIME_KERNELS) != 0;
     const bool count = (flags & S3DG_RENDER_COUNT_WORK) != 0;
@@ -592,25 +626,17 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
 
     // one batch's kernels; li = index of its first bounce launch (timing)
     auto launch_batch = [&](size_t li) {
-        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, S, P, d_desc, d_segs);
+        hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, Sx, P, d_desc, d_segs);
         int32_t cur = 0;
         for (int32_t depth = 0; depth < max_bounce; ++depth, ++li) {
             if (timed) (void)hipEventRecord(ev[4 * li], ctx->stream);
-            if (count)
-                hipLaunchKernelGGL(k_extend<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, cur, gs, gcap,
-                                   ws.d_error, ctx->refill);
-            else
-                hipLaunchKernelGGL(k_extend<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, cur, gs, gcap,
-                                   ws.d_error, ctx->refill);
+            void *ext_args[] = {&Sx, &P, &cur, &gs, &gcap, &d_err, &refill};
+            (void)hipLaunchKernel(K.ext[count ? 1 : 0], dim3(tgrid), dim3(kBlock), ext_args, 0, ctx->stream);
             if (timed) (void)hipEventRecord(ev[4 * li + 1], ctx->stream);
-            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, S, P, cur, depth, rr_start);
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, Sx, P, cur, depth, rr_start);
             if (timed) (void)hipEventRecord(ev[4 * li + 2], ctx->stream);
-            if (count)
-                hipLaunchKernelGGL(k_shadow<true>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, gs, gcap,
-                                   ws.d_error, ctx->refill);
-            else
-                hipLaunchKernelGGL(k_shadow<false>, dim3(tgrid), dim3(kBlock), 0, ctx->stream, S, P, gs, gcap,
-                                   ws.d_error, ctx->refill);
+            void *sh_args[] = {&Sx, &P, &gs, &gcap, &d_err, &refill};
+            (void)hipLaunchKernel(K.sh[count ? 1 : 0], dim3(tgrid), dim3(kBlock), sh_args, 0, ctx->stream);
             if (timed) (void)hipEventRecord(ev[4 * li + 3], ctx->stream);
             cur ^= 1;
         }
@@ -623,8 +649,9 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     const bool use_graph = !timed && !count;
     if (use_graph) {
         const uint64_t key[8] = {(uint64_t)(uintptr_t)ws.d_render, slots, (uint64_t)max_bounce,
-                                 (uint64_t)(uint32_t)rr_start, tgrid, (uint64_t)(uintptr_t)S.nodes,
-                                 (uint64_t)(uintptr_t)S.recs, ((uint64_t)ctx->refill << 32) | S.tri_batch};
+                                 ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
+                                 (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
+                                 ((uint64_t)refill << 32) | Sx.tri_batch};
         if (!ws.graph || memcmp(key, ws.graph_key, sizeof(key)) != 0) {
             if (ws.graph) {
                 (void)hipGraphExecDestroy(ws.graph);
@@ -709,6 +736,87 @@ extern "C" int s3dg_render_device(s3dg_scene *sc, const s3dg_render_config *cfg,
                        cfg->passes_in_flight, stats);
 }
 
+extern "C" int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out) {
+    if (!ctx || !out) return S3DG_ERR_INVALID;
+    out->traversal_variant = ctx->variant;
+    out->refill = ctx->refill;
+    out->tri_batch = ctx->tri_batch;
+    out->passes_in_flight = ctx->passes_in_flight;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_set_tuning(s3dg_context *ctx, const s3dg_tuning *t) {
+    if (!ctx || !t) return S3DG_ERR_INVALID;
+    if (t->traversal_variant >= (uint32_t)kVariantCount || t->refill < 1 || t->refill > 64 || t->tri_batch < 1 ||
+        t->tri_batch > 65)
+        return set_error(ctx, S3DG_ERR_INVALID, "tuning out of range");
+    ctx->variant = t->traversal_variant;
+    ctx->refill = t->refill;
+    ctx->tri_batch = t->tri_batch;
+    ctx->passes_in_flight = t->passes_in_flight;
+    return S3DG_OK;
+}
+
+// Coordinate descent over the traversal variant, then the refill threshold,
+// then the triangle batching threshold; each candidate is timed (device time)
+// on its second trial render, after a first one that captures its graph.
+extern "C" int s3dg_autotune(s3dg_scene *sc, const s3dg_render_config *cfg, s3dg_tuning *best, double *best_ms) {
+    if (!sc || !cfg || cfg->width <= 0 || cfg->height <= 0) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = sc->ctx;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t bytes = (size_t)cfg->width * cfg->height * 3 * sizeof(float);
+    int rc = ensure_buffer(ctx, &ctx->scratch, &ctx->scratch_bytes, bytes);
+    if (rc) return rc;
+    s3dg_render_config trial = *cfg;
+    trial.spp = std::max(1, std::min(cfg->spp, 16));
+    trial.flags = 0;
+    auto time_one = [&](const s3dg_tuning &t, double &ms) -> int {
+        int r = s3dg_set_tuning(ctx, &t);
+        if (r) return r;
+        s3dg_render_stats st{};
+        for (int k = 0; k < 2; ++k) {
+            r = s3dg_render_device(sc, &trial, (float *)ctx->scratch, &st);
+            if (r) return r;
+        }
+        ms = st.milliseconds;
+        return S3DG_OK;
+    };
+    s3dg_tuning cur{};
+    s3dg_get_tuning(ctx, &cur);
+    const s3dg_tuning start = cur;
+    double cur_ms = 0.0;
+    rc = time_one(cur, cur_ms);
+    if (rc) return rc;
+    const uint32_t refills[] = {16, 24, 32, 48};
+    const uint32_t batches[] = {8, 16, 24};
+    for (int axis = 0; axis < 3; ++axis) {
+        const int n = axis == 0 ? kVariantCount : axis == 1 ? 4 : 3;
+        s3dg_tuning axis_best = cur;
+        for (int i = 0; i < n; ++i) {
+            s3dg_tuning t = cur;
+            if (axis == 0) t.traversal_variant = (uint32_t)i;
+            else if (axis == 1) t.refill = refills[i];
+            else t.tri_batch = batches[i];
+            if (memcmp(&t, &cur, sizeof(t)) == 0) continue;
+            double ms = 0.0;
+            rc = time_one(t, ms);
+            if (rc) {
+                s3dg_set_tuning(ctx, &start);
+                return rc;
+            }
+            if (ms < cur_ms) {
+                cur_ms = ms;
+                axis_best = t;
+            }
+        }
+        cur = axis_best;
+    }
+    s3dg_set_tuning(ctx, &cur);
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+    return S3DG_OK;
+}
+
 extern "C" int s3dg_render(s3dg_scene *sc, const s3dg_render_config *cfg, float *accum, s3dg_render_stats *stats) {
     if (!sc || !cfg || !accum) return S3DG_ERR_INVALID;
     s3dg_context *ctx = sc->ctx;
@@ -733,7 +841,7 @@ extern "C" int s3dg_render_guides_device(s3dg_scene *sc, const s3dg_render_confi
     S3DG_HIP(ctx, hipSetDevice(ctx->device));
     const uint32_t n = (uint32_t)cfg->width * (uint32_t)cfg->height;
     const uint32_t grid = batch_grid(n);
-    int rc = ensure_gstack(ctx, sc->ws, grid * kBlock, sc->dev.stack_need);
+    int rc = ensure_gstack(ctx, sc->ws, grid * kBlock, sc->dev.stack_need, kLdsStack);
     if (rc) return rc;
     const int32_t gcap = stack_capacity(sc->dev.stack_need);
     hipLaunchKernelGGL(k_guides, dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, cfg->width, cfg->height,

@@ -35,6 +35,11 @@ extern "C" int s3dg_create(s3dg_context **out, int device) {
         const long v = std::strtol(env, nullptr, 10);
         if (v >= 1 && v <= 64) ctx->refill = (uint32_t)v;
     }
+    // S3DG_VARIANT (0..2) overrides the traversal kernel variant
+    if (const char *env = std::getenv("S3DG_VARIANT")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 0 && v < kVariantCount) ctx->variant = (uint32_t)v;
+    }
     // S3DG_TRI_BATCH (1..65) overrides the pending-triangle lane count that keeps a step testing
     if (const char *env = std::getenv("S3DG_TRI_BATCH")) {
         const long v = std::strtol(env, nullptr, 10);
@@ -102,17 +107,18 @@ extern "C" void *s3dg_stream(s3dg_context *ctx) { return ctx ? (void *)ctx->stre
 // ---------------------------------------------------------------------------
 namespace {
 
-template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n,
-                                                   s3dg_hit *__restrict__ hits, int32_t *__restrict__ shapes,
-                                                   uint2 *gstack, int32_t gcap, int32_t *err, unsigned long long *counters,
-                                                   uint32_t *cursor, uint32_t refill) {
-    __shared__ uint2 lds[kLdsStack * kBlock];
+template <bool ANY, bool COUNT, int LDS, int WAVES>
+__global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
+k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n, s3dg_hit *__restrict__ hits,
+        int32_t *__restrict__ shapes, uint2 *gstack, int32_t gcap, int32_t *err, unsigned long long *counters,
+        uint32_t *cursor, uint32_t refill) {
+    __shared__ uint2 lds[LDS * kBlock];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    Stack st;
+    Stack<LDS> st;
     st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     Tracer<ANY, COUNT> tr;
+    tr.reset_counts();
     uint32_t i = 0;
     refill_loop(
         cursor, n, refill,
@@ -138,8 +144,33 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const s3dg_ray *__
             o.w = tr.best.v;
             reinterpret_cast<float4 *>(hits)[i] = o;
             if (shapes) shapes[i] = tr.best.shape;
-            tr.count(counters);
         });
+    tr.flush(counters);
+}
+
+// Batch kernels of one traversal variant, [any][count].
+struct BatchKernels {
+    const void *k[2][2];
+    int lds;
+};
+
+template <int V>
+BatchKernels batch_kernels_of() {
+    constexpr int L = kVariants[V].lds, W = kVariants[V].waves;
+    return BatchKernels{{{reinterpret_cast<const void *>(&k_trace<false, false, L, W>),
+                          reinterpret_cast<const void *>(&k_trace<false, true, L, W>)},
+                         {reinterpret_cast<const void *>(&k_trace<true, false, L, W>),
+                          reinterpret_cast<const void *>(&k_trace<true, true, L, W>)}},
+                        L};
+}
+
+BatchKernels batch_kernels(uint32_t v) {
+    static_assert(kVariantCount == 3, "one case per variant");
+    switch (v) {
+        case 0: return batch_kernels_of<0>();
+        case 2: return batch_kernels_of<2>();
+        default: return batch_kernels_of<1>();
+    }
 }
 
 __global__ void k_clear_u32(uint32_t *p) {
@@ -184,8 +215,8 @@ void workspace_free(Workspace &ws) {
     ws = Workspace{};
 }
 
-int ensure_gstack(s3dg_context *ctx, Workspace &ws, uint32_t threads, int32_t stack_need) {
-    const size_t need = (size_t)threads * stack_capacity(stack_need) * sizeof(uint2);
+int ensure_gstack(s3dg_context *ctx, Workspace &ws, uint32_t threads, int32_t stack_need, int lds) {
+    const size_t need = (size_t)threads * stack_capacity(stack_need, lds) * sizeof(uint2);
     return ensure_buffer(ctx, &ws.d_gstack, &ws.gstack_bytes, need);
 }
 
@@ -204,32 +235,24 @@ int launch_trace(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mod
                  int32_t *d_shapes, unsigned long long *d_counters) {
     s3dg_context *ctx = sc->ctx;
     if (n == 0) return S3DG_OK;
+    const BatchKernels K = batch_kernels(ctx->variant);
+    const void *kern = K.k[mode == S3DG_TRACE_ANY ? 1 : 0][d_counters ? 1 : 0];
     uint32_t grid = 0;
-    int rc = resident_grid(ctx, reinterpret_cast<const void *>(&k_trace<false, false>), &grid);
+    int rc = resident_grid(ctx, kern, &grid);
     if (rc) return rc;
     grid = std::min(grid, (n + kBlock - 1) / kBlock);
-    rc = ensure_gstack(ctx, sc->ws, grid * kBlock, sc->dev.stack_need);
+    rc = ensure_gstack(ctx, sc->ws, grid * kBlock, sc->dev.stack_need, K.lds);
     if (rc) return rc;
     uint2 *gs = (uint2 *)sc->ws.d_gstack;
-    const int32_t cap = stack_capacity(sc->dev.stack_need);
+    int32_t cap = stack_capacity(sc->dev.stack_need, K.lds);
     uint32_t *cursor = trace_cursor(sc->ws);
-    const uint32_t refill = ctx->refill;
+    uint32_t refill = ctx->refill;
+    SceneDev S = sc->dev;
+    S.tri_batch = ctx->tri_batch;
+    int32_t *d_err = sc->ws.d_error;
     hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, ctx->stream, cursor);
-    if (mode == S3DG_TRACE_ANY) {
-        if (d_counters)
-            hipLaunchKernelGGL((k_trace<true, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
-        else
-            hipLaunchKernelGGL((k_trace<true, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
-    } else {
-        if (d_counters)
-            hipLaunchKernelGGL((k_trace<false, true>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
-        else
-            hipLaunchKernelGGL((k_trace<false, false>), dim3(grid), dim3(kBlock), 0, ctx->stream, sc->dev, d_rays, n,
-                               d_hits, d_shapes, gs, cap, sc->ws.d_error, d_counters, cursor, refill);
-    }
+    void *args[] = {&S, &d_rays, &n, &d_hits, &d_shapes, &gs, &cap, &d_err, &d_counters, &cursor, &refill};
+    S3DG_HIP(ctx, hipLaunchKernel(kern, dim3(grid), dim3(kBlock), args, 0, ctx->stream));
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
 }

@@ -158,7 +158,7 @@ __device__ __forceinline__ RayPre precompute(const DRay &r) {
 // and a group of a node's hit leaves whose triangles are still to test.
 enum { ENT_GROUP = 0u, ENT_INST = 1u, ENT_RESTORE = 2u, ENT_LEAVES = 3u };
 
-// Traversal stack: the first kLdsStack entries of a lane live in LDS (stride
+// Traversal stack: the first LDS entries of a lane live in LDS (stride
 // kBlock), the rest in a global overflow area (stride gstride).  The pointers
 // carry their address spaces so that the two halves are never merged into one
 // flat access (which would wait on both the LDS and the vector-memory counters).
@@ -168,6 +168,7 @@ typedef __attribute__((address_space(1))) uint64_t glb_u64;
 __device__ __forceinline__ uint64_t pack_entry(uint2 e) { return (uint64_t)e.x | ((uint64_t)e.y << 32); }
 __device__ __forceinline__ uint2 unpack_entry(uint64_t v) { return make_uint2((uint32_t)v, (uint32_t)(v >> 32)); }
 
+template <int LDS>
 struct Stack {
     lds_u64 *lds;
     glb_u64 *gst;
@@ -177,10 +178,10 @@ struct Stack {
     int32_t sp;
 
     __device__ __forceinline__ void push(uint2 e) {
-        if (sp < kLdsStack) {
+        if (sp < LDS) {
             lds[sp * kBlock] = pack_entry(e);
-        } else if (sp - kLdsStack < gcap) {
-            gst[(size_t)(sp - kLdsStack) * gstride] = pack_entry(e);
+        } else if (sp - LDS < gcap) {
+            gst[(size_t)(sp - LDS) * gstride] = pack_entry(e);
         } else {
             atomicOr(err, 1);
             return;
@@ -189,16 +190,16 @@ struct Stack {
     }
     __device__ __forceinline__ uint2 top() const {
         const int32_t i = sp - 1;
-        if (i < kLdsStack) return unpack_entry(lds[i * kBlock]);
-        uint64_t v = gst[(size_t)(i - kLdsStack) * gstride];
+        if (i < LDS) return unpack_entry(lds[i * kBlock]);
+        uint64_t v = gst[(size_t)(i - LDS) * gstride];
         // keeps the two loads apart (an empty asm that names the loaded value)
         __asm__ volatile("" : "+v"(v));
         return unpack_entry(v);
     }
     __device__ __forceinline__ void set_top(uint2 e) {
         const int32_t i = sp - 1;
-        if (i < kLdsStack) lds[i * kBlock] = pack_entry(e);
-        else gst[(size_t)(i - kLdsStack) * gstride] = pack_entry(e);
+        if (i < LDS) lds[i * kBlock] = pack_entry(e);
+        else gst[(size_t)(i - LDS) * gstride] = pack_entry(e);
     }
     // Lane `lane` of the block's stacks over the shared array `base`.
     __device__ __forceinline__ void init(uint2 *base, uint32_t lane, uint2 *global_base, uint32_t global_stride,
@@ -250,7 +251,6 @@ struct Tracer {
         tri_cur = tri_end = 0;
         leaf_mask = 0;
         leaf_node = 0;
-        if (COUNT) visits = tests = 0;
     }
 
     __device__ __forceinline__ void take(float t, float u, float v, int32_t prim, int32_t shape) {
@@ -296,7 +296,8 @@ struct Tracer {
     // enters the instance (or tests the sphere), a restore entry returns to
     // the world ray, a leaf group becomes the pending triangles.  Returns true
     // when an any-hit search is over.
-    __device__ __forceinline__ bool pop(const SceneDev &S, Stack &st) {
+    template <class ST>
+    __device__ __forceinline__ bool pop(const SceneDev &S, ST &st) {
         const uint2 e = st.top();
         const uint32_t kind = e.y >> 30;
         if (kind == ENT_GROUP) {
@@ -363,7 +364,8 @@ struct Tracer {
         return n;
     }
 
-    __device__ __forceinline__ void visit(const SceneDev &S, Stack &st, const NodeLines &nl) {
+    template <class ST>
+    __device__ __forceinline__ void visit(const SceneDev &S, ST &st, const NodeLines &nl) {
         const uint4 *np = reinterpret_cast<const uint4 *>(S.nodes + node);
         const uint4 h0 = nl.h0, h1 = nl.h1, q0 = nl.q0, q1 = nl.q1, q2 = nl.q2;
         if (COUNT) ++visits;
@@ -448,7 +450,8 @@ struct Tracer {
     // One step; true when the ray is finished.  The next node's lines are
     // requested before the pending triangles are tested, so that the two
     // memory round trips overlap.
-    __device__ __forceinline__ bool step(const SceneDev &S, Stack &st) {
+    template <class ST>
+    __device__ __forceinline__ bool step(const SceneDev &S, ST &st) {
         bool tris = tri_cur < tri_end;
         if (!have_node && !tris) {
             if (pop(S, st)) return true;
@@ -473,24 +476,42 @@ struct Tracer {
         return !have_node && tri_cur >= tri_end && leaf_mask == 0 && st.sp == 0;
     }
 
-    __device__ __forceinline__ void count(unsigned long long *counters) const {
+    // Work counters (COUNT): visits and tests accumulate over every ray the
+    // lane traces; flush() adds the wave's totals with one atomic each and
+    // must be reached by all lanes of the wave.
+    __device__ __forceinline__ void reset_counts() {
+        visits = 0;
+        tests = 0;
+    }
+    __device__ __forceinline__ void flush(unsigned long long *counters) const {
         if (COUNT) {
-            atomicAdd(&counters[0], (unsigned long long)visits);
-            atomicAdd(&counters[1], (unsigned long long)tests);
+            unsigned long long v = visits, t = tests;
+            for (int off = 32; off > 0; off >>= 1) {
+                v += __shfl_xor(v, off);
+                t += __shfl_xor(t, off);
+            }
+            if (__lane_id() == 0) {
+                atomicAdd(&counters[0], v);
+                atomicAdd(&counters[1], t);
+            }
         }
     }
 };
 
 // Whole traversal of one ray (for kernels without lane refill).
-template <bool ANY, bool COUNT>
+template <bool ANY, bool COUNT, class ST>
 __device__ __forceinline__ DHit traverse(const SceneDev &S, const SceneRec &rec, const DRay &r, float tmin,
-                                         float tmax, Stack &st, unsigned long long *counters) {
+                                         float tmax, ST &st, unsigned long long *counters) {
     Tracer<ANY, COUNT> tr;
+    tr.reset_counts();
     st.sp = 0;
     tr.start(rec, r, tmin, tmax);
     while (!tr.step(S, st)) {
     }
-    tr.count(counters);
+    if (COUNT) {
+        atomicAdd(&counters[0], (unsigned long long)tr.visits);
+        atomicAdd(&counters[1], (unsigned long long)tr.tests);
+    }
     return tr.best;
 }
 
