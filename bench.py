@@ -436,7 +436,8 @@ def main():
     tuning = None
     if not args.no_autotune:
         t, _ = scene.autotune(cfg(args.spp))
-        tuning = {"traversal_variant": t.traversal_variant, "refill": t.refill, "tri_batch": t.tri_batch}
+        tuning = {"traversal_variant": t.traversal_variant, "refill": t.refill, "tri_batch": t.tri_batch,
+                  "fetch_chunk": t.fetch_chunk}
     for w in range(args.warmup):
         scene.render_device(cfg(args.spp, first_pass=0), d_accum)
     ctx.synchronize()

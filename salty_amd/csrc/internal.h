@@ -19,9 +19,11 @@ struct s3dg_context {
     void *small = nullptr;
     size_t small_bytes = 0;
     // traversal kernels hand new rays to a wave once this many of its lanes are idle
-    uint32_t refill = 32;
+    uint32_t refill = 48;
     // a traversal step keeps testing triangles while this many lanes have some pending
     uint32_t tri_batch = 16;
+    // queue entries a traversal wave takes from the shared cursor at a time
+    uint32_t fetch_chunk = 128;
     // traversal kernel variant (index into kVariants)
     uint32_t variant = 1;
     // passes traced together when a render leaves it to the library (0: automatic)

@@ -119,17 +119,37 @@ __device__ __forceinline__ const Seg &find_seg(const Seg *__restrict__ segs, uin
 constexpr uint32_t kAutoPathsInFlight = 1u << 27;
 constexpr uint32_t kMaxPathsInFlight = 1u << 28;
 
-// Wave-aggregated queue append: one atomic per wavefront.
-__device__ __forceinline__ uint32_t queue_slot(uint32_t *counter, bool want) {
-    const uint64_t mask = __ballot(want);
-    if (!want) return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-    const uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + rank;
+// Workgroup-aggregated appends to two queues: one global atomic per queue per
+// workgroup (instead of one per wave), so that the few counters every
+// workgroup of the grid appends to are not a serialisation point.  Every
+// thread of the workgroup must call it (it synchronises the workgroup).
+__device__ __forceinline__ void block_append2(uint32_t *c0, bool w0, uint32_t &s0, uint32_t *c1, bool w1,
+                                              uint32_t &s1, uint32_t *sh /* 4 * (kBlock / 64) + 2 words of LDS */) {
+    constexpr int kWaves = kBlock / 64;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+    const uint64_t m0 = __ballot(w0), m1 = __ballot(w1);
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (lane == 0) {
+        sh[2 * wave] = (uint32_t)__popcll(m0);
+        sh[2 * wave + 1] = (uint32_t)__popcll(m1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t0 = 0, t1 = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            const uint32_t a = sh[2 * w], b = sh[2 * w + 1];
+            sh[2 * kWaves + 2 * w] = t0;
+            sh[2 * kWaves + 2 * w + 1] = t1;
+            t0 += a;
+            t1 += b;
+        }
+        sh[4 * kWaves] = t0 ? atomicAdd(c0, t0) : 0u;
+        sh[4 * kWaves + 1] = t1 ? atomicAdd(c1, t1) : 0u;
+    }
+    __syncthreads();
+    s0 = sh[4 * kWaves] + sh[2 * kWaves + 2 * wave] + (uint32_t)__popcll(m0 & below);
+    s1 = sh[4 * kWaves + 1] + sh[2 * kWaves + 2 * wave + 1] + (uint32_t)__popcll(m1 & below);
+    __syncthreads();   // the words are rewritten by the next call
 }
 
 // Persistent-kernel work fetch: lane 0 of the wave takes the next 64 queue
@@ -176,7 +196,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const 
 
 template <bool COUNT, int LDS, int WAVES>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
-k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_t *err, uint32_t refill) {
+k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_t *err, FetchParams fetch) {
     __shared__ uint2 lds[LDS * kBlock];
     const uint32_t n = P.cnt[cur];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -194,7 +214,7 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
     tr.reset_counts();
     uint32_t p = 0;
     refill_loop(
-        &P.cnt[4], n, refill,
+        &P.cnt[4], n, fetch,
         [&](uint32_t i) {
             p = q[i];
             DRay r;
@@ -221,6 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t *q = P.q[cur];
+    __shared__ uint32_t sh_append[4 * (kBlock / 64) + 2];
     if (gtid == 0) P.cnt[4] = 0;   // the next extend launch fetches from zero
     const uint32_t trips = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < trips; ++it) {
@@ -303,16 +324,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
             }
             P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
         }
-        const uint32_t si = queue_slot(&P.cnt[2], shadow);
+        uint32_t si = 0, ni = 0;
+        block_append2(&P.cnt[2], shadow, si, &P.cnt[cur ^ 1], cont, ni, sh_append);
         if (shadow) P.qs[si] = p;
-        const uint32_t ni = queue_slot(&P.cnt[cur ^ 1], cont);
         if (cont) P.q[cur ^ 1][ni] = p;
     }
 }
 
 template <bool COUNT, int LDS, int WAVES>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
-k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, uint32_t refill) {
+k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, FetchParams fetch) {
     __shared__ uint2 lds[LDS * kBlock];
     const uint32_t n = P.cnt[2];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -326,7 +347,7 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, uint3
     float tl = 0.0f, lu = 0.0f, lv = 0.0f;
     int32_t ltri = -1;
     refill_loop(
-        &P.cnt[5], n, refill,
+        &P.cnt[5], n, fetch,
         [&](uint32_t i) {
             p = P.qs[i];
             const GLight &lt = S.lights[P.slight[p]];
@@ -606,7 +627,7 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     int32_t gcap = stack_capacity(S.stack_need, K.lds);
     uint2 *gs = (uint2 *)ws.d_gstack;
     int32_t *d_err = ws.d_error;
-    uint32_t refill = ctx->refill;
+    FetchParams fetch{ctx->refill, ctx->fetch_chunk};
     SceneDev Sx = S;
     Sx.tri_batch = ctx->tri_batch;
 
@@ -630,12 +651,12 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
         int32_t cur = 0;
         for (int32_t depth = 0; depth < max_bounce; ++depth, ++li) {
             if (timed) (void)hipEventRecord(ev[4 * li], ctx->stream);
-            void *ext_args[] = {&Sx, &P, &cur, &gs, &gcap, &d_err, &refill};
+            void *ext_args[] = {&Sx, &P, &cur, &gs, &gcap, &d_err, &fetch};
             (void)hipLaunchKernel(K.ext[count ? 1 : 0], dim3(tgrid), dim3(kBlock), ext_args, 0, ctx->stream);
             if (timed) (void)hipEventRecord(ev[4 * li + 1], ctx->stream);
             hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, ctx->stream, Sx, P, cur, depth, rr_start);
             if (timed) (void)hipEventRecord(ev[4 * li + 2], ctx->stream);
-            void *sh_args[] = {&Sx, &P, &gs, &gcap, &d_err, &refill};
+            void *sh_args[] = {&Sx, &P, &gs, &gcap, &d_err, &fetch};
             (void)hipLaunchKernel(K.sh[count ? 1 : 0], dim3(tgrid), dim3(kBlock), sh_args, 0, ctx->stream);
             if (timed) (void)hipEventRecord(ev[4 * li + 3], ctx->stream);
             cur ^= 1;
@@ -651,7 +672,7 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
         const uint64_t key[8] = {(uint64_t)(uintptr_t)ws.d_render, slots, (uint64_t)max_bounce,
                                  ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
                                  (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
-                                 ((uint64_t)refill << 32) | Sx.tri_batch};
+                                 ((uint64_t)fetch.refill << 48) | ((uint64_t)fetch.chunk << 16) | Sx.tri_batch};
         if (!ws.graph || memcmp(key, ws.graph_key, sizeof(key)) != 0) {
             if (ws.graph) {
                 (void)hipGraphExecDestroy(ws.graph);
@@ -742,23 +763,25 @@ extern "C" int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out) {
     out->refill = ctx->refill;
     out->tri_batch = ctx->tri_batch;
     out->passes_in_flight = ctx->passes_in_flight;
+    out->fetch_chunk = ctx->fetch_chunk;
     return S3DG_OK;
 }
 
 extern "C" int s3dg_set_tuning(s3dg_context *ctx, const s3dg_tuning *t) {
     if (!ctx || !t) return S3DG_ERR_INVALID;
     if (t->traversal_variant >= (uint32_t)kVariantCount || t->refill < 1 || t->refill > 64 || t->tri_batch < 1 ||
-        t->tri_batch > 65)
+        t->tri_batch > 65 || t->fetch_chunk < 64 || t->fetch_chunk > 65536)
         return set_error(ctx, S3DG_ERR_INVALID, "tuning out of range");
     ctx->variant = t->traversal_variant;
     ctx->refill = t->refill;
     ctx->tri_batch = t->tri_batch;
     ctx->passes_in_flight = t->passes_in_flight;
+    ctx->fetch_chunk = t->fetch_chunk;
     return S3DG_OK;
 }
 
-// Coordinate descent over the traversal variant, then the refill threshold,
-// then the triangle batching threshold; each candidate is timed (device time)
+// Coordinate descent over the traversal variant, the refill threshold, the
+// triangle batching threshold and the fetch chunk; each candidate is timed (device time)
 // on its second trial render, after a first one that captures its graph.
 extern "C" int s3dg_autotune(s3dg_scene *sc, const s3dg_render_config *cfg, s3dg_tuning *best, double *best_ms) {
     if (!sc || !cfg || cfg->width <= 0 || cfg->height <= 0) return S3DG_ERR_INVALID;
@@ -789,14 +812,16 @@ extern "C" int s3dg_autotune(s3dg_scene *sc, const s3dg_render_config *cfg, s3dg
     if (rc) return rc;
     const uint32_t refills[] = {16, 24, 32, 48};
     const uint32_t batches[] = {8, 16, 24};
-    for (int axis = 0; axis < 3; ++axis) {
+    const uint32_t chunks[] = {64, 128, 256};
+    for (int axis = 0; axis < 4; ++axis) {
         const int n = axis == 0 ? kVariantCount : axis == 1 ? 4 : 3;
         s3dg_tuning axis_best = cur;
         for (int i = 0; i < n; ++i) {
             s3dg_tuning t = cur;
             if (axis == 0) t.traversal_variant = (uint32_t)i;
             else if (axis == 1) t.refill = refills[i];
-            else t.tri_batch = batches[i];
+            else if (axis == 2) t.tri_batch = batches[i];
+            else t.fetch_chunk = chunks[i];
             if (memcmp(&t, &cur, sizeof(t)) == 0) continue;
             double ms = 0.0;
             rc = time_one(t, ms);

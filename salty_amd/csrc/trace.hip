@@ -35,6 +35,11 @@ extern "C" int s3dg_create(s3dg_context **out, int device) {
         const long v = std::strtol(env, nullptr, 10);
         if (v >= 1 && v <= 64) ctx->refill = (uint32_t)v;
     }
+    // S3DG_FETCH_CHUNK (64..65536) overrides the queue entries a traversal wave takes at a time
+    if (const char *env = std::getenv("S3DG_FETCH_CHUNK")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 64 && v <= 65536) ctx->fetch_chunk = (uint32_t)v;
+    }
     // S3DG_VARIANT (0..2) overrides the traversal kernel variant
     if (const char *env = std::getenv("S3DG_VARIANT")) {
         const long v = std::strtol(env, nullptr, 10);
@@ -111,7 +116,7 @@ template <bool ANY, bool COUNT, int LDS, int WAVES>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
 k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n, s3dg_hit *__restrict__ hits,
         int32_t *__restrict__ shapes, uint2 *gstack, int32_t gcap, int32_t *err, unsigned long long *counters,
-        uint32_t *cursor, uint32_t refill) {
+        uint32_t *cursor, FetchParams fetch) {
     __shared__ uint2 lds[LDS * kBlock];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,7 +126,7 @@ k_trace(SceneDev S, const s3dg_ray *__restrict__ rays, uint32_t n, s3dg_hit *__r
     tr.reset_counts();
     uint32_t i = 0;
     refill_loop(
-        cursor, n, refill,
+        cursor, n, fetch,
         [&](uint32_t k) {
             i = k;
             const float4 a = reinterpret_cast<const float4 *>(rays + i)[0];
@@ -246,12 +251,12 @@ int launch_trace(s3dg_scene *sc, const s3dg_ray *d_rays, uint32_t n, int32_t mod
     uint2 *gs = (uint2 *)sc->ws.d_gstack;
     int32_t cap = stack_capacity(sc->dev.stack_need, K.lds);
     uint32_t *cursor = trace_cursor(sc->ws);
-    uint32_t refill = ctx->refill;
+    FetchParams fetch{ctx->refill, ctx->fetch_chunk};
     SceneDev S = sc->dev;
     S.tri_batch = ctx->tri_batch;
     int32_t *d_err = sc->ws.d_error;
     hipLaunchKernelGGL(k_clear_u32, dim3(1), dim3(64), 0, ctx->stream, cursor);
-    void *args[] = {&S, &d_rays, &n, &d_hits, &d_shapes, &gs, &cap, &d_err, &d_counters, &cursor, &refill};
+    void *args[] = {&S, &d_rays, &n, &d_hits, &d_shapes, &gs, &cap, &d_err, &d_counters, &cursor, &fetch};
     S3DG_HIP(ctx, hipLaunchKernel(kern, dim3(grid), dim3(kBlock), args, 0, ctx->stream));
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;

@@ -515,30 +515,58 @@ __device__ __forceinline__ DHit traverse(const SceneDev &S, const SceneRec &rec,
     return tr.best;
 }
 
-// Persistent-kernel work distribution with lane refill: lanes that have no
-// ray take new ones from `cursor` whenever at least `refill` lanes of the
-// wave are idle.  `begin(i)` sets lane state up for queue entry i and returns
-// false when the entry needs no traversal; `finish()` consumes the result.
+// Persistent-kernel work distribution with lane refill.  Each wave owns a
+// chunk of queue entries [cb, ce) taken from the global cursor `chunk` at a
+// time; whenever at least `refill` lanes are idle they get the next entries
+// of the chunk (the chunk is renewed with one atomic when it runs short), so
+// the cursor sees one atomic per chunk instead of one per refill.
+// `begin(i)` sets lane state up for queue entry i and returns false when the
+// entry needs no traversal; `finish()` consumes the result.  The wave leaves
+// once the cursor has passed n, its chunk is used up and no lane is active.
+struct FetchParams {
+    uint32_t refill;   // idle lanes that trigger a refill (1..64)
+    uint32_t chunk;    // entries taken from the global cursor at a time (>= 64)
+};
+
 template <class Begin, class Step, class Finish>
-__device__ __forceinline__ void refill_loop(uint32_t *cursor, uint32_t n, uint32_t refill, Begin begin, Step step,
+__device__ __forceinline__ void refill_loop(uint32_t *cursor, uint32_t n, FetchParams fp, Begin begin, Step step,
                                             Finish finish) {
-    bool active = false, drained = false;
+    bool active = false;
+    bool exhausted = false;   // the global cursor has passed n
+    uint32_t cb = 0, ce = 0;  // this wave's chunk (entries < n only)
     const uint64_t below = (1ull << __lane_id()) - 1ull;
     for (;;) {
         const uint64_t idle = __ballot(!active);
         const uint32_t want = (uint32_t)__popcll(idle);
-        if (!drained && want >= refill) {
-            uint32_t base = 0;
-            if (__lane_id() == 0) base = atomicAdd(cursor, want);
-            base = __shfl(base, 0);
-            if (base + want >= n) drained = true;
+        const bool drained = exhausted && cb >= ce;
+        if (!drained && want >= fp.refill) {
+            const uint32_t avail = ce - cb;
+            uint32_t nb = 0, ne = 0;
+            if (avail < want && !exhausted) {
+                uint32_t base = 0;
+                if (__lane_id() == 0) base = atomicAdd(cursor, fp.chunk);
+                nb = __shfl(base, 0);
+                ne = nb < n ? (n - nb < fp.chunk ? n : nb + fp.chunk) : nb;
+                exhausted = nb + fp.chunk >= n;
+            }
             if (!active) {
-                const uint32_t i = base + (uint32_t)__popcll(idle & below);
+                const uint32_t r = (uint32_t)__popcll(idle & below);
+                uint32_t i = n;
+                if (r < avail) i = cb + r;
+                else if (nb + (r - avail) < ne) i = nb + (r - avail);
                 if (i < n) active = begin(i);
+            }
+            if (avail >= want) {
+                cb += want;
+            } else if (ne > nb) {
+                cb = nb + (want - avail) < ne ? nb + (want - avail) : ne;
+                ce = ne;
+            } else {
+                cb = ce;
             }
         }
         if (!active) {
-            if (drained && __ballot(active) == 0) break;
+            if (exhausted && cb >= ce && __ballot(active) == 0) break;
             continue;
         }
         if (step()) {
