@@ -30,6 +30,22 @@ TRI_BYTES = 36                 # v0, e0, e1 as Triangle::IsHit reads them
 EXT_IO_BYTES = 4 + 24 + 20     # queue index, ray origin+direction, hit record
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_extend_mesh_scene_hbm_traffic.md")
+
+
+def extend_traffic_bytes():
+    """HBM bytes per k_extend launch of the default workload, from the committed
+    counter profile (FETCH_SIZE and WRITE_SIZE scaled by a plain copy), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("| `k_extend<false"):
+                    return float(line.strip().strip("|").split("|")[-1]) * 1e6
+    except OSError:
+        pass
+    return None
+
+
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
@@ -458,6 +474,8 @@ def main():
 
     rays = rays_ext + rays_sh
     grays = rays / elapsed / 1e9
+    default_workload = (W, H, args.spp, args.depth, args.seed) == (1920, 1080, 64, 12, 0)
+    traffic = extend_traffic_bytes() if default_workload else None
     avg_ext_ms = ext_ms / max(launches, 1)
     bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + EXT_IO_BYTES
     bytes_per_launch = bytes_per_ray * rays_ext / max(launches, 1)
@@ -495,7 +513,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/k_extend_mesh_scene_hbm_traffic.md" if traffic else None,
             "avg_launch_ms": avg_ext_ms,
             "launches": launches,
             "bytes_per_ray": bytes_per_ray,
