@@ -236,9 +236,9 @@ struct Child {
 class Compressor {
 public:
     Compressor(const RefTree &t, const std::vector<Aabb> &prim_boxes, uint32_t tri_offset, bool top,
-               float margin, Compressed &out)
+               float margin, Compressed &out, std::vector<int32_t> &leaf_order)
         : t_(t), pbox_(prim_boxes), tri_offset_(tri_offset), top_(top), margin_(margin), out_(out),
-          run_(top ? 1 : 255) {}
+          leaf_order_(leaf_order), run_(top ? 1 : 255) {}
 
     uint32_t run() {
         const uint32_t root = (uint32_t)out_.nodes.size();
@@ -284,6 +284,7 @@ private:
     bool top_;
     float margin_;
     Compressed &out_;
+    std::vector<int32_t> &leaf_order_;
     int32_t run_;   // most prims one leaf slot may hold
 
     Aabb prims_box(int32_t start, int32_t count) const {
@@ -414,8 +415,8 @@ private:
             g.exp[a] = (uint8_t)(e + 127);
             step[a] = std::ldexp(1.0, e);
         }
-        g.child_base = base;
-        g.flags = top_ ? 1u : 0u;
+        g.child_base = base | (top_ ? kTopLevelBit : 0u);
+        g.leaf_base = tri_offset_ + (uint32_t)leaf_order_.size();
         for (int s = 0; s < 8; ++s) {
             for (int a = 0; a < 3; ++a) { g.qlo[a][s] = 255; g.qhi[a][s] = 0; }
             const int k = by_slot[s];
@@ -428,12 +429,13 @@ private:
             }
             if (kids[k].kind == CH_LEAF) {
                 g.cnt[s] = (uint8_t)kids[k].count;
-                g.leaf_start[s] = tri_offset_ + (uint32_t)kids[k].start;
+                for (int32_t i = 0; i < kids[k].count; ++i) leaf_order_.push_back(kids[k].start + i);
             } else {
                 g.imask |= (uint8_t)(1u << s);
             }
         }
     }
+
 };
 
 }  // namespace
@@ -453,8 +455,9 @@ void build_reference_tree(const std::vector<Aabb> &boxes, int kind, RefTree &out
 }
 
 uint32_t compress_tree(const RefTree &tree, const std::vector<Aabb> &prim_boxes, uint32_t tri_offset,
-                       bool top_level, float margin, Compressed &out) {
-    Compressor c(tree, prim_boxes, tri_offset, top_level, margin, out);
+                       bool top_level, float margin, Compressed &out, std::vector<int32_t> &leaf_order) {
+    leaf_order.clear();
+    Compressor c(tree, prim_boxes, tri_offset, top_level, margin, out, leaf_order);
     return c.run();
 }
 

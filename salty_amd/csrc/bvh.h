@@ -44,12 +44,14 @@ struct Compressed {
 };
 
 // Appends the compressed form of `tree` (whose prims have the boxes
-// prim_boxes, in input order) to out.nodes.  Leaf slots point at
-// tri_offset + depth-first position (or instance index for top-level trees).
-// Returns the root node index.  `margin` is the absolute amount by which every
-// child box is grown before quantisation so that rounding in the kernels'
-// slab tests can never reject a box containing an accepted hit.
+// prim_boxes, in input order) to out.nodes.  The prims are given a new order
+// in which every node's leaves are contiguous: leaf_order[j] is the
+// depth-first position of the prim stored at tri_offset + j (triangles, or
+// shape records for top-level trees).  Returns the root node index.  `margin`
+// is the absolute amount by which every child box is grown before
+// quantisation so that rounding in the kernels' slab tests can never reject a
+// box containing an accepted hit.
 uint32_t compress_tree(const RefTree &tree, const std::vector<Aabb> &prim_boxes, uint32_t tri_offset,
-                       bool top_level, float margin, Compressed &out);
+                       bool top_level, float margin, Compressed &out, std::vector<int32_t> &leaf_order);
 
 }  // namespace s3dg

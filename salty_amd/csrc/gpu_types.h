@@ -9,19 +9,20 @@ namespace s3dg {
 // power-of-two step per axis (2^(exp-127)); quantisation rounds outward so a
 // child box always contains the exact child box it stands for.
 // Internal children of a node are stored contiguously from child_base in slot
-// order; leaf children name a range of the depth-first triangle array.
+// order; the leaves' triangles (or top-level shape records) are stored
+// contiguously from leaf_base in slot order, cnt[s] of them for slot s.
 struct alignas(128) GNode {
     float    origin[3];     //   0
     uint8_t  exp[3];        //  12  biased exponent of the quantisation step
     uint8_t  imask;         //  15  bit s set: slot s is an internal child
-    uint32_t child_base;    //  16  index of the first internal child
-    uint8_t  cnt[8];        //  20  leaf slot: triangle (or instance) count
-    uint32_t flags;         //  28  bit 0: top-level node (leaves are instances)
-    uint32_t leaf_start[8]; //  32  leaf slot: first triangle / instance
-    uint8_t  qlo[3][8];     //  64  quantised lower planes, [axis][slot]
-    uint8_t  qhi[3][8];     //  88  quantised upper planes
-    uint8_t  pad[16];       // 112
+    uint32_t child_base;    //  16  index of the first internal child; bit 31: top-level node
+    uint32_t leaf_base;     //  20  first triangle / shape record of the leaf slots
+    uint8_t  cnt[8];        //  24  leaf slot: triangle (or shape record) count
+    uint8_t  qlo[3][8];     //  32  quantised lower planes, [axis][slot]
+    uint8_t  qhi[3][8];     //  56  quantised upper planes
+    uint8_t  pad[48];       //  80
 };
+constexpr uint32_t kTopLevelBit = 0x80000000u;
 static_assert(sizeof(GNode) == 128, "GNode must be one 128-byte line");
 
 // Triangle as Triangle::IsHit uses it: v0 and the two edges computed once on

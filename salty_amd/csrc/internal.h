@@ -55,6 +55,7 @@ struct SceneDev {
     const GLight *lights;
     const int32_t *inst_of_shape;   // top-level shape index -> instance record
     const SceneRec *recs;
+    const uint32_t *tri_rank;   // depth-first rank of each stored triangle (tie order)
     int32_t stack_need;     // deepest stack the trees can require
     uint32_t tri_batch;     // lanes with pending triangles that keep a step testing
 };
@@ -78,6 +79,7 @@ struct SceneMeshInfo {
     uint32_t tri_count;
     uint32_t root;          // compressed root node
     std::vector<int32_t> order;   // depth-first -> input index
+    std::vector<int32_t> stored_of_input;   // input index -> position in the stored (leaf) order
 };
 
 }  // namespace s3dg
@@ -91,7 +93,7 @@ struct s3dg_scene {
     s3dg_scene_stats stats{};
     // device allocations
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
-    void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr;
+    void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr, *d_rank = nullptr;
     struct {
         uint32_t nodes, tris, insts, mats, lights, shapes;   // element counts of the device arrays
     } counts{};

@@ -226,6 +226,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
 
     std::vector<GTri> tris;
     std::vector<GTriShade> shade;
+    std::vector<uint32_t> rank;   // depth-first (tie) rank of every stored triangle
     Compressed comp;
     std::vector<Aabb> mesh_box(d->mesh_count);
     int32_t blas_depth = 0;
@@ -233,6 +234,8 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     for (uint32_t mi = 0; mi < d->mesh_count; ++mi) total_tris += d->meshes[mi].vertex_count / 3;
     tris.reserve(total_tris);
     shade.reserve(total_tris);
+    rank.reserve(total_tris);
+    std::vector<int32_t> leaf_order;
 
     for (uint32_t mi = 0; mi < d->mesh_count; ++mi) {
         const s3dg_mesh_desc &md = d->meshes[mi];
@@ -266,8 +269,19 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         info.tri_offset = (uint32_t)tris.size();
         info.tri_count = n;
         info.order = tree.order;
-        for (uint32_t k = 0; k < n; ++k) {
-            const int32_t src = tree.order[k];
+        const Aabb &rb = tree.nodes[tree.root].box;
+        mesh_box[mi] = rb;
+        comp.max_depth = 0;
+        info.root = compress_tree(tree, boxes, info.tri_offset, false, scale_margin(rb), comp, leaf_order);
+        blas_depth = std::max(blas_depth, comp.max_depth);
+        // triangles in the order of the compressed leaves; the depth-first
+        // position of the reference tree stays their tie rank
+        info.stored_of_input.assign(n, 0);
+        for (uint32_t j = 0; j < n; ++j) {
+            const int32_t dfs = leaf_order[j];
+            const int32_t src = tree.order[dfs];
+            info.stored_of_input[src] = (int32_t)j;
+            rank.push_back(info.tri_offset + (uint32_t)dfs);
             const s3dg_vertex *v = md.vertices + 3 * src;
             GTri t;
             GTriShade s;
@@ -290,11 +304,6 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
             tris.push_back(t);
             shade.push_back(s);
         }
-        const Aabb &rb = tree.nodes[tree.root].box;
-        mesh_box[mi] = rb;
-        comp.max_depth = 0;
-        info.root = compress_tree(tree, boxes, info.tri_offset, false, scale_margin(rb), comp);
-        blas_depth = std::max(blas_depth, comp.max_depth);
         sc->meshes.push_back(std::move(info));
     }
 
@@ -383,14 +392,16 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     } else {
         RefTree top;
         build_reference_tree(wbox, 8, top);
-        inst_dev.resize(d->shape_count);
-        for (uint32_t k = 0; k < d->shape_count; ++k) {
-            inst_dev[k] = inst[top.order[k]];
-            inst_pos[top.order[k]] = (int32_t)k;
-        }
         Aabb all = top.nodes[top.root].box;
         comp.max_depth = 0;
-        sc->rec.root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp);
+        std::vector<int32_t> top_order;
+        sc->rec.root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp, top_order);
+        inst_dev.resize(d->shape_count);
+        for (uint32_t k = 0; k < d->shape_count; ++k) {
+            const int32_t shape = top.order[top_order[k]];
+            inst_dev[k] = inst[shape];
+            inst_pos[shape] = (int32_t)k;
+        }
         tlas_depth = comp.max_depth;
         sc->rec.single_shape = -1;
         sc->stats.tlas_nodes = 0;
@@ -429,11 +440,8 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
             }
             g.type = LIGHT_QUAD;
             g.count = l.triangle_count;
-            for (int k = 0; k < l.triangle_count; ++k) {
-                const int32_t want = l.first_triangle + k;
-                for (uint32_t p = 0; p < mi.tri_count; ++p)
-                    if (mi.order[p] == want) { g.tri[k] = (int32_t)(mi.tri_offset + p); break; }
-            }
+            for (int k = 0; k < l.triangle_count; ++k)
+                g.tri[k] = (int32_t)(mi.tri_offset + (uint32_t)mi.stored_of_input[l.first_triangle + k]);
             for (int a = 0; a < 3; ++a) {
                 g.corner[a] = l.corner[a];
                 g.e1[a] = l.edge1[a];
@@ -486,7 +494,8 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         (rc = upload(ctx, &sc->d_shade, shade, total)) || (rc = upload(ctx, &sc->d_insts, inst_dev, total)) ||
         (rc = upload(ctx, &sc->d_mats, mats, total)) || (rc = upload(ctx, &sc->d_lights, lights, total)) ||
         (rc = upload(ctx, &sc->d_inst_of_shape, inst_pos, total)) ||
-        (rc = upload(ctx, &sc->d_rec, std::vector<SceneRec>{sc->rec}, total))) {
+        (rc = upload(ctx, &sc->d_rec, std::vector<SceneRec>{sc->rec}, total)) ||
+        (rc = upload(ctx, &sc->d_rank, rank, total))) {
         s3dg_scene_destroy(sc);
         return rc;
     }
@@ -502,6 +511,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     dv.lights = (const GLight *)sc->d_lights;
     dv.inst_of_shape = (const int32_t *)sc->d_inst_of_shape;
     dv.recs = (const SceneRec *)sc->d_rec;
+    dv.tri_rank = (const uint32_t *)sc->d_rank;
 
     sc->stats.triangles = tris.size();
     sc->stats.blas_nodes = comp.nodes.size();
@@ -516,7 +526,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
 extern "C" int s3dg_scene_destroy(s3dg_scene *sc) {
     if (!sc) return S3DG_ERR_INVALID;
     void *ptrs[] = {sc->d_nodes, sc->d_tris, sc->d_shade, sc->d_insts, sc->d_mats, sc->d_lights,
-                    sc->d_inst_of_shape, sc->d_rec};
+                    sc->d_inst_of_shape, sc->d_rec, sc->d_rank};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     workspace_free(sc->ws);

@@ -28,6 +28,7 @@ struct s3dg_scene_set {
     std::vector<SceneRec> recs;
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
     void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr, *d_recs = nullptr;
+    void *d_rank = nullptr;
     Workspace ws;
 };
 
@@ -37,10 +38,9 @@ __global__ void k_rebase_nodes(GNode *__restrict__ nodes, uint32_t n, uint32_t n
                                uint32_t inst_off) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         GNode &g = nodes[i];
-        if (g.imask) g.child_base += node_off;
-        const uint32_t off = (g.flags & 1u) ? inst_off : tri_off;
-        for (int s = 0; s < 8; ++s)
-            if (g.cnt[s] && !((g.imask >> s) & 1u)) g.leaf_start[s] += off;
+        const uint32_t top = g.child_base & kTopLevelBit;
+        if (g.imask) g.child_base = ((g.child_base & ~kTopLevelBit) + node_off) | top;
+        g.leaf_base += top ? inst_off : tri_off;
     }
 }
 
@@ -72,6 +72,10 @@ __global__ void k_rebase_lights(GLight *__restrict__ lights, uint32_t n, int32_t
     }
 }
 
+__global__ void k_add_u32(uint32_t *__restrict__ a, uint32_t n, uint32_t off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] += off;
+}
+
 __global__ void k_add_i32(int32_t *__restrict__ a, uint32_t n, int32_t off) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] += off;
 }
@@ -79,8 +83,8 @@ __global__ void k_add_i32(int32_t *__restrict__ a, uint32_t n, int32_t off) {
 unsigned grid_for(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 4096u)); }
 
 void free_set(s3dg_scene_set *set) {
-    void *ptrs[] = {set->d_nodes, set->d_tris, set->d_shade, set->d_insts,
-                    set->d_mats, set->d_lights, set->d_inst_of_shape, set->d_recs};
+    void *ptrs[] = {set->d_nodes, set->d_tris,   set->d_shade,          set->d_insts, set->d_mats,
+                    set->d_lights, set->d_inst_of_shape, set->d_recs, set->d_rank};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     workspace_free(set->ws);
@@ -126,7 +130,8 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         (e = alloc(&set->d_mats, (size_t)tot.mats * sizeof(GMat))) != hipSuccess ||
         (e = alloc(&set->d_lights, (size_t)tot.lights * sizeof(GLight))) != hipSuccess ||
         (e = alloc(&set->d_inst_of_shape, (size_t)tot.shapes * sizeof(int32_t))) != hipSuccess ||
-        (e = alloc(&set->d_recs, (size_t)n * sizeof(SceneRec))) != hipSuccess) {
+        (e = alloc(&set->d_recs, (size_t)n * sizeof(SceneRec))) != hipSuccess ||
+        (e = alloc(&set->d_rank, (size_t)tot.tris * sizeof(uint32_t))) != hipSuccess) {
         free_set(set);
         return hip_check(ctx, e, "hipMalloc(scene set)");
     }
@@ -145,6 +150,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         GTriShade *shade = (GTriShade *)set->d_shade + o.tris;
         GInst *insts = (GInst *)set->d_insts + o.insts;
         GLight *lights = (GLight *)set->d_lights + o.lights;
+        uint32_t *rank = (uint32_t *)set->d_rank + o.tris;
         int32_t *ios = (int32_t *)set->d_inst_of_shape + o.shapes;
         struct {
             void *dst;
@@ -156,7 +162,8 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
                       {insts, sc->d_insts, (size_t)c.insts * sizeof(GInst)},
                       {(GMat *)set->d_mats + o.mats, sc->d_mats, (size_t)c.mats * sizeof(GMat)},
                       {lights, sc->d_lights, (size_t)c.lights * sizeof(GLight)},
-                      {ios, sc->d_inst_of_shape, (size_t)c.shapes * sizeof(int32_t)}};
+                      {ios, sc->d_inst_of_shape, (size_t)c.shapes * sizeof(int32_t)},
+                      {rank, sc->d_rank, (size_t)c.tris * sizeof(uint32_t)}};
         for (auto &cp : copies)
             if (cp.bytes) {
                 e = hipMemcpyAsync(cp.dst, cp.src, cp.bytes, hipMemcpyDeviceToDevice, s);
@@ -171,9 +178,11 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         if (c.insts)
             hipLaunchKernelGGL(k_rebase_insts, dim3(grid_for(c.insts)), dim3(256), 0, s, insts, c.insts, o.nodes,
                                (int32_t)o.shapes, (int32_t)o.mats);
-        if (c.tris)
+        if (c.tris) {
             hipLaunchKernelGGL(k_rebase_shade, dim3(grid_for(c.tris)), dim3(256), 0, s, shade, c.tris,
                                (int32_t)o.mats);
+            hipLaunchKernelGGL(k_add_u32, dim3(grid_for(c.tris)), dim3(256), 0, s, rank, c.tris, o.tris);
+        }
         if (c.lights)
             hipLaunchKernelGGL(k_rebase_lights, dim3(grid_for(c.lights)), dim3(256), 0, s, lights, c.lights,
                                (int32_t)o.tris, (int32_t)o.insts, (int32_t)o.shapes);
@@ -202,6 +211,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
     dv.lights = (const GLight *)set->d_lights;
     dv.inst_of_shape = (const int32_t *)set->d_inst_of_shape;
     dv.recs = (const SceneRec *)set->d_recs;
+    dv.tri_rank = (const uint32_t *)set->d_rank;
     dv.stack_need = stack_need;
     dv.tri_batch = ctx->tri_batch;
     *out = set;

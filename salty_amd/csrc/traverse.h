@@ -213,6 +213,19 @@ struct Stack {
     }
 };
 
+// First triangle (or shape record) of leaf slot s of a node: leaf_base plus
+// the counts of the slots below s (count bytes of slots 0-3 in lo, 4-7 in hi;
+// inner slots count 0).
+__device__ __forceinline__ uint32_t slot_first(uint32_t leaf_base, uint32_t lo, uint32_t hi, uint32_t s) {
+    const uint32_t mlo = s >= 4u ? lo : (lo & ((1u << (8u * s)) - 1u));
+    const uint32_t mhi = s <= 4u ? 0u : (hi & ((1u << (8u * (s - 4u))) - 1u));
+    return __builtin_amdgcn_sad_u8(mlo, 0u, __builtin_amdgcn_sad_u8(mhi, 0u, leaf_base));
+}
+
+__device__ __forceinline__ uint32_t slot_count(uint32_t lo, uint32_t hi, uint32_t s) {
+    return ((s < 4u ? lo : hi) >> ((s & 3u) * 8u)) & 0xffu;
+}
+
 // Incremental traversal of one ray.  start() sets the ray up; each step()
 // tests at most one pending triangle (more while most lanes of the wave have
 // triangles pending), visits at most one node and pops a stack entry when the
@@ -230,7 +243,8 @@ struct Tracer {
     float tmin, tmax, lo_cull, bound;
     uint32_t node;       // next node to visit (when have_node)
     uint32_t tri_cur, tri_end;   // triangles of the current leaf still to test
-    uint32_t leaf_node, leaf_mask;   // node whose hit leaves (mask of slots) follow
+    uint32_t leaf_mask;          // further hit leaf slots of the node below
+    uint32_t leaf_base, leaf_lo, leaf_hi;   // that node's leaf_base and count words
     int32_t cur_shape;
     bool have_node;
     uint32_t visits, tests;
@@ -250,7 +264,7 @@ struct Tracer {
         have_node = true;
         tri_cur = tri_end = 0;
         leaf_mask = 0;
-        leaf_node = 0;
+        leaf_base = leaf_lo = leaf_hi = 0;
     }
 
     __device__ __forceinline__ void take(float t, float u, float v, int32_t prim, int32_t shape) {
@@ -262,14 +276,12 @@ struct Tracer {
         bound = fminf(cull_hi(tmax), cull_hi(t));
     }
 
-    // Moves to the triangles of the next pending leaf of leaf_node.
-    __device__ __forceinline__ void next_leaf(const SceneDev &S) {
+    // Moves to the triangles of the next pending leaf slot.
+    __device__ __forceinline__ void next_leaf() {
         const uint32_t s = (uint32_t)__builtin_ctz(leaf_mask);
         leaf_mask &= leaf_mask - 1u;
-        const uint32_t *w = reinterpret_cast<const uint32_t *>(S.nodes + leaf_node);
-        const uint32_t cnts = w[5 + (s >> 2)];   // cnt bytes of slots 0-3, 4-7
-        tri_cur = w[8 + s];
-        tri_end = tri_cur + ((cnts >> ((s & 3u) * 8u)) & 0xffu);
+        tri_cur = slot_first(leaf_base, leaf_lo, leaf_hi, s);
+        tri_end = tri_cur + slot_count(leaf_lo, leaf_hi, s);
     }
 
     // Tests triangle tri_cur.  Returns true when an any-hit search is over.
@@ -285,10 +297,11 @@ struct Tracer {
             // the first triangle in shape order, then depth-first order, wins a tie
             const bool win = t < best.t ||
                              (t == best.t && best.prim >= 0 &&
-                              (cur_shape < best.shape || (cur_shape == best.shape && (int32_t)i < best.prim)));
+                              (cur_shape < best.shape ||
+                               (cur_shape == best.shape && S.tri_rank[i] < S.tri_rank[best.prim])));
             if (win) take(t, u, v, (int32_t)i, cur_shape);
         }
-        if (tri_cur == tri_end && leaf_mask) next_leaf(S);
+        if (tri_cur == tri_end && leaf_mask) next_leaf();
         return false;
     }
 
@@ -311,9 +324,12 @@ struct Tracer {
             have_node = true;
         } else if (kind == ENT_LEAVES) {
             st.sp--;
-            leaf_node = e.x;
+            const uint4 h1 = reinterpret_cast<const uint4 *>(S.nodes + e.x)[1];
+            leaf_base = h1.y;
+            leaf_lo = h1.z;
+            leaf_hi = h1.w;
             leaf_mask = e.y & 0xffu;
-            next_leaf(S);
+            next_leaf();
         } else if (kind == ENT_INST) {
             st.sp--;
             const GInst &in = S.insts[e.x];
@@ -349,8 +365,11 @@ struct Tracer {
     // the pending triangles (or a stacked leaf group when triangles are still
     // pending), the nearest hit inner child is visited next and the others are
     // pushed as one group.
+    // The node's header lines (origin, exponents, masks, child and leaf bases,
+    // counts), requested before a step's triangle tests; the plane lines are
+    // loaded by visit() itself.
     struct NodeLines {
-        uint4 h0, h1, q0, q1, q2;
+        uint4 h0, h1;
     };
 
     __device__ __forceinline__ NodeLines fetch(const SceneDev &S) const {
@@ -358,16 +377,18 @@ struct Tracer {
         NodeLines n;
         n.h0 = np[0];
         n.h1 = np[1];
-        n.q0 = np[4];
-        n.q1 = np[5];
-        n.q2 = np[6];
         return n;
     }
 
+    // Visits `node`: slab tests of its eight children; the hit leaves become
+    // the pending triangles (or a stacked leaf group when triangles are still
+    // pending), the nearest hit inner child is visited next and the others are
+    // pushed as one group.
     template <class ST>
     __device__ __forceinline__ void visit(const SceneDev &S, ST &st, const NodeLines &nl) {
+        const uint4 h0 = nl.h0, h1 = nl.h1;
         const uint4 *np = reinterpret_cast<const uint4 *>(S.nodes + node);
-        const uint4 h0 = nl.h0, h1 = nl.h1, q0 = nl.q0, q1 = nl.q1, q2 = nl.q2;
+        const uint4 q0 = np[2], q1 = np[3], q2 = np[4];   // planes: lo x, lo y | lo z, hi x | hi y, hi z
         if (COUNT) ++visits;
         const float ox = __uint_as_float(h0.x), oy = __uint_as_float(h0.y), oz = __uint_as_float(h0.z);
         const uint32_t imask = h0.w >> 24;
@@ -376,9 +397,9 @@ struct Tracer {
         const float sz = __uint_as_float(((h0.w >> 16) & 0xffu) << 23);
         const float ax = sx * pre.ix, ay = sy * pre.iy, az = sz * pre.iz;
         const float bx = (ox - ray.ox) * pre.ix, by = (oy - ray.oy) * pre.iy, bz = (oz - ray.oz) * pre.iz;
-        const uint32_t child_base = h1.x;
-        const uint32_t cnt_lo = h1.y, cnt_hi = h1.z;
-        const uint32_t top = h1.w & 1u;
+        const uint32_t top = h1.x & kTopLevelBit;
+        const uint32_t child_base = h1.x & ~kTopLevelBit;
+        const uint32_t lbase = h1.y, cnt_lo = h1.z, cnt_hi = h1.w;
         // near / far planes per axis from the direction signs: for a positive
         // direction component the low plane is entered first
         const bool fx = (pre.oct & 1u) != 0, fy = (pre.oct & 2u) != 0, fz = (pre.oct & 4u) != 0;
@@ -408,10 +429,7 @@ struct Tracer {
         // valid slots: internal children or leaves with a non-zero count
         uint32_t valid = imask;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const uint32_t c = ((s < 4 ? cnt_lo : cnt_hi) >> ((s & 3) * 8)) & 0xffu;
-            valid |= (c != 0u ? 1u : 0u) << s;
-        }
+        for (int s = 0; s < 8; ++s) valid |= (slot_count(cnt_lo, cnt_hi, (uint32_t)s) != 0u ? 1u : 0u) << s;
         hits &= valid;
         const uint32_t leaves = hits & ~imask;
         if (leaves) {
@@ -423,14 +441,14 @@ struct Tracer {
                     const uint32_t k = 31u - __builtin_clz(lk);
                     lk &= ~(1u << k);
                     const uint32_t s = k ^ pre.oct;
-                    st.push(make_uint2(reinterpret_cast<const uint32_t *>(np)[8 + s], ENT_INST << 30));
+                    st.push(make_uint2(slot_first(lbase, cnt_lo, cnt_hi, s), ENT_INST << 30));
                 }
             } else if (tri_cur == tri_end && leaf_mask == 0) {
-                const uint32_t s = (uint32_t)__builtin_ctz(leaves);
-                leaf_node = node;
-                leaf_mask = leaves & (leaves - 1u);
-                tri_cur = reinterpret_cast<const uint32_t *>(np)[8 + s];
-                tri_end = tri_cur + (((s < 4 ? cnt_lo : cnt_hi) >> ((s & 3u) * 8u)) & 0xffu);
+                leaf_base = lbase;
+                leaf_lo = cnt_lo;
+                leaf_hi = cnt_hi;
+                leaf_mask = leaves;
+                next_leaf();
             } else {
                 st.push(make_uint2(node, (ENT_LEAVES << 30) | leaves));
             }
@@ -470,7 +488,7 @@ struct Tracer {
         if (have_node) {
             // keeps the compiler from evaluating node fields before the
             // triangle tests (which would wait for the node lines there)
-            __asm__ volatile("" : "+v"(nl.h0.w), "+v"(nl.h1.w));
+            __asm__ volatile("" : "+v"(nl.h0.w), "+v"(nl.h1.x));
             visit(S, st, nl);
         }
         return !have_node && tri_cur >= tri_end && leaf_mask == 0 && st.sp == 0;
