@@ -312,6 +312,9 @@ typedef struct s3dg_tuning {
     uint32_t tri_batch;          /* lanes with pending triangles that keep a step testing (1..65) */
     uint32_t passes_in_flight;   /* path capacity used when a render passes 0 (0 = automatic) */
     uint32_t fetch_chunk;        /* queue entries a traversal wave takes at a time (64..65536) */
+    uint32_t leaf_size;          /* most triangles per leaf of the traversal trees of scenes
+                                    created afterwards (1..8); 0: traverse the reference-shaped
+                                    tree.  Tied hits are ranked by the reference tree either way */
 } s3dg_tuning;
 
 int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out);

@@ -220,6 +220,139 @@ private:
 };
 
 // ---------------------------------------------------------------------------
+// Traversal tree: binned SAH over the prim centres on all three axes, binary,
+// leaves of at most `max_leaf` prims.  Only its shape differs from the
+// reference tree; which hit wins a tie is decided by the reference tree's
+// depth-first rank, so the traversal tree changes the speed, never a result.
+// ---------------------------------------------------------------------------
+class TraversalBuilder {
+public:
+    static constexpr int kBins = 32;
+
+    TraversalBuilder(const std::vector<Aabb> &boxes, int max_leaf, float node_cost, RefTree &t)
+        : boxes_(boxes), max_leaf_(std::max(1, max_leaf)), node_cost_(node_cost), t_(t) {
+        const size_t n = boxes.size();
+        ctr_.resize(3 * n);
+        for (size_t i = 0; i < n; ++i)
+            for (int a = 0; a < 3; ++a) ctr_[3 * i + a] = 0.5f * (boxes[i].lo[a] + boxes[i].hi[a]);
+        t_.order.resize(n);
+        for (size_t i = 0; i < n; ++i) t_.order[i] = (int32_t)i;
+    }
+
+    int32_t build() {
+        const int32_t root = new_node(0, (int32_t)boxes_.size());
+        std::vector<int32_t> todo{root};
+        while (!todo.empty()) {
+            const int32_t ni = todo.back();
+            todo.pop_back();
+            int32_t left_count;
+            if (!split(t_.nodes[ni], left_count)) continue;   // stays a leaf
+            const int32_t b = t_.nodes[ni].begin, c = t_.nodes[ni].count;
+            const int32_t l = new_node(b, left_count), r = new_node(b + left_count, c - left_count);
+            RefNode &n = t_.nodes[ni];
+            n.kind = 2;
+            n.first_child = (int32_t)t_.children.size();
+            n.child_count = 2;
+            t_.children.push_back(l);
+            t_.children.push_back(r);
+            todo.push_back(r);
+            todo.push_back(l);
+        }
+        return root;
+    }
+
+private:
+    const std::vector<Aabb> &boxes_;
+    const int max_leaf_;
+    const float node_cost_;   // cost of a binary node relative to one prim test
+    RefTree &t_;
+    std::vector<float> ctr_;
+
+    int32_t new_node(int32_t begin, int32_t count) {
+        RefNode n;
+        n.kind = 0;
+        n.first_child = -1;
+        n.child_count = 0;
+        n.begin = begin;
+        n.count = count;
+        n.box = empty_box();
+        for (int32_t i = 0; i < count; ++i) grow(n.box, boxes_[t_.order[begin + i]]);
+        t_.nodes.push_back(n);
+        return (int32_t)t_.nodes.size() - 1;
+    }
+
+    // Chooses a split of the node's prims and partitions them (left side
+    // first); false when the node stays a leaf.
+    bool split(const RefNode &n, int32_t &left_count) {
+        const int32_t b = n.begin, c = n.count;
+        if (c <= 1) return false;
+        float cmin[3] = {kFMax, kFMax, kFMax}, cmax[3] = {-kFMax, -kFMax, -kFMax};
+        for (int32_t i = 0; i < c; ++i) {
+            const float *p = &ctr_[3 * (size_t)t_.order[b + i]];
+            for (int a = 0; a < 3; ++a) {
+                cmin[a] = std::min(cmin[a], p[a]);
+                cmax[a] = std::max(cmax[a], p[a]);
+            }
+        }
+        const float node_sa = surface_area(n.box);
+        float best = kFMax;
+        int best_axis = -1, best_bin = 0;
+        float scale[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) {
+            const float ext = cmax[a] - cmin[a];
+            if (!(ext > 0.0f)) continue;
+            scale[a] = (float)kBins / ext * (1.0f - 1e-6f);
+            Aabb bb[kBins];
+            int32_t bc[kBins];
+            for (int k = 0; k < kBins; ++k) { bb[k] = empty_box(); bc[k] = 0; }
+            for (int32_t i = 0; i < c; ++i) {
+                const int32_t pi = t_.order[b + i];
+                const int k = bin_of(ctr_[3 * (size_t)pi + a], cmin[a], scale[a]);
+                ++bc[k];
+                grow(bb[k], boxes_[pi]);
+            }
+            float right_cost[kBins];
+            Aabb acc = empty_box();
+            int32_t cnt = 0;
+            for (int k = kBins - 1; k > 0; --k) {
+                grow(acc, bb[k]);
+                cnt += bc[k];
+                right_cost[k] = cnt ? surface_area(acc) * (float)cnt : 0.0f;
+            }
+            acc = empty_box();
+            cnt = 0;
+            for (int k = 0; k < kBins - 1; ++k) {
+                grow(acc, bb[k]);
+                cnt += bc[k];
+                if (cnt == 0 || cnt == c) continue;
+                const float cost = surface_area(acc) * (float)cnt + right_cost[k + 1];
+                if (cost < best) { best = cost; best_axis = a; best_bin = k + 1; }
+            }
+        }
+        const float split_cost = node_sa > 0.0f && best < kFMax ? node_cost_ + best / node_sa : kFMax;
+        if (c <= max_leaf_ && (float)c <= split_cost) return false;
+        if (best_axis < 0) {
+            // every centre coincides: halve the range as it stands
+            left_count = c / 2;
+            return true;
+        }
+        const int a = best_axis;
+        const float lo = cmin[a], sc = scale[a];
+        int32_t *first = t_.order.data() + b;
+        int32_t *mid = std::partition(first, first + c, [&](int32_t pi) {
+            return bin_of(ctr_[3 * (size_t)pi + a], lo, sc) < best_bin;
+        });
+        left_count = (int32_t)(mid - first);
+        return true;
+    }
+
+    static int bin_of(float x, float lo, float scale) {
+        const int k = (int)((x - lo) * scale);
+        return k < 0 ? 0 : (k >= kBins ? kBins - 1 : k);
+    }
+};
+
+// ---------------------------------------------------------------------------
 // Re-layout into 8-wide compressed nodes.
 // ---------------------------------------------------------------------------
 enum { CH_LEAF = 0, CH_TREE = 1, CH_RANGE = 2 };
@@ -452,6 +585,17 @@ void build_reference_tree(const std::vector<Aabb> &boxes, int kind, RefTree &out
     if (kind == 2) out.root = b.bvh2(0, n);
     else if (kind == 4) out.root = b.bvh4(0, n);
     else out.root = b.bvh8(0, n);
+}
+
+void build_traversal_tree(const std::vector<Aabb> &boxes, int max_leaf, RefTree &out, float node_cost) {
+    out.nodes.clear();
+    out.children.clear();
+    out.order.clear();
+    out.root = -1;
+    if (boxes.empty()) return;
+    out.nodes.reserve(boxes.size() / 2 + 16);
+    TraversalBuilder b(boxes, max_leaf, node_cost, out);
+    out.root = b.build();
 }
 
 uint32_t compress_tree(const RefTree &tree, const std::vector<Aabb> &prim_boxes, uint32_t tri_offset,

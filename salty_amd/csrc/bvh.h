@@ -1,6 +1,7 @@
 // bvh.h -- host side of the acceleration structure: the reference binned-SAH
 // tree (BVH8 with BVH4/Leaf fall-backs, or BVH4 / BVH2) built over triangle
-// bounding boxes, and its re-layout into compressed 128-byte 8-wide nodes.
+// bounding boxes, whose depth-first order ranks tied hits; the tree the kernels
+// traverse; and the re-layout of either into compressed 128-byte 8-wide nodes.
 #pragma once
 #include <stdint.h>
 #include <vector>
@@ -36,6 +37,13 @@ struct RefTree {
 // src/src/s3d_bvh4.cpp:237-423, src/src/s3d_bvh2.cpp:174-274).  Prim centres
 // are box centres, as Triangle::GetCenter returns (s3d_triangle.cpp:131-132).
 void build_reference_tree(const std::vector<Aabb> &boxes, int kind, RefTree &out);
+
+// Builds the tree the kernels traverse: binned SAH on all three axes (32
+// bins), binary, leaves of at most max_leaf prims (more only where every centre
+// coincides and the range cannot be split).  RefNode kinds are 2 and 0; order
+// is the depth-first prim order of this tree.
+// node_cost is the SAH cost of a binary node relative to one prim test.
+void build_traversal_tree(const std::vector<Aabb> &boxes, int max_leaf, RefTree &out, float node_cost = 0.5f);
 
 // Output of the re-layout.
 struct Compressed {

@@ -28,6 +28,9 @@ struct s3dg_context {
     uint32_t variant = 1;
     // passes traced together when a render leaves it to the library (0: automatic)
     uint32_t passes_in_flight = 0;
+    // most triangles per leaf of the traversal trees of scenes created from
+    // now on (1..8); 0 traverses the reference-shaped tree instead
+    uint32_t leaf_size = 2;
 };
 
 namespace s3dg {

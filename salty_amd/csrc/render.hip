@@ -4,7 +4,8 @@
 //
 // One pass adds one pixel-centre sample per pixel (like one iteration of the
 // reference's progressive loop).  Several passes are traced together: path
-// slot p holds pixel p % npix of pass p / npix, so the deep bounces, where
+// slot p holds the (p % npix)-th pixel, in the band order of slot_pixel, of
+// pass p / npix, so the deep bounces, where
 // Russian roulette has thinned the queues, still fill the GPU.  Per bounce:
 // extend (closest hit of the queued rays), shade (emission, light sample,
 // material sample, Russian roulette, compaction into the next queue and the
@@ -82,7 +83,8 @@ size_t carve_bytes(uint32_t n) { return path_bytes(n) + 40 * 256; }
 
 // A batch of paths is a list of segments: consecutive passes of one frame
 // (a scene, its resolution, seed and accumulation buffer).  Path slot p of
-// segment g is pixel (p - path0) % npix of pass pass0 + (p - path0) / npix.
+// segment g is pixel slot_pixel((p - path0) % npix) of pass
+// pass0 + (p - path0) / npix.
 struct Seg {
     uint32_t path0;     // first path slot
     uint32_t npix;      // width * height
@@ -97,6 +99,19 @@ struct Seg {
 struct BatchDesc {
     uint32_t nseg, paths, pixels, pad;
 };
+
+// Pixel (row-major index) of the j-th slot of a pass: the image is cut into
+// bands of kBand rows and each band is walked column by column, so the 64
+// paths of a wave start in a kBand x 8 block of pixels and their primary and
+// first shadow rays stay coherent.  A bijection on [0, W*H) for any W, H.
+constexpr uint32_t kBand = 8;
+__device__ __forceinline__ uint32_t slot_pixel(uint32_t j, uint32_t W, uint32_t H) {
+    const uint32_t band = j / (kBand * W);
+    const uint32_t y0 = band * kBand;
+    const uint32_t bh = min(kBand, H - y0);
+    const uint32_t r = j - y0 * W;
+    return (y0 + r % bh) * W + r / bh;
+}
 
 // Segment holding path slot (or accumulation thread) i: binary search on
 // path0 (or pix0); segments are sorted by both.
@@ -174,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const 
     for (uint32_t p = gtid; p < npaths; p += gridDim.x * blockDim.x) {
         const Seg &g = find_seg<false>(segs, nseg, p);
         const uint32_t local = p - g.path0;
-        const uint32_t pix = local % g.npix;
+        const uint32_t pix = slot_pixel(local % g.npix, (uint32_t)g.width, (uint32_t)g.height);
         const uint32_t pass = g.pass0 + local / g.npix;
         const uint32_t x = pix % (uint32_t)g.width, y = pix / (uint32_t)g.width;
         Pcg rng;
@@ -417,11 +432,11 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, const BatchDes
     const uint32_t npixels = bd->pixels, nseg = bd->nseg;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < npixels; t += gridDim.x * blockDim.x) {
         const Seg &g = find_seg<true>(segs, nseg, t);
-        const uint32_t pix = t - g.pix0;
-        float *acc = g.accum + 3 * (size_t)pix;
+        const uint32_t j = t - g.pix0;
+        float *acc = g.accum + 3 * (size_t)slot_pixel(j, (uint32_t)g.width, (uint32_t)g.height);
         float r = acc[0], gg = acc[1], b = acc[2];
         for (uint32_t s = 0; s < g.passes; ++s) {
-            const size_t q = (size_t)g.path0 + (size_t)s * g.npix + pix;
+            const size_t q = (size_t)g.path0 + (size_t)s * g.npix + j;
             r = r + P.lr[q];
             gg = gg + P.lg[q];
             b = b + P.lb[q];
@@ -764,19 +779,21 @@ extern "C" int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out) {
     out->tri_batch = ctx->tri_batch;
     out->passes_in_flight = ctx->passes_in_flight;
     out->fetch_chunk = ctx->fetch_chunk;
+    out->leaf_size = ctx->leaf_size;
     return S3DG_OK;
 }
 
 extern "C" int s3dg_set_tuning(s3dg_context *ctx, const s3dg_tuning *t) {
     if (!ctx || !t) return S3DG_ERR_INVALID;
     if (t->traversal_variant >= (uint32_t)kVariantCount || t->refill < 1 || t->refill > 64 || t->tri_batch < 1 ||
-        t->tri_batch > 65 || t->fetch_chunk < 64 || t->fetch_chunk > 65536)
+        t->tri_batch > 65 || t->fetch_chunk < 64 || t->fetch_chunk > 65536 || t->leaf_size > 8)
         return set_error(ctx, S3DG_ERR_INVALID, "tuning out of range");
     ctx->variant = t->traversal_variant;
     ctx->refill = t->refill;
     ctx->tri_batch = t->tri_batch;
     ctx->passes_in_flight = t->passes_in_flight;
     ctx->fetch_chunk = t->fetch_chunk;
+    ctx->leaf_size = t->leaf_size;
     return S3DG_OK;
 }
 

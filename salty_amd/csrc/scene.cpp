@@ -263,25 +263,30 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
                 b.hi[a] = hi;
             }
         }
-        RefTree tree;
+        // the reference tree fixes the tie rank (its depth-first order); the
+        // kernels traverse the traversal tree unless leaf_size is 0
+        RefTree tree, trav;
         build_reference_tree(boxes, kind, tree);
+        if (ctx->leaf_size) build_traversal_tree(boxes, (int)ctx->leaf_size, trav);
+        const RefTree &walk = ctx->leaf_size ? trav : tree;
         SceneMeshInfo info;
         info.tri_offset = (uint32_t)tris.size();
         info.tri_count = n;
         info.order = tree.order;
+        std::vector<int32_t> ref_pos(n);
+        for (uint32_t j = 0; j < n; ++j) ref_pos[tree.order[j]] = (int32_t)j;
         const Aabb &rb = tree.nodes[tree.root].box;
         mesh_box[mi] = rb;
         comp.max_depth = 0;
-        info.root = compress_tree(tree, boxes, info.tri_offset, false, scale_margin(rb), comp, leaf_order);
+        info.root = compress_tree(walk, boxes, info.tri_offset, false, scale_margin(rb), comp, leaf_order);
         blas_depth = std::max(blas_depth, comp.max_depth);
-        // triangles in the order of the compressed leaves; the depth-first
-        // position of the reference tree stays their tie rank
+        // triangles in the order of the compressed leaves, ranked by their
+        // depth-first position in the reference tree
         info.stored_of_input.assign(n, 0);
         for (uint32_t j = 0; j < n; ++j) {
-            const int32_t dfs = leaf_order[j];
-            const int32_t src = tree.order[dfs];
+            const int32_t src = walk.order[leaf_order[j]];
             info.stored_of_input[src] = (int32_t)j;
-            rank.push_back(info.tri_offset + (uint32_t)dfs);
+            rank.push_back(info.tri_offset + (uint32_t)ref_pos[src]);
             const s3dg_vertex *v = md.vertices + 3 * src;
             GTri t;
             GTriShade s;
@@ -390,8 +395,10 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         inst_dev = inst;
         inst_pos[0] = 0;
     } else {
+        // ties between shapes go by shape index, whatever the tree
         RefTree top;
-        build_reference_tree(wbox, 8, top);
+        if (ctx->leaf_size) build_traversal_tree(wbox, 1, top);
+        else build_reference_tree(wbox, 8, top);
         Aabb all = top.nodes[top.root].box;
         comp.max_depth = 0;
         std::vector<int32_t> top_order;

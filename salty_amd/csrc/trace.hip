@@ -45,6 +45,11 @@ extern "C" int s3dg_create(s3dg_context **out, int device) {
         const long v = std::strtol(env, nullptr, 10);
         if (v >= 0 && v < kVariantCount) ctx->variant = (uint32_t)v;
     }
+    // S3DG_LEAF_SIZE (0..8) overrides the leaf size of the traversal trees
+    if (const char *env = std::getenv("S3DG_LEAF_SIZE")) {
+        const long v = std::strtol(env, nullptr, 10);
+        if (v >= 0 && v <= 8) ctx->leaf_size = (uint32_t)v;
+    }
     // S3DG_TRI_BATCH (1..65) overrides the pending-triangle lane count that keeps a step testing
     if (const char *env = std::getenv("S3DG_TRI_BATCH")) {
         const long v = std::strtol(env, nullptr, 10);

@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _settings():
-    return [abi.Tuning(v, r, b, 0, c) for v, r, b, c in ((0, 32, 16, 256), (1, 32, 16, 64), (2, 32, 16, 4096),
+    return [abi.Tuning(v, r, b, 0, c, 4) for v, r, b, c in ((0, 32, 16, 256), (1, 32, 16, 64), (2, 32, 16, 4096),
                                                          (1, 1, 1, 100), (1, 64, 65, 256), (2, 7, 40, 777))]
 
 
