@@ -5,7 +5,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <atomic>
 #include <deque>
+#include <thread>
 #include <utility>
 
 namespace s3dg {
@@ -239,26 +241,50 @@ public:
         for (size_t i = 0; i < n; ++i) t_.order[i] = (int32_t)i;
     }
 
+    // The top of the tree is split on the calling thread down to nodes of at
+    // most kSerialPrims prims; the subtrees below those are then
+    // built on worker threads into their own node lists and appended.  Each
+    // subtree partitions only its own range of `order`, so the tree is the
+    // same for any thread count.
     int32_t build() {
-        const int32_t root = new_node(0, (int32_t)boxes_.size());
-        std::vector<int32_t> todo{root};
-        while (!todo.empty()) {
-            const int32_t ni = todo.back();
-            todo.pop_back();
-            int32_t left_count;
-            if (!split(t_.nodes[ni], left_count)) continue;   // stays a leaf
-            const int32_t b = t_.nodes[ni].begin, c = t_.nodes[ni].count;
-            const int32_t l = new_node(b, left_count), r = new_node(b + left_count, c - left_count);
-            RefNode &n = t_.nodes[ni];
-            n.kind = 2;
-            n.first_child = (int32_t)t_.children.size();
-            n.child_count = 2;
-            t_.children.push_back(l);
-            t_.children.push_back(r);
-            todo.push_back(r);
-            todo.push_back(l);
+        Part top;
+        new_node(top, 0, (int32_t)boxes_.size());
+        std::vector<int32_t> pending;
+        expand(top, 0, boxes_.size() > (size_t)kSerialPrims ? kSerialPrims : 0, pending);
+        std::vector<Part> parts(pending.size());
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<size_t> next{0};
+        auto work = [&]() {
+            for (size_t k; (k = next.fetch_add(1)) < pending.size();) {
+                Part &p = parts[k];
+                p.nodes.push_back(top.nodes[pending[k]]);
+                std::vector<int32_t> none;
+                expand(p, 0, 0, none);
+            }
+        };
+        std::vector<std::thread> pool;
+        if (pending.size() > 1)
+            for (unsigned i = 1; i < hw; ++i) pool.emplace_back(work);
+        work();
+        for (auto &th : pool) th.join();
+        // append every subtree; its local node 0 replaces the pending node
+        t_.nodes = std::move(top.nodes);
+        t_.children = std::move(top.children);
+        for (size_t k = 0; k < pending.size(); ++k) {
+            const Part &p = parts[k];
+            const int32_t off = (int32_t)t_.nodes.size() - 1;   // local i > 0 -> off + i
+            const int32_t coff = (int32_t)t_.children.size();
+            for (size_t i = 1; i < p.nodes.size(); ++i) {
+                RefNode n = p.nodes[i];
+                if (n.child_count) n.first_child += coff;
+                t_.nodes.push_back(n);
+            }
+            for (int32_t c : p.children) t_.children.push_back(c == 0 ? pending[k] : off + c);
+            RefNode root = p.nodes[0];
+            if (root.child_count) root.first_child += coff;
+            t_.nodes[pending[k]] = root;
         }
-        return root;
+        return 0;
     }
 
 private:
@@ -268,7 +294,14 @@ private:
     RefTree &t_;
     std::vector<float> ctr_;
 
-    int32_t new_node(int32_t begin, int32_t count) {
+    static constexpr int32_t kSerialPrims = 1 << 15;
+
+    struct Part {
+        std::vector<RefNode> nodes;
+        std::vector<int32_t> children;
+    };
+
+    int32_t new_node(Part &p, int32_t begin, int32_t count) const {
         RefNode n;
         n.kind = 0;
         n.first_child = -1;
@@ -277,8 +310,35 @@ private:
         n.count = count;
         n.box = empty_box();
         for (int32_t i = 0; i < count; ++i) grow(n.box, boxes_[t_.order[begin + i]]);
-        t_.nodes.push_back(n);
-        return (int32_t)t_.nodes.size() - 1;
+        p.nodes.push_back(n);
+        return (int32_t)p.nodes.size() - 1;
+    }
+
+    // Splits node `from` of p and its descendants depth first.  With
+    // defer > 0, descendants of at most `defer` prims are not split but
+    // listed in `pending`.
+    void expand(Part &p, int32_t from, int32_t defer, std::vector<int32_t> &pending) {
+        std::vector<int32_t> todo{from};
+        while (!todo.empty()) {
+            const int32_t ni = todo.back();
+            todo.pop_back();
+            if (defer > 0 && ni != from && p.nodes[ni].count <= defer) {
+                pending.push_back(ni);
+                continue;
+            }
+            int32_t left_count;
+            if (!split(p.nodes[ni], left_count)) continue;   // stays a leaf
+            const int32_t b = p.nodes[ni].begin, c = p.nodes[ni].count;
+            const int32_t l = new_node(p, b, left_count), r = new_node(p, b + left_count, c - left_count);
+            RefNode &n = p.nodes[ni];
+            n.kind = 2;
+            n.first_child = (int32_t)p.children.size();
+            n.child_count = 2;
+            p.children.push_back(l);
+            p.children.push_back(r);
+            todo.push_back(r);
+            todo.push_back(l);
+        }
     }
 
     // Chooses a split of the node's prims and partitions them (left side

@@ -38,11 +38,10 @@ struct PathBuf {
     float *ht, *hu, *hv;                    // extension hit
     int32_t *hprim, *hshape;
     float *sx, *sy, *sz;                    // shadow ray direction
-    float *swr, *swg, *swb;                 // W at the shading point
-    float *sfr, *sfg, *sfb;                 // fs = base colour / pi
-    float *scos, *sd2;                      // |n.l|, squared light distance
-    int32_t *slight;                        // light index (in the scene set's light array)
+    float *stl;                             // distance to the light along it (< 0: light missed)
+    float *scr, *scg, *scb;                 // W * NEE, added to L when the light is visible
     uint32_t *scene;                        // scene of the path (index into SceneDev::recs)
+    int32_t single_scene;                   // >= 0: every path is in this scene (scene[] unused)
     uint32_t *q[2];                         // extension queues (ping-pong)
     uint32_t *qs;                           // shadow queue
     uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count,
@@ -50,8 +49,8 @@ struct PathBuf {
     unsigned long long *tot;                // [0] extension rays, [1] shadow rays
 };
 
-// 30 four-byte fields + 1 eight-byte field per path and three queues
-constexpr size_t kPathBytes = 30 * 4 + 8 + 3 * 4;
+// 25 four-byte fields + 1 eight-byte field per path and three queues
+constexpr size_t kPathBytes = 25 * 4 + 8 + 3 * 4;
 
 size_t path_bytes(uint32_t n) { return (size_t)n * kPathBytes + 4096; }
 
@@ -67,17 +66,20 @@ PathBuf carve(void *base, uint32_t n) {
     b.tot = (unsigned long long *)take(64);
     b.rng = (uint64_t *)take((size_t)n * 8);
     float **f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.wr, &b.wg, &b.wb, &b.lr, &b.lg, &b.lb,
-                   &b.ht, &b.hu, &b.hv, &b.sx, &b.sy, &b.sz, &b.swr, &b.swg, &b.swb, &b.sfr, &b.sfg, &b.sfb,
-                   &b.scos, &b.sd2};
+                   &b.ht, &b.hu, &b.hv, &b.sx, &b.sy, &b.sz, &b.stl, &b.scr, &b.scg, &b.scb};
     for (float **x : f) *x = (float *)take((size_t)n * 4);
     b.hprim = (int32_t *)take((size_t)n * 4);
     b.hshape = (int32_t *)take((size_t)n * 4);
-    b.slight = (int32_t *)take((size_t)n * 4);
     b.scene = (uint32_t *)take((size_t)n * 4);
     b.q[0] = (uint32_t *)take((size_t)n * 4);
     b.q[1] = (uint32_t *)take((size_t)n * 4);
     b.qs = (uint32_t *)take((size_t)n * 4);
+    b.single_scene = -1;
     return b;
+}
+
+__device__ __forceinline__ uint32_t scene_of(const PathBuf &P, uint32_t p) {
+    return P.single_scene >= 0 ? (uint32_t)P.single_scene : P.scene[p];
 }
 size_t carve_bytes(uint32_t n) { return path_bytes(n) + 40 * 256; }
 
@@ -199,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const 
         const float fy = (0.5f + (float)y) / (float)g.height - 0.5f;
         V3 o, d;
         camera_ray(S.recs[g.scene].cam, fx, fy, rng, o, d);
-        P.scene[p] = g.scene;
+        if (P.single_scene < 0) P.scene[p] = g.scene;
         P.ox[p] = o.x; P.oy[p] = o.y; P.oz[p] = o.z;
         P.dx[p] = d.x; P.dy[p] = d.y; P.dz[p] = d.z;
         P.wr[p] = 1.0f; P.wg[p] = 1.0f; P.wb[p] = 1.0f;
@@ -236,7 +238,7 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
             r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
             r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
             st.sp = 0;
-            tr.start(S.recs[P.scene[p]], r, kRefHitMin, kRefHitMax);
+            tr.start(S.recs[scene_of(P, p)], r, kRefHitMin, kRefHitMax);
             return true;
         },
         [&]() { return tr.step(S, st); },
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
             const int32_t prim = P.hprim[p];
             if (prim == -1) {
                 // miss: L += W * environment (SampleIBL(dir)*10 in the reference)
-                L = L + W * ld3(S.recs[P.scene[p]].env);
+                L = L + W * ld3(S.recs[scene_of(P, p)].env);
             } else {
                 const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
                 const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                 L = L + W * ld3(m.emissive);
                 Pcg rng;
                 rng.state = P.rng[p];
-                const SceneRec &rec = S.recs[P.scene[p]];
+                const SceneRec &rec = S.recs[scene_of(P, p)];
                 if (!m.delta && rec.light_count > 0) {
                     // NextEventEstimation sampling half (s3d_pt.cpp:258-268)
                     const uint32_t li = (uint32_t)rec.light_base + rng.u32() % (uint32_t)rec.light_count;
@@ -315,13 +317,53 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                     V3 ldir = lpos - pos;
                     const float d2 = dot3(ldir, ldir);
                     ldir = safe_unit3(ldir);
-                    const V3 fs = base_color(m) / kPi;
+                    // Where the shadow ray meets the sampled light, by that
+                    // shape's own test, and the NextEventEstimation term
+                    // (s3d_pt.cpp:274-289) it adds if nothing is hit before.
+                    DRay r;
+                    r.ox = pos.x; r.oy = pos.y; r.oz = pos.z;
+                    r.dx = ldir.x; r.dy = ldir.y; r.dz = ldir.z;
+                    float tl = -1.0f, lu = 0.0f, lv = 0.0f;
+                    int32_t ltri = -1;
+                    if (lt.type == LIGHT_SPHERE) {
+                        const GInst &ls = S.insts[lt.tri[0]];
+                        tl = sphere_test(ls.center[0], ls.center[1], ls.center[2], ls.radius, r, kRefHitMin);
+                    } else {
+                        float best = kFMaxF;
+                        for (int k = 0; k < lt.count; ++k) {
+                            float t, u, v;
+                            if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) &&
+                                t < best) {
+                                best = t; lu = u; lv = v; ltri = lt.tri[k];
+                            }
+                        }
+                        if (ltri >= 0) tl = best;
+                    }
+                    V3 c = v3(0.0f, 0.0f, 0.0f);
+                    if (tl >= 0.0f) {
+                        const V3 ph = pos + ldir * tl;
+                        V3 ln;
+                        int32_t lmat;
+                        if (lt.type == LIGHT_SPHERE) {
+                            const GInst &ls = S.insts[lt.tri[0]];
+                            ln = unit3(ph - ld3(ls.center));
+                            lmat = ls.material;
+                        } else {
+                            ln = tri_normal(S.shade[ltri], lu, lv);
+                            lmat = S.shade[ltri].material;
+                        }
+                        const V3 fs = base_color(m) / kPi;
+                        const float cos_shadow = fabsf(dot3(normal, ldir));
+                        const float cos_light = fabsf(dot3(ln, -ldir));
+                        const float G = (cos_shadow * cos_light) / d2;
+                        const float brdf_pdf = fabsf(ldir.z / kPi) * cos_light / d2;
+                        const float mis = lt.pdf / (brdf_pdf + lt.pdf);
+                        const V3 nee = ld3(S.mats[lmat].emissive) * fs * mis * (G / lt.pdf);
+                        c = W * nee;
+                    }
                     P.sx[p] = ldir.x; P.sy[p] = ldir.y; P.sz[p] = ldir.z;
-                    P.swr[p] = W.x; P.swg[p] = W.y; P.swb[p] = W.z;
-                    P.sfr[p] = fs.x; P.sfg[p] = fs.y; P.sfb[p] = fs.z;
-                    P.scos[p] = fabsf(dot3(normal, ldir));
-                    P.sd2[p] = d2;
-                    P.slight[p] = (int32_t)li;
+                    P.stl[p] = tl;
+                    P.scr[p] = c.x; P.scg[p] = c.y; P.scb[p] = c.z;
                     shadow = true;
                 }
                 V3 out;
@@ -359,69 +401,27 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, Fetch
     Tracer<true, COUNT> tr;
     tr.reset_counts();
     uint32_t p = 0;
-    float tl = 0.0f, lu = 0.0f, lv = 0.0f;
-    int32_t ltri = -1;
     refill_loop(
         &P.cnt[5], n, fetch,
         [&](uint32_t i) {
             p = P.qs[i];
-            const GLight &lt = S.lights[P.slight[p]];
+            // the light was missed (no NEE term): nothing to trace
+            const float tl = P.stl[p];
+            if (!(tl >= 0.0f)) return false;
             DRay r;
             r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
             r.dx = P.sx[p]; r.dy = P.sy[p]; r.dz = P.sz[p];
-            // Distance to the sampled light along the shadow ray, with the
-            // reference's own test of that shape.
-            tl = -1.0f;
-            ltri = -1;
-            if (lt.type == LIGHT_SPHERE) {
-                const GInst &ls = S.insts[lt.tri[0]];
-                tl = sphere_test(ls.center[0], ls.center[1], ls.center[2], ls.radius, r, kRefHitMin);
-            } else {
-                float best = kFMaxF;
-                for (int k = 0; k < lt.count; ++k) {
-                    float t, u, v;
-                    if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) && t < best) {
-                        best = t; lu = u; lv = v; ltri = lt.tri[k];
-                    }
-                }
-                if (ltri >= 0) tl = best;
-            }
-            if (!(tl >= 0.0f)) return false;
             // "closest hit is the light" == no other hit strictly before it
             st.sp = 0;
-            tr.start(S.recs[P.scene[p]], r, kRefHitMin, nextafterf(tl, 0.0f));
+            tr.start(S.recs[scene_of(P, p)], r, kRefHitMin, nextafterf(tl, 0.0f));
             return true;
         },
         [&]() { return tr.step(S, st); },
         [&]() {
             if (tr.best.prim != -1) return;
-            // NextEventEstimation contribution (s3d_pt.cpp:274-289)
-            const GLight &lt = S.lights[P.slight[p]];
-            const V3 o = v3(tr.wray.ox, tr.wray.oy, tr.wray.oz), ld = v3(tr.wray.dx, tr.wray.dy, tr.wray.dz);
-            const V3 ph = o + ld * tl;
-            V3 ln;
-            int32_t lmat;
-            if (lt.type == LIGHT_SPHERE) {
-                const GInst &ls = S.insts[lt.tri[0]];
-                ln = unit3(ph - ld3(ls.center));
-                lmat = ls.material;
-            } else {
-                ln = tri_normal(S.shade[ltri], lu, lv);
-                lmat = S.shade[ltri].material;
-            }
-            const float cos_shadow = P.scos[p];
-            const float cos_light = fabsf(dot3(ln, -ld));
-            const float d2 = P.sd2[p];
-            const V3 fs = v3(P.sfr[p], P.sfg[p], P.sfb[p]);
-            const float G = (cos_shadow * cos_light) / d2;
-            const float brdf_pdf = fabsf(ld.z / kPi) * cos_light / d2;
-            const float mis = lt.pdf / (brdf_pdf + lt.pdf);
-            const V3 nee = ld3(S.mats[lmat].emissive) * fs * mis * (G / lt.pdf);
-            const V3 W = v3(P.swr[p], P.swg[p], P.swb[p]);
-            const V3 c = W * nee;
-            P.lr[p] = P.lr[p] + c.x;
-            P.lg[p] = P.lg[p] + c.y;
-            P.lb[p] = P.lb[p] + c.z;
+            P.lr[p] = P.lr[p] + P.scr[p];
+            P.lg[p] = P.lg[p] + P.scg[p];
+            P.lb[p] = P.lb[p] + P.scb[p];
         });
     tr.flush(P.tot + 4);
 }
@@ -605,6 +605,10 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     int rc = ensure_buffer(ctx, &ws.d_render, &ws.render_bytes, carve_bytes(slots));
     if (rc) return rc;
     PathBuf P = carve(ws.d_render, slots);
+    // paths skip the per-path scene index when every job renders one scene
+    P.single_scene = (int32_t)jobs[0].scene;
+    for (size_t j = 1; j < njobs; ++j)
+        if (jobs[j].scene != jobs[0].scene) P.single_scene = -1;
     // device tables: the slot the kernels read (descriptor + segments), then
     // every batch's descriptor and segments, copied into the slot in turn
     std::vector<BatchDesc> descs(batches.size());
@@ -684,7 +688,8 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     // read are rewritten before each replay).
     const bool use_graph = !timed && !count;
     if (use_graph) {
-        const uint64_t key[8] = {(uint64_t)(uintptr_t)ws.d_render, slots, (uint64_t)max_bounce,
+        const uint64_t key[8] = {(uint64_t)(uintptr_t)ws.d_render,
+                                 slots | ((uint64_t)(uint32_t)(P.single_scene + 1) << 32), (uint64_t)max_bounce,
                                  ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
                                  (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
                                  ((uint64_t)fetch.refill << 48) | ((uint64_t)fetch.chunk << 16) | Sx.tri_batch};
