@@ -72,15 +72,23 @@ struct GMat {
 };
 static_assert(sizeof(GMat) == 64, "GMat is 64 bytes");
 
+// Per top-level shape: its shape record and the material its hits shade with
+// in place of the triangles' own (-1: none; a sphere's own material).
+struct GShapeRef {
+    int32_t inst;
+    int32_t material;
+};
+static_assert(sizeof(GShapeRef) == 8, "GShapeRef is 8 bytes");
+
 enum { LIGHT_SPHERE = 0, LIGHT_QUAD = 1 };
 
-// Light record (64 bytes).
+// Light record (64 bytes), everything its sampling and its NEE term read.
 struct GLight {
-    float   corner[3];
+    float   corner[3];      // QUAD: corner; SPHERE: centre
     int32_t type;           // LIGHT_*
-    float   e1[3];
+    float   e1[3];          // QUAD: first edge; SPHERE: e1[0] = radius
     int32_t shape;          // top-level shape index
-    float   e2[3];
+    float   e2[3];          // QUAD: second edge; SPHERE: emission of the sphere's material
     float   pdf;            // QUAD: 1/area; SPHERE: 1/(4 pi r^2)
     int32_t tri[3];         // QUAD: global depth-first indices of its triangles;
                             // SPHERE: tri[0] = instance record of the sphere

@@ -56,7 +56,7 @@ struct SceneDev {
     const GInst *insts;
     const GMat *mats;
     const GLight *lights;
-    const int32_t *inst_of_shape;   // top-level shape index -> instance record
+    const GShapeRef *shape_ref;     // top-level shape index -> record + material override
     const SceneRec *recs;
     const uint32_t *tri_rank;   // depth-first rank of each stored triangle (tie order)
     int32_t stack_need;     // deepest stack the trees can require
@@ -96,7 +96,7 @@ struct s3dg_scene {
     s3dg_scene_stats stats{};
     // device allocations
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
-    void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr, *d_rank = nullptr;
+    void *d_mats = nullptr, *d_lights = nullptr, *d_shape_ref = nullptr, *d_rank = nullptr;
     struct {
         uint32_t nodes, tris, insts, mats, lights, shapes;   // element counts of the device arrays
     } counts{};

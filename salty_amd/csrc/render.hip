@@ -136,38 +136,60 @@ __device__ __forceinline__ const Seg &find_seg(const Seg *__restrict__ segs, uin
 constexpr uint32_t kAutoPathsInFlight = 1u << 27;
 constexpr uint32_t kMaxPathsInFlight = 1u << 28;
 
-// Workgroup-aggregated appends to two queues: one global atomic per queue per
-// workgroup (instead of one per wave), so that the few counters every
-// workgroup of the grid appends to are not a serialisation point.  Every
-// thread of the workgroup must call it (it synchronises the workgroup).
-__device__ __forceinline__ void block_append2(uint32_t *c0, bool w0, uint32_t &s0, uint32_t *c1, bool w1,
-                                              uint32_t &s1, uint32_t *sh /* 4 * (kBlock / 64) + 2 words of LDS */) {
-    constexpr int kWaves = kBlock / 64;
-    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t m0 = __ballot(w0), m1 = __ballot(w1);
-    const uint64_t below = (1ull << lane) - 1ull;
-    if (lane == 0) {
-        sh[2 * wave] = (uint32_t)__popcll(m0);
-        sh[2 * wave + 1] = (uint32_t)__popcll(m1);
+// Appends of a workgroup to two queues, staged in LDS over several loop
+// iterations: a wave reserves its entries in the stage with one LDS atomic,
+// and the workgroup reserves the global space of a full stage with one global
+// atomic per queue and writes it out contiguously.  The few queue counters
+// every workgroup of the grid appends to are thus hit once per ~kStage
+// entries (the hardware serialises atomics on one address), not once per
+// wave or per iteration.  Every thread of the workgroup must make the same
+// calls (they synchronise the workgroup).
+struct StagedAppend2 {
+    static constexpr uint32_t kStage = 2048;   // entries per queue
+    uint32_t buf[2][kStage];
+    uint32_t count[2];
+    uint32_t base[2];
+
+    __device__ __forceinline__ void init() {
+        if (threadIdx.x < 2) count[threadIdx.x] = 0;
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t t0 = 0, t1 = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            const uint32_t a = sh[2 * w], b = sh[2 * w + 1];
-            sh[2 * kWaves + 2 * w] = t0;
-            sh[2 * kWaves + 2 * w + 1] = t1;
-            t0 += a;
-            t1 += b;
+    // Empties the stage when it could not take another iteration (all = true:
+    // whatever it holds).
+    __device__ __forceinline__ void flush_if_full(uint32_t *c0, uint32_t *q0, uint32_t *c1, uint32_t *q1, bool all) {
+        __syncthreads();
+        const uint32_t n0 = count[0], n1 = count[1];
+        if (all ? (n0 | n1) != 0u : (n0 > kStage - kBlock || n1 > kStage - kBlock)) {
+            if (threadIdx.x == 0) {
+                base[0] = n0 ? atomicAdd(c0, n0) : 0u;
+                base[1] = n1 ? atomicAdd(c1, n1) : 0u;
+            }
+            __syncthreads();
+            const uint32_t b0 = base[0], b1 = base[1];
+            for (uint32_t k = threadIdx.x; k < n0; k += kBlock) q0[b0 + k] = buf[0][k];
+            for (uint32_t k = threadIdx.x; k < n1; k += kBlock) q1[b1 + k] = buf[1][k];
+            __syncthreads();
+            if (threadIdx.x < 2) count[threadIdx.x] = 0;
         }
-        sh[4 * kWaves] = t0 ? atomicAdd(c0, t0) : 0u;
-        sh[4 * kWaves + 1] = t1 ? atomicAdd(c1, t1) : 0u;
+        __syncthreads();
     }
-    __syncthreads();
-    s0 = sh[4 * kWaves] + sh[2 * kWaves + 2 * wave] + (uint32_t)__popcll(m0 & below);
-    s1 = sh[4 * kWaves + 1] + sh[2 * kWaves + 2 * wave + 1] + (uint32_t)__popcll(m1 & below);
-    __syncthreads();   // the words are rewritten by the next call
-}
+    // Stages v in queue 0 if w0 and in queue 1 if w1 (at most kBlock entries
+    // per queue between two flush_if_full calls).
+    __device__ __forceinline__ void add(bool w0, bool w1, uint32_t v) {
+        const uint32_t lane = __lane_id();
+        const uint64_t m0 = __ballot(w0), m1 = __ballot(w1);
+        const uint64_t below = (1ull << lane) - 1ull;
+        uint32_t s0 = 0, s1 = 0;
+        if (lane == 0) {
+            if (m0) s0 = atomicAdd(&count[0], (uint32_t)__popcll(m0));
+            if (m1) s1 = atomicAdd(&count[1], (uint32_t)__popcll(m1));
+        }
+        s0 = __shfl(s0, 0) + (uint32_t)__popcll(m0 & below);
+        s1 = __shfl(s1, 0) + (uint32_t)__popcll(m1 & below);
+        if (w0) buf[0][s0] = v;
+        if (w1) buf[1][s1] = v;
+    }
+};
 
 // Persistent-kernel work fetch: lane 0 of the wave takes the next 64 queue
 // entries; the wave leaves its loop once the cursor has passed the queue end.
@@ -258,7 +280,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t *q = P.q[cur];
-    __shared__ uint32_t sh_append[4 * (kBlock / 64) + 2];
+    __shared__ StagedAppend2 stage;
+    stage.init();
     if (gtid == 0) P.cnt[4] = 0;   // the next extend launch fetches from zero
     const uint32_t trips = (n + stride - 1) / stride;
     for (uint32_t it = 0; it < trips; ++it) {
@@ -280,17 +303,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                 const float t = P.ht[p];
                 const V3 pos = o + d * t;
                 V3 normal;
-                int32_t mat;
+                const GShapeRef sr = S.shape_ref[P.hshape[p]];
+                int32_t mat = sr.material;
                 if (prim == -2) {
-                    const GInst &in = S.insts[S.inst_of_shape[P.hshape[p]]];
-                    normal = unit3(pos - ld3(in.center));
-                    mat = in.material;
+                    normal = unit3(pos - ld3(S.insts[sr.inst].center));
                 } else {
                     const GTriShade &sh = S.shade[prim];
                     normal = tri_normal(sh, P.hu[p], P.hv[p]);
-                    mat = sh.material;
-                    const int32_t ovr = S.insts[S.inst_of_shape[P.hshape[p]]].material;
-                    if (ovr >= 0) mat = ovr;
+                    if (mat < 0) mat = sh.material;
                 }
                 const GMat &m = S.mats[mat];
                 L = L + W * ld3(m.emissive);
@@ -303,12 +323,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                     const GLight &lt = S.lights[li];
                     V3 lpos;
                     if (lt.type == LIGHT_SPHERE) {
-                        const GInst &ls = S.insts[lt.tri[0]];
                         // Sphere::Sample (s3d_sphere.cpp:106-115)
                         const float r1 = k2Pi * rng.f32();
                         const float r2 = 1.0f - 2.0f * rng.f32();
                         const float r3 = sqrtf(1.0f - r2 * r2);
-                        lpos = ld3(ls.center) + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (ls.radius + 1e-1f);
+                        lpos = ld3(lt.corner) + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (lt.e1[0] + 1e-1f);
                     } else {
                         const float u = rng.f32();
                         const float v = rng.f32();
@@ -326,8 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                     float tl = -1.0f, lu = 0.0f, lv = 0.0f;
                     int32_t ltri = -1;
                     if (lt.type == LIGHT_SPHERE) {
-                        const GInst &ls = S.insts[lt.tri[0]];
-                        tl = sphere_test(ls.center[0], ls.center[1], ls.center[2], ls.radius, r, kRefHitMin);
+                        tl = sphere_test(lt.corner[0], lt.corner[1], lt.corner[2], lt.e1[0], r, kRefHitMin);
                     } else {
                         float best = kFMaxF;
                         for (int k = 0; k < lt.count; ++k) {
@@ -342,15 +360,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                     V3 c = v3(0.0f, 0.0f, 0.0f);
                     if (tl >= 0.0f) {
                         const V3 ph = pos + ldir * tl;
-                        V3 ln;
-                        int32_t lmat;
+                        V3 ln, le;
                         if (lt.type == LIGHT_SPHERE) {
-                            const GInst &ls = S.insts[lt.tri[0]];
-                            ln = unit3(ph - ld3(ls.center));
-                            lmat = ls.material;
+                            ln = unit3(ph - ld3(lt.corner));
+                            le = ld3(lt.e2);
                         } else {
                             ln = tri_normal(S.shade[ltri], lu, lv);
-                            lmat = S.shade[ltri].material;
+                            le = ld3(S.mats[S.shade[ltri].material].emissive);
                         }
                         const V3 fs = base_color(m) / kPi;
                         const float cos_shadow = fabsf(dot3(normal, ldir));
@@ -358,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                         const float G = (cos_shadow * cos_light) / d2;
                         const float brdf_pdf = fabsf(ldir.z / kPi) * cos_light / d2;
                         const float mis = lt.pdf / (brdf_pdf + lt.pdf);
-                        const V3 nee = ld3(S.mats[lmat].emissive) * fs * mis * (G / lt.pdf);
+                        const V3 nee = le * fs * mis * (G / lt.pdf);
                         c = W * nee;
                     }
                     P.sx[p] = ldir.x; P.sy[p] = ldir.y; P.sz[p] = ldir.z;
@@ -381,11 +397,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
             }
             P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
         }
-        uint32_t si = 0, ni = 0;
-        block_append2(&P.cnt[2], shadow, si, &P.cnt[cur ^ 1], cont, ni, sh_append);
-        if (shadow) P.qs[si] = p;
-        if (cont) P.q[cur ^ 1][ni] = p;
+        stage.flush_if_full(&P.cnt[2], P.qs, &P.cnt[cur ^ 1], P.q[cur ^ 1], false);
+        stage.add(shadow, cont, p);
     }
+    stage.flush_if_full(&P.cnt[2], P.qs, &P.cnt[cur ^ 1], P.q[cur ^ 1], true);
 }
 
 template <bool COUNT, int LDS, int WAVES>
@@ -477,15 +492,13 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
         if (h.prim != -1) {
             const V3 pos = o + d * h.t;
             int32_t mat;
+            const GShapeRef sr = S.shape_ref[h.shape];
+            mat = sr.material;
             if (h.prim == -2) {
-                const GInst &in = S.insts[S.inst_of_shape[h.shape]];
-                nn = unit3(pos - ld3(in.center));
-                mat = in.material;
+                nn = unit3(pos - ld3(S.insts[sr.inst].center));
             } else {
                 nn = tri_normal(S.shade[h.prim], h.u, h.v);
-                mat = S.shade[h.prim].material;
-                const int32_t ovr = S.insts[S.inst_of_shape[h.shape]].material;
-                if (ovr >= 0) mat = ovr;
+                if (mat < 0) mat = S.shade[h.prim].material;
             }
             a = base_color(S.mats[mat]);
             t = h.t;

@@ -433,6 +433,12 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
             }
             g.type = LIGHT_SPHERE;
             g.tri[0] = inst_pos[l.shape];
+            const GInst &sp = inst[l.shape];
+            for (int a = 0; a < 3; ++a) {
+                g.corner[a] = sp.center[a];
+                g.e2[a] = mats[sp.material].emissive[a];
+            }
+            g.e1[0] = sp.radius;
             // Sphere::Sample pdf (s3d_sphere.cpp:113): 1 / (4 pi r^2), left to right
             g.pdf = 1.0f / (4.0f * 3.1415926535897932384626433832795f * s.radius * s.radius);
         } else if (l.type == S3DG_LIGHT_QUAD) {
@@ -492,6 +498,11 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     sc->counts.nodes = (uint32_t)comp.nodes.size();
     sc->counts.tris = (uint32_t)tris.size();
     sc->counts.insts = (uint32_t)inst_dev.size();
+    std::vector<GShapeRef> shape_ref(d->shape_count);
+    for (uint32_t k = 0; k < d->shape_count; ++k) {
+        shape_ref[k].inst = inst_pos[k];
+        shape_ref[k].material = inst[k].material;
+    }
     sc->counts.mats = (uint32_t)mats.size();
     sc->counts.lights = (uint32_t)lights.size();
     sc->counts.shapes = (uint32_t)inst_pos.size();
@@ -500,7 +511,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     if ((rc = upload(ctx, &sc->d_nodes, comp.nodes, total)) || (rc = upload(ctx, &sc->d_tris, tris, total)) ||
         (rc = upload(ctx, &sc->d_shade, shade, total)) || (rc = upload(ctx, &sc->d_insts, inst_dev, total)) ||
         (rc = upload(ctx, &sc->d_mats, mats, total)) || (rc = upload(ctx, &sc->d_lights, lights, total)) ||
-        (rc = upload(ctx, &sc->d_inst_of_shape, inst_pos, total)) ||
+        (rc = upload(ctx, &sc->d_shape_ref, shape_ref, total)) ||
         (rc = upload(ctx, &sc->d_rec, std::vector<SceneRec>{sc->rec}, total)) ||
         (rc = upload(ctx, &sc->d_rank, rank, total))) {
         s3dg_scene_destroy(sc);
@@ -516,7 +527,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     dv.insts = (const GInst *)sc->d_insts;
     dv.mats = (const GMat *)sc->d_mats;
     dv.lights = (const GLight *)sc->d_lights;
-    dv.inst_of_shape = (const int32_t *)sc->d_inst_of_shape;
+    dv.shape_ref = (const GShapeRef *)sc->d_shape_ref;
     dv.recs = (const SceneRec *)sc->d_rec;
     dv.tri_rank = (const uint32_t *)sc->d_rank;
 
@@ -533,7 +544,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
 extern "C" int s3dg_scene_destroy(s3dg_scene *sc) {
     if (!sc) return S3DG_ERR_INVALID;
     void *ptrs[] = {sc->d_nodes, sc->d_tris, sc->d_shade, sc->d_insts, sc->d_mats, sc->d_lights,
-                    sc->d_inst_of_shape, sc->d_rec, sc->d_rank};
+                    sc->d_shape_ref, sc->d_rec, sc->d_rank};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     workspace_free(sc->ws);

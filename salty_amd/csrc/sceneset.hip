@@ -27,7 +27,7 @@ struct s3dg_scene_set {
     SceneDev dev{};
     std::vector<SceneRec> recs;
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
-    void *d_mats = nullptr, *d_lights = nullptr, *d_inst_of_shape = nullptr, *d_recs = nullptr;
+    void *d_mats = nullptr, *d_lights = nullptr, *d_shape_ref = nullptr, *d_recs = nullptr;
     void *d_rank = nullptr;
     Workspace ws;
 };
@@ -76,15 +76,18 @@ __global__ void k_add_u32(uint32_t *__restrict__ a, uint32_t n, uint32_t off) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] += off;
 }
 
-__global__ void k_add_i32(int32_t *__restrict__ a, uint32_t n, int32_t off) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] += off;
+__global__ void k_rebase_shape_refs(GShapeRef *__restrict__ refs, uint32_t n, int32_t inst_off, int32_t mat_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        refs[i].inst += inst_off;
+        if (refs[i].material >= 0) refs[i].material += mat_off;
+    }
 }
 
 unsigned grid_for(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 4096u)); }
 
 void free_set(s3dg_scene_set *set) {
     void *ptrs[] = {set->d_nodes, set->d_tris,   set->d_shade,          set->d_insts, set->d_mats,
-                    set->d_lights, set->d_inst_of_shape, set->d_recs, set->d_rank};
+                    set->d_lights, set->d_shape_ref, set->d_recs, set->d_rank};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     workspace_free(set->ws);
@@ -129,7 +132,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         (e = alloc(&set->d_insts, (size_t)tot.insts * sizeof(GInst))) != hipSuccess ||
         (e = alloc(&set->d_mats, (size_t)tot.mats * sizeof(GMat))) != hipSuccess ||
         (e = alloc(&set->d_lights, (size_t)tot.lights * sizeof(GLight))) != hipSuccess ||
-        (e = alloc(&set->d_inst_of_shape, (size_t)tot.shapes * sizeof(int32_t))) != hipSuccess ||
+        (e = alloc(&set->d_shape_ref, (size_t)tot.shapes * sizeof(GShapeRef))) != hipSuccess ||
         (e = alloc(&set->d_recs, (size_t)n * sizeof(SceneRec))) != hipSuccess ||
         (e = alloc(&set->d_rank, (size_t)tot.tris * sizeof(uint32_t))) != hipSuccess) {
         free_set(set);
@@ -151,7 +154,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         GInst *insts = (GInst *)set->d_insts + o.insts;
         GLight *lights = (GLight *)set->d_lights + o.lights;
         uint32_t *rank = (uint32_t *)set->d_rank + o.tris;
-        int32_t *ios = (int32_t *)set->d_inst_of_shape + o.shapes;
+        GShapeRef *refs = (GShapeRef *)set->d_shape_ref + o.shapes;
         struct {
             void *dst;
             const void *src;
@@ -162,7 +165,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
                       {insts, sc->d_insts, (size_t)c.insts * sizeof(GInst)},
                       {(GMat *)set->d_mats + o.mats, sc->d_mats, (size_t)c.mats * sizeof(GMat)},
                       {lights, sc->d_lights, (size_t)c.lights * sizeof(GLight)},
-                      {ios, sc->d_inst_of_shape, (size_t)c.shapes * sizeof(int32_t)},
+                      {refs, sc->d_shape_ref, (size_t)c.shapes * sizeof(GShapeRef)},
                       {rank, sc->d_rank, (size_t)c.tris * sizeof(uint32_t)}};
         for (auto &cp : copies)
             if (cp.bytes) {
@@ -187,8 +190,8 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
             hipLaunchKernelGGL(k_rebase_lights, dim3(grid_for(c.lights)), dim3(256), 0, s, lights, c.lights,
                                (int32_t)o.tris, (int32_t)o.insts, (int32_t)o.shapes);
         if (c.shapes)
-            hipLaunchKernelGGL(k_add_i32, dim3(grid_for(c.shapes)), dim3(256), 0, s, ios, c.shapes,
-                               (int32_t)o.insts);
+            hipLaunchKernelGGL(k_rebase_shape_refs, dim3(grid_for(c.shapes)), dim3(256), 0, s, refs, c.shapes,
+                               (int32_t)o.insts, (int32_t)o.mats);
         SceneRec r = sc->rec;
         r.root += o.nodes;
         if (r.single_shape >= 0) r.single_shape += (int32_t)o.shapes;
@@ -209,7 +212,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
     dv.insts = (const GInst *)set->d_insts;
     dv.mats = (const GMat *)set->d_mats;
     dv.lights = (const GLight *)set->d_lights;
-    dv.inst_of_shape = (const int32_t *)set->d_inst_of_shape;
+    dv.shape_ref = (const GShapeRef *)set->d_shape_ref;
     dv.recs = (const SceneRec *)set->d_recs;
     dv.tri_rank = (const uint32_t *)set->d_rank;
     dv.stack_need = stack_need;
