@@ -19,7 +19,7 @@ struct s3dg_context {
     void *small = nullptr;
     size_t small_bytes = 0;
     // traversal kernels hand new rays to a wave once this many of its lanes are idle
-    uint32_t refill = 48;
+    uint32_t refill = 16;
     // a traversal step keeps testing triangles while this many lanes have some pending
     uint32_t tri_batch = 16;
     // queue entries a traversal wave takes from the shared cursor at a time

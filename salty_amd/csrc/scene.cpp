@@ -492,7 +492,9 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     sc->rec.light_base = 0;
     // per BLAS level: one child group and one leaf group; per TLAS level: one
     // child group and up to eight shapes; one restore entry
-    dv.stack_need = 2 * blas_depth + 9 * tlas_depth + 4;
+    // at most two entries per node on the path (a group of inner children and
+    // a group of leaves or shapes) plus the return from an instance
+    dv.stack_need = 2 * blas_depth + 2 * tlas_depth + 4;
     dv.tri_batch = ctx->tri_batch;
 
     sc->counts.nodes = (uint32_t)comp.nodes.size();

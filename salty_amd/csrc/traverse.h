@@ -154,8 +154,10 @@ __device__ __forceinline__ RayPre precompute(const DRay &r) {
 }
 
 // Stack entries: a group of a node's hit inner children still to visit, a
-// top-level shape to enter, the return to the world ray after an instance,
-// and a group of a node's hit leaves whose triangles are still to test.
+// group of a top-level node's hit shapes still to enter, the return to the
+// world ray after an instance, and a group of a node's hit leaves whose
+// triangles are still to test.  A top-level node thus takes at most two
+// entries, like any other node.
 enum { ENT_GROUP = 0u, ENT_INST = 1u, ENT_RESTORE = 2u, ENT_LEAVES = 3u };
 
 // Traversal stack: the first LDS entries of a lane live in LDS (stride
@@ -331,8 +333,14 @@ struct Tracer {
             leaf_mask = e.y & 0xffu;
             next_leaf();
         } else if (kind == ENT_INST) {
-            st.sp--;
-            const GInst &in = S.insts[e.x];
+            uint32_t rem = e.y & 0xffu;
+            const uint32_t s = (uint32_t)__builtin_ctz(rem) ^ pre.oct;
+            const uint32_t lslots = (e.y >> 8) & 0xffu;
+            const uint32_t rec = e.x + (uint32_t)__builtin_popcount(lslots & ((1u << s) - 1u));
+            rem &= rem - 1u;
+            if (rem) st.set_top(make_uint2(e.x, (e.y & ~0xffu) | rem));
+            else st.sp--;
+            const GInst &in = S.insts[rec];
             if (in.type == INST_SPHERE) {
                 const float d = sphere_test(in.center[0], in.center[1], in.center[2], in.radius, ray, tmin);
                 if (d >= 0.0f && d <= tmax) {
@@ -434,15 +442,11 @@ struct Tracer {
         const uint32_t leaves = hits & ~imask;
         if (leaves) {
             if (top) {
-                // top-level leaves are single shapes: queued, visited in turn,
-                // far ones pushed first so that the nearest is popped first
-                uint32_t lk = to_korder(leaves, pre.oct);
-                while (lk) {
-                    const uint32_t k = 31u - __builtin_clz(lk);
-                    lk &= ~(1u << k);
-                    const uint32_t s = k ^ pre.oct;
-                    st.push(make_uint2(slot_first(lbase, cnt_lo, cnt_hi, s), ENT_INST << 30));
-                }
+                // the node's hit shape leaves as one entry, entered nearest
+                // first; each top-level leaf slot holds one shape record, so
+                // a slot's record is lbase plus the leaf slots below it
+                const uint32_t lslots = valid & ~imask;
+                st.push(make_uint2(lbase, (ENT_INST << 30) | (lslots << 8) | to_korder(leaves, pre.oct)));
             } else if (tri_cur == tri_end && leaf_mask == 0) {
                 leaf_base = lbase;
                 leaf_lo = cnt_lo;
