@@ -130,7 +130,7 @@ __device__ __forceinline__ const Seg &find_seg(const Seg *__restrict__ segs, uin
 }
 
 // Paths traced together when the caller leaves passes_in_flight at 0 (about
-// 19 GB of path state: all 64 passes of a 1080p frame at once, which keeps the
+// 16 GB of path state: all 64 passes of a 1080p frame at once, which keeps the
 // deep-bounce launches full), and the most ever held at once.  The automatic
 // choice also stays below a quarter of the free device memory.
 constexpr uint32_t kAutoPathsInFlight = 1u << 27;
