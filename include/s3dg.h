@@ -54,7 +54,10 @@ enum {
 };
 
 /* Union of the MaterialFactory::Create* parameters
- * (src/include/s3d_materialfactory.h:22-49). */
+ * (src/include/s3d_materialfactory.h:22-49).  A texture wraps the material in
+ * a TexturedMaterial (src/src/s3d_texturedmaterial.cpp:63-87): Shade and
+ * GetBaseColor are multiplied by Texture::SampleColor(texcoord), the emission
+ * is not. */
 typedef struct s3dg_material {
     int32_t type;           /* S3DG_MAT_*                              */
     float   diffuse[3];     /* Lambert / Plastic diffuse colour         */
@@ -62,7 +65,17 @@ typedef struct s3dg_material {
     float   emissive[3];    /* IMaterial::GetEmissive                   */
     float   ior;            /* Glass index of refraction                */
     float   power;          /* Phong / Plastic exponent                 */
+    int32_t texture;        /* 0: none; k > 0: the scene's texture k-1  */
 } s3dg_material;
+
+/* s3d::Texture (src/include/s3d_texture.h): width x height RGB texels, texel
+ * (x, y) at rgb[3 * (width * y + x)] (Texture::GetColor); rows as the file
+ * stores them (a BMP's first row is the image's bottom row). */
+typedef struct s3dg_texture_desc {
+    int32_t      width;
+    int32_t      height;
+    const float *rgb;
+} s3dg_texture_desc;
 
 /* Acceleration structure kind built for a mesh: BVH2::Create, BVH4::Create
  * or BVH8::Create (src/include/s3d_bvh{2,4,8}.h).  Mesh::Init always uses
@@ -143,7 +156,13 @@ typedef struct s3dg_scene_desc {
     uint32_t               light_count;
     const s3dg_light_desc *lights;
     s3dg_camera_desc       camera;
-    float                  environment[3];  /* radiance on a miss; replaces IBL*10 */
+    float                  environment[3];  /* radiance on a miss when no IBL texture */
+    uint32_t               texture_count;
+    const s3dg_texture_desc *textures;
+    int32_t                environment_texture;  /* 0: constant environment; k > 0:  */
+                           /* textures[k-1] as the IBL, Scene::SampleIBL: lat-long     */
+                           /* Texture::SampleColor(dir) * 10 (src/include/s3d_scene.h:80-82, */
+                           /* src/src/s3d_texture.cpp:957-972)                          */
 } s3dg_scene_desc;
 
 /* ---------------------------------------------------------------------------
@@ -264,6 +283,41 @@ int s3dg_scene_mesh_order(const s3dg_scene *scene, uint32_t mesh, int32_t *order
  * and the number of tree nodes, without a device.  kind is S3DG_BVH2/4/8. */
 int s3dg_build_order(uint32_t vertex_count, const s3dg_vertex *vertices, int32_t kind, int32_t *order,
                      uint32_t *node_count);
+
+/* ---------------------------------------------------------------------------
+ * Host-only loaders of the reference's file formats.
+ * ------------------------------------------------------------------------- */
+
+/* Images as RGB floats (free with s3dg_image_free).  BMP: LoadFromBMP
+ * (src/src/s3d_texture.cpp:1029-1077): 24-bit only, texels read right after the
+ * 54 header bytes with no row padding, rows kept in file order, channel =
+ * byte / 255; the reference leaves the last texel unwritten, here it is 0.
+ * HDR: LoadFromHDR (src/src/s3d_texture.cpp:321-350, RGBE with old and new
+ * run-length lines, exposure and gamma ignored as there). */
+int s3dg_image_load_bmp(const char *path, int32_t *width, int32_t *height, float **rgb);
+int s3dg_image_load_hdr(const char *path, int32_t *width, int32_t *height, float **rgb);
+void s3dg_image_free(float *rgb);
+
+/* SMD mesh file: Mesh::LoadFromFile (src/src/s3d_mesh.cpp:250-465).  Header
+ * "SMD\0", version 2, data header of six u32 (triangle, material, texture
+ * counts and struct sizes); texture names (256 bytes each, relative to the
+ * file's folder, a '\\' in a name read as '/'); per material an s32 type then its fields; 100-byte
+ * triangles (3 x {position, normal, texcoord} + s32 material).  Materials map
+ * to s3dg_material records, texture = ColorMap (or DiffuseMap) + 1.  Rejected as there:
+ * a bad magic, version or data-header size, no triangles or no materials; in
+ * addition an unknown material type or a short file is an error here. */
+typedef struct s3dg_smd {
+    uint32_t       vertex_count;        /* 3 per triangle                      */
+    s3dg_vertex   *vertices;
+    int32_t       *triangle_material;   /* MaterialId per triangle (-1: none)  */
+    uint32_t       material_count;
+    s3dg_material *materials;
+    uint32_t       texture_count;
+    char         (*texture_paths)[512]; /* folder of the file + "/" + name      */
+} s3dg_smd;
+
+int s3dg_smd_load(const char *path, s3dg_smd **out);
+void s3dg_smd_free(s3dg_smd *smd);
 
 /* ---------------------------------------------------------------------------
  * Traversal: replaces Scene::Intersect / IShape::IsHit(const RaySet&, HitRecord&)

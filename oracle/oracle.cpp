@@ -589,10 +589,12 @@ struct Material {
     V3 diffuse, specular, emissive;
     float ior, power;
     float thr0, thr1;
+    int texture;   // TexturedMaterial wrapper (src/src/s3d_texturedmaterial.cpp): -1 none
 };
 static Material make_material(const s3dg_material &m) {
     Material r;
     r.type = m.type;
+    r.texture = m.texture - 1;
     r.diffuse = load3(m.diffuse);
     r.specular = load3(m.specular);
     r.emissive = load3(m.emissive);
@@ -753,11 +755,52 @@ struct Light {
     V3 corner, e1, e2;
     float pdf;
 };
+// Texture (src/include/s3d_texture.h:106-111 GetColor, src/src/s3d_texture.cpp:925-972).
+struct Texture {
+    int w = 0, h = 0;
+    std::vector<V3> px;
+    V3 get(int x, int y) const { return px[(size_t)w * y + x]; }
+    // SampleColor(const Vector2&): bilinear over the corners of uv * (size - 1),
+    // wrapped by abs(i % size).  The float -> s32 conversion of NaN gives 0 and
+    // the range is held to +-2^30 (undefined in the reference).
+    V3 sample(float u, float v) const {
+        float fx = u * (float)(w - 1);
+        float fy = v * (float)(h - 1);
+        auto to_i = [](float f) -> int {
+            f = floorf(f);
+            if (f != f) return 0;
+            return (int)smin(smax(f, -1073741824.0f), 1073741824.0f);
+        };
+        int x0 = to_i(fx), y0 = to_i(fy);
+        int x1 = x0 + 1, y1 = y0 + 1;
+        float x_w0 = (float)x1 - fx, x_w1 = fx - (float)x0;
+        float y_w0 = (float)y1 - fy, y_w1 = fy - (float)y0;
+        x0 = abs(x0 % w); x1 = abs(x1 % w);
+        y0 = abs(y0 % h); y1 = abs(y1 % h);
+        return x_w0 * (y_w0 * get(x0, y0) + y_w1 * get(x0, y1)) + x_w1 * (y_w0 * get(x1, y0) + y_w1 * get(x1, y1));
+    }
+    // SampleColor(const Vector3&): latitude-longitude map; IsZero is |x| <= FLT_EPSILON
+    // (s3d_math.h:125-126).
+    V3 sample_dir(V3 dir) const {
+        float u = 0.0f;
+        const float theta = acosf(dir.y);
+        float v = theta * 0.31830988618379067153776752674503f;            // F_1DIVPI
+        if (!(fabsf(dir.x) <= FLT_EPSILON) && !(fabsf(dir.z) <= FLT_EPSILON)) {
+            const float phi = atan2f(dir.z, dir.x);
+            u = (dir.z < 0.0f) ? (phi + k2Pi) : phi;
+            u *= 0.15915494309189533576888376337251f;                    // F_1DIV2PI
+        }
+        return sample(u, v);
+    }
+};
+
 struct Scene {
     std::vector<Mesh *> meshes;
     std::vector<Shape> shapes;
     std::vector<Material> materials;
     std::vector<Light> lights;
+    std::vector<Texture> textures;
+    int env_tex = -1;   // IBL texture, -1: constant environment
     V3 env;
     int cam_type;
     V3 cam_pos, cam_dir, CX, CY, CZ;
@@ -820,16 +863,30 @@ static void camera_ray(const Scene &sc, float x, float y, PCG *rng, V3 &org, V3 
 
 // Shading point data of a hit (CalcParam + material)
 static void hit_surface(const Scene &sc, const Ray &ray, const Hit &h, V3 &pos, V3 &normal,
-                        const Material *&mat) {
+                        const Material *&mat, float *uv = nullptr) {
     pos = ray.pos + ray.dir * h.distance;
     const Shape &s = sc.shapes[h.shape];
     if (h.prim == -2) {
         normal = unit(pos - s.center);
         mat = &sc.materials[s.material];
+        if (uv) {
+            // Sphere::CalcParam texcoord (s3d_sphere.cpp:94-104)
+            const float theta = acosf(normal.y);
+            float phi = atan2f(normal.x, normal.z);
+            if (phi < 0.0f) phi += k2Pi;
+            uv[0] = phi * 0.15915494309189533576888376337251f;
+            uv[1] = (kPi - theta) * 0.31830988618379067153776752674503f;
+        }
     } else {
         const Mesh *m = sc.meshes[s.mesh];
         const Tri &t = m->tris[m->prims[h.prim]];
         normal = tri_normal(t, h.b1, h.b2);   // object space for instances (no CalcParam of Instance)
+        if (uv) {
+            // Triangle::CalcParam texcoord (s3d_triangle.cpp:118-120)
+            const float alpha = 1.0f - h.b1 - h.b2;
+            uv[0] = t.uv[0][0] * alpha + t.uv[1][0] * h.b1 + t.uv[2][0] * h.b2;
+            uv[1] = t.uv[0][1] * alpha + t.uv[1][1] * h.b1 + t.uv[2][1] * h.b2;
+        }
         mat = &sc.materials[t.material];
         // instance material override (an extension of the reference's Instance)
         if (s.type == S3DG_SHAPE_INSTANCE && s.material >= 0) mat = &sc.materials[s.material];
@@ -839,7 +896,7 @@ static void hit_surface(const Scene &sc, const Ray &ray, const Hit &h, V3 &pos, 
 // PathTracer::NextEventEstimation (src/src/s3d_pt.cpp:251-292) with Scene::GetLight
 // (s3d_scene.h:84-90).  QUAD lights are an extension (the
 // reference samples spheres only).
-static V3 nee(const Scene &sc, V3 position, V3 normal, const Material &mat, PCG &rng,
+static V3 nee(const Scene &sc, V3 position, V3 normal, const float *texcoord, const Material &mat, PCG &rng,
               uint64_t &shadow_rays) {
     uint32_t li = rng.u32() % (uint32_t)sc.lights.size();
     const Light &L = sc.lights[li];
@@ -882,15 +939,19 @@ static V3 nee(const Scene &sc, V3 position, V3 normal, const Material &mat, PCG 
     hit_surface(sc, sr, sh, p, light_normal, lm);
     float cos_shadow = fabsf(dot(normal, light_dir));
     float cos_light = fabsf(dot(light_normal, -light_dir));
-    V3 fs = base_color(mat) / kPi;
+    // GetBaseColor(texcoord): TexturedMaterial multiplies by the texture
+    V3 base = base_color(mat);
+    if (mat.texture >= 0) base = base * sc.textures[mat.texture].sample(texcoord[0], texcoord[1]);
+    V3 fs = base / kPi;
     float G = (cos_shadow * cos_light) / light_dist2;
     float brdf_pdf = fabsf(sr.dir.z / kPi) * cos_light / light_dist2;
     float mis = light_pdf / (brdf_pdf + light_pdf);
     return lm->emissive * fs * mis * (G / light_pdf);
 }
 
-// PathTracer::Radiance (src/src/s3d_pt.cpp:182-246).  The miss term uses the
-// constant environment in place of SampleIBL(dir)*10 (s3d_scene.h:80-82).
+// PathTracer::Radiance (src/src/s3d_pt.cpp:182-246).  The miss term is
+// SampleIBL(dir) = lat-long SampleColor(dir) * 10 (s3d_scene.h:80-82), or the
+// constant environment when the scene has no IBL texture.
 static V3 radiance(const Scene &sc, V3 org, V3 dir, PCG &rng, int max_depth, int rr_start,
                    uint64_t &ext_rays, uint64_t &shadow_rays) {
     V3 W(1, 1, 1), L(0, 0, 0);
@@ -900,18 +961,23 @@ static V3 radiance(const Scene &sc, V3 org, V3 dir, PCG &rng, int max_depth, int
         Hit h;
         ext_rays++;
         if (!scene_intersect(sc, ray, h)) {
-            L = L + W * sc.env;
+            if (sc.env_tex >= 0) L = L + W * (sc.textures[sc.env_tex].sample_dir(ray.dir) * V3(10.0f, 10.0f, 10.0f));
+            else L = L + W * sc.env;
             break;
         }
         V3 pos, normal;
         const Material *mat;
-        hit_surface(sc, ray, h, pos, normal, mat);
+        float uv[2];
+        hit_surface(sc, ray, h, pos, normal, mat, uv);
         L = L + W * mat->emissive;
         if (!has_delta(*mat) && !sc.lights.empty())
-            L = L + W * nee(sc, pos, normal, *mat, rng, shadow_rays);
+            L = L + W * nee(sc, pos, normal, uv, *mat, rng, shadow_rays);
         V3 out;
         bool dice = false;
-        W = W * shade(*mat, ray.dir, normal, rng, out, dice);
+        // TexturedMaterial::Shade = inner Shade times the texture
+        V3 wt = shade(*mat, ray.dir, normal, rng, out, dice);
+        if (mat->texture >= 0) wt = wt * sc.textures[mat->texture].sample(uv[0], uv[1]);
+        W = W * wt;
         if (dice && depth >= rr_start) break;
         if (W.x < FLT_EPSILON && W.y < FLT_EPSILON && W.z < FLT_EPSILON) break;
         ray.pos = pos;
@@ -1125,6 +1191,15 @@ void *orc_scene_create(const s3dg_scene_desc *d) {
         sc->lights.push_back(L);
     }
     sc->env = load3(d->environment);
+    for (uint32_t i = 0; i < d->texture_count; ++i) {
+        Texture t;
+        t.w = d->textures[i].width;
+        t.h = d->textures[i].height;
+        const float *rgb = d->textures[i].rgb;
+        for (size_t k = 0; k < (size_t)t.w * t.h; ++k) t.px.push_back(V3(rgb[3 * k], rgb[3 * k + 1], rgb[3 * k + 2]));
+        sc->textures.push_back(std::move(t));
+    }
+    sc->env_tex = d->environment_texture - 1;
     const s3dg_camera_desc &c = d->camera;
     sc->cam_type = c.type;
     sc->cam_pos = load3(c.position);
@@ -1222,8 +1297,10 @@ int orc_render_guides(const void *sp, const s3dg_render_config *cfg, float *albe
             if (scene_intersect(*sc, r, h)) {
                 V3 p;
                 const Material *m;
-                hit_surface(*sc, r, h, p, n, m);
+                float uv[2];
+                hit_surface(*sc, r, h, p, n, m, uv);
                 a = base_color(*m);
+                if (m->texture >= 0) a = a * sc->textures[m->texture].sample(uv[0], uv[1]);
                 t = h.distance;
             }
             store3(albedo + 3 * idx, a);

@@ -38,6 +38,11 @@ _SIGNATURES = {
     "s3dg_scene_get_stats": [C.c_void_p, C.POINTER(abi.SceneStats)],
     "s3dg_scene_mesh_order": [C.c_void_p, C.c_uint32, C.c_void_p],
     "s3dg_build_order": [C.c_uint32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_image_load_bmp": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_void_p)],
+    "s3dg_image_load_hdr": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_void_p)],
+    "s3dg_image_free": [C.c_void_p],
+    "s3dg_smd_load": [C.c_char_p, C.POINTER(C.POINTER(abi.Smd))],
+    "s3dg_smd_free": [C.POINTER(abi.Smd)],
     "s3dg_trace": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p],
     "s3dg_trace_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p],
     "s3dg_trace_counted_device": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p, C.c_void_p],
@@ -83,6 +88,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         fn.restype = C.c_int
     lib.s3dg_last_error.restype = C.c_char_p
     lib.s3dg_stream.restype = C.c_void_p
+    lib.s3dg_image_free.restype = None
+    lib.s3dg_smd_free.restype = None
     _lib = lib
     return lib
 
@@ -105,6 +112,55 @@ def build_order(vertices: np.ndarray, kind: int = abi.BVH8):
     if rc != abi.OK:
         raise S3dgError(f"s3dg_build_order failed ({rc})")
     return order, nodes.value
+
+
+def _load_image(fn, path: str) -> np.ndarray:
+    w, h, p = C.c_int32(), C.c_int32(), C.c_void_p()
+    rc = fn(os.fsencode(path), C.byref(w), C.byref(h), C.byref(p))
+    if rc != abi.OK:
+        raise S3dgError(f"cannot load {path} ({rc})")
+    lib = load_library()
+    try:
+        buf = (C.c_float * (w.value * h.value * 3)).from_address(p.value)
+        return np.frombuffer(buf, dtype=np.float32).reshape(h.value, w.value, 3).copy()
+    finally:
+        lib.s3dg_image_free(p)
+
+
+def load_bmp(path: str) -> np.ndarray:
+    """24-bit BMP as (height, width, 3) float32 in file row order (host only)."""
+    return _load_image(load_library().s3dg_image_load_bmp, path)
+
+
+def load_hdr(path: str) -> np.ndarray:
+    """Radiance RGBE image as (height, width, 3) float32 (host only)."""
+    return _load_image(load_library().s3dg_image_load_hdr, path)
+
+
+def load_smd(path: str) -> dict:
+    """SMD mesh file (host only): vertices (VERTEX_DTYPE), triangle_material
+    (int32, -1: none), materials (dicts as SceneDesc takes them, with a texture
+    index into texture_paths or -1) and texture_paths."""
+    lib = load_library()
+    p = C.POINTER(abi.Smd)()
+    rc = lib.s3dg_smd_load(os.fsencode(path), C.byref(p))
+    if rc != abi.OK:
+        raise S3dgError(f"cannot load {path} ({rc})")
+    try:
+        m = p.contents
+        nv = m.vertex_count
+        vbuf = (C.c_byte * (nv * VERTEX_DTYPE.itemsize)).from_address(m.vertices)
+        vertices = np.frombuffer(vbuf, dtype=VERTEX_DTYPE).copy()
+        tri_mat = np.ctypeslib.as_array(m.triangle_material, shape=(nv // 3,)).copy()
+        mats = []
+        for i in range(m.material_count):
+            c = m.materials[i]
+            mats.append(dict(type=c.type, diffuse=tuple(c.diffuse), specular=tuple(c.specular),
+                             emissive=tuple(c.emissive), ior=c.ior, power=c.power, texture=c.texture - 1))
+        paths = [m.texture_paths[i].value.decode() for i in range(m.texture_count)]
+        return dict(vertices=vertices, triangle_material=tri_mat, materials=mats, texture_paths=paths)
+    finally:
+        lib.s3dg_smd_free(p)
 
 
 class Context:

@@ -37,7 +37,17 @@ class Vertex(C.Structure):
 
 class Material(C.Structure):
     _fields_ = [("type", C.c_int32), ("diffuse", C.c_float * 3), ("specular", C.c_float * 3),
-                ("emissive", C.c_float * 3), ("ior", C.c_float), ("power", C.c_float)]
+                ("emissive", C.c_float * 3), ("ior", C.c_float), ("power", C.c_float), ("texture", C.c_int32)]
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgb", C.c_void_p)]
+
+
+class Smd(C.Structure):
+    _fields_ = [("vertex_count", C.c_uint32), ("vertices", C.c_void_p), ("triangle_material", C.POINTER(C.c_int32)),
+                ("material_count", C.c_uint32), ("materials", C.POINTER(Material)),
+                ("texture_count", C.c_uint32), ("texture_paths", C.POINTER(C.c_char * 512))]
 
 
 class MeshDesc(C.Structure):
@@ -67,7 +77,9 @@ class SceneDescC(C.Structure):
                 ("shape_count", C.c_uint32), ("shapes", C.POINTER(ShapeDesc)),
                 ("material_count", C.c_uint32), ("materials", C.POINTER(Material)),
                 ("light_count", C.c_uint32), ("lights", C.POINTER(LightDesc)),
-                ("camera", CameraDesc), ("environment", C.c_float * 3)]
+                ("camera", CameraDesc), ("environment", C.c_float * 3),
+                ("texture_count", C.c_uint32), ("textures", C.POINTER(TextureDesc)),
+                ("environment_texture", C.c_int32)]
 
 
 class RenderConfig(C.Structure):
@@ -152,10 +164,13 @@ class Camera:
 class SceneDesc:
     meshes: list = field(default_factory=list)
     shapes: list = field(default_factory=list)
-    materials: list = field(default_factory=list)   # dicts: type, diffuse, specular, emissive, ior, power
+    materials: list = field(default_factory=list)   # dicts: type, diffuse, specular, emissive, ior, power,
+                                                    # texture (index into textures, -1 / absent: none)
     lights: list = field(default_factory=list)
     camera: Camera | None = None
     environment: tuple = (0.0, 0.0, 0.0)
+    textures: list = field(default_factory=list)    # float32 arrays (height, width, 3), row y = GetColor row y
+    environment_texture: int = -1                   # IBL: index into textures, -1: constant environment
 
     def triangle_count(self) -> int:
         return sum(len(m.vertices) // 3 for m in self.meshes)
@@ -194,6 +209,7 @@ class SceneDesc:
             mats[i].emissive[:] = list(m.get("emissive", (0.0, 0.0, 0.0)))
             mats[i].ior = m.get("ior", 1.5)
             mats[i].power = m.get("power", 1.0)
+            mats[i].texture = int(m.get("texture", -1)) + 1
         lights = (LightDesc * max(1, len(self.lights)))()
         for i, l in enumerate(self.lights):
             lights[i].type = l.type
@@ -203,7 +219,15 @@ class SceneDesc:
             lights[i].corner[:] = list(l.corner)
             lights[i].edge1[:] = list(l.edge1)
             lights[i].edge2[:] = list(l.edge2)
-        keep += [meshes, shapes, mats, lights]
+        texs = (TextureDesc * max(1, len(self.textures)))()
+        for i, t in enumerate(self.textures):
+            a = np.ascontiguousarray(t, dtype=np.float32)
+            assert a.ndim == 3 and a.shape[2] == 3
+            keep.append(a)
+            texs[i].width = a.shape[1]
+            texs[i].height = a.shape[0]
+            texs[i].rgb = a.ctypes.data
+        keep += [meshes, shapes, mats, lights, texs]
         d = SceneDescC()
         d.mesh_count = len(self.meshes)
         d.meshes = meshes
@@ -223,6 +247,9 @@ class SceneDesc:
         d.camera.near_clip = cam.near_clip
         d.camera.lens_radius = cam.lens_radius
         d.environment[:] = list(self.environment)
+        d.texture_count = len(self.textures)
+        d.textures = texs
+        d.environment_texture = self.environment_texture + 1
         return d, keep
 
 

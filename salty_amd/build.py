@@ -21,7 +21,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libs3dg.so")
 
-SOURCES = ["bvh.cpp", "scene.cpp", "trace.hip", "render.hip", "sceneset.hip", "post.hip", "cnn.hip"]
+SOURCES = ["bvh.cpp", "scene.cpp", "io.cpp", "trace.hip", "render.hip", "sceneset.hip", "post.hip", "cnn.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -68,9 +68,25 @@ struct GMat {
     float   power;
     int32_t type;
     int32_t delta;
-    int32_t pad;
+    int32_t texture;        // texture record of a TexturedMaterial, -1: none
 };
 static_assert(sizeof(GMat) == 64, "GMat is 64 bytes");
+
+// Texture record: RGB texels from `offset` (in floats) of the texel array,
+// texel (x, y) at offset + 3 * (width * y + x).
+struct GTexture {
+    uint64_t offset;
+    int32_t  width;
+    int32_t  height;
+};
+static_assert(sizeof(GTexture) == 16, "GTexture is 16 bytes");
+
+// Vertex texture coordinates of a stored triangle (u0 v0 u1 v1 u2 v2), read
+// only when its material is textured.
+struct GTriUV {
+    float uv[6];
+};
+static_assert(sizeof(GTriUV) == 24, "GTriUV is 24 bytes");
 
 // Per top-level shape: its shape record and the material its hits shade with
 // in place of the triangles' own (-1: none; a sphere's own material).

@@ -44,7 +44,7 @@ struct SceneRec {
     int32_t light_base;     // first light of the scene in the light array
     int32_t light_count;
     float env[3];
-    int32_t pad;
+    int32_t env_tex;        // IBL texture (index into SceneDev::textures), -1: constant env
     GCamera cam;
 };
 
@@ -59,6 +59,9 @@ struct SceneDev {
     const GShapeRef *shape_ref;     // top-level shape index -> record + material override
     const SceneRec *recs;
     const uint32_t *tri_rank;   // depth-first rank of each stored triangle (tie order)
+    const GTexture *textures;   // texture records (null when the scene has none)
+    const float *texels;        // RGB texels of all textures
+    const GTriUV *tri_uv;       // vertex texture coordinates per stored triangle (null: no textures)
     int32_t stack_need;     // deepest stack the trees can require
     uint32_t tri_batch;     // lanes with pending triangles that keep a step testing
 };
@@ -97,8 +100,11 @@ struct s3dg_scene {
     // device allocations
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
     void *d_mats = nullptr, *d_lights = nullptr, *d_shape_ref = nullptr, *d_rank = nullptr;
+    void *d_textures = nullptr, *d_texels = nullptr, *d_uv = nullptr;
     struct {
         uint32_t nodes, tris, insts, mats, lights, shapes;   // element counts of the device arrays
+        uint32_t textures;                                    // texture records
+        uint64_t texel_floats;                                // floats of the texel array
     } counts{};
     s3dg::Workspace ws;
 };

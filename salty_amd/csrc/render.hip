@@ -199,6 +199,13 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *cursor) {
     return __shfl(base, 0);
 }
 
+// Radiance of a ray leaving the scene: Scene::SampleIBL (s3d_scene.h:80-82),
+// the lat-long texture times 10, or the scene's constant environment.
+__device__ __forceinline__ V3 environment(const SceneDev &S, const SceneRec &rec, V3 dir) {
+    if (rec.env_tex < 0) return ld3(rec.env);
+    return tex_sample_dir(S.texels, S.textures[rec.env_tex], dir) * 10.0f;
+}
+
 __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const BatchDesc *__restrict__ bd,
                                                     const Seg *__restrict__ segs) {
     const uint32_t npaths = bd->paths, nseg = bd->nseg;
@@ -295,8 +302,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
             V3 L = v3(P.lr[p], P.lg[p], P.lb[p]);
             const int32_t prim = P.hprim[p];
             if (prim == -1) {
-                // miss: L += W * environment (SampleIBL(dir)*10 in the reference)
-                L = L + W * ld3(S.recs[scene_of(P, p)].env);
+                // miss: L += W * SampleIBL(dir) (s3d_pt.cpp:199-203), or the
+                // constant environment when the scene has no IBL texture
+                L = L + W * environment(S, S.recs[scene_of(P, p)], v3(P.dx[p], P.dy[p], P.dz[p]));
             } else {
                 const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
                 const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
@@ -314,6 +322,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                 }
                 const GMat &m = S.mats[mat];
                 L = L + W * ld3(m.emissive);
+                // TexturedMaterial: Shade and GetBaseColor times the texture
+                // at the hit's texcoord (1 when untextured, an exact no-op)
+                V3 tcol = v3(1.0f, 1.0f, 1.0f);
+                if (m.texture >= 0) {
+                    float tu, tv;
+                    if (prim == -2) sphere_texcoord(normal, tu, tv);
+                    else tri_texcoord(S.tri_uv[prim], P.hu[p], P.hv[p], tu, tv);
+                    tcol = tex_sample(S.texels, S.textures[m.texture], tu, tv);
+                }
                 Pcg rng;
                 rng.state = P.rng[p];
                 const SceneRec &rec = S.recs[scene_of(P, p)];
@@ -368,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                             ln = tri_normal(S.shade[ltri], lu, lv);
                             le = ld3(S.mats[S.shade[ltri].material].emissive);
                         }
-                        const V3 fs = base_color(m) / kPi;
+                        const V3 fs = (base_color(m) * tcol) / kPi;
                         const float cos_shadow = fabsf(dot3(normal, ldir));
                         const float cos_light = fabsf(dot3(ln, -ldir));
                         const float G = (cos_shadow * cos_light) / d2;
@@ -384,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                 }
                 V3 out;
                 bool dice = false;
-                W = W * shade(m, d, normal, rng, out, dice);
+                W = W * (shade(m, d, normal, rng, out, dice) * tcol);
                 P.rng[p] = rng.state;
                 P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
                 const bool stop = (dice && depth >= rr_start) ||
@@ -500,7 +517,14 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
                 nn = tri_normal(S.shade[h.prim], h.u, h.v);
                 if (mat < 0) mat = S.shade[h.prim].material;
             }
-            a = base_color(S.mats[mat]);
+            const GMat &m = S.mats[mat];
+            a = base_color(m);
+            if (m.texture >= 0) {
+                float tu, tv;
+                if (h.prim == -2) sphere_texcoord(nn, tu, tv);
+                else tri_texcoord(S.tri_uv[h.prim], h.u, h.v, tu, tv);
+                a = a * tex_sample(S.texels, S.textures[m.texture], tu, tv);
+            }
             t = h.t;
         }
         albedo[3 * p] = a.x; albedo[3 * p + 1] = a.y; albedo[3 * p + 2] = a.z;

@@ -221,12 +221,46 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     if (!sc) return set_error(ctx, S3DG_ERR_NOMEM, "host allocation");
     sc->ctx = ctx;
 
+    // ---- textures (TexturedMaterial, IBL) -----------------------------------
+    if (d->texture_count && !d->textures) {
+        delete sc;
+        return set_error(ctx, S3DG_ERR_INVALID, "null texture array");
+    }
+    std::vector<GTexture> texrecs(d->texture_count);
+    uint64_t texel_floats = 0;
+    for (uint32_t i = 0; i < d->texture_count; ++i) {
+        const s3dg_texture_desc &t = d->textures[i];
+        if (t.width <= 0 || t.height <= 0 || !t.rgb || (int64_t)t.width * t.height > (1ll << 28)) {
+            delete sc;
+            return set_error(ctx, S3DG_ERR_INVALID, "bad texture");
+        }
+        texrecs[i].offset = texel_floats;
+        texrecs[i].width = t.width;
+        texrecs[i].height = t.height;
+        texel_floats += 3ull * (uint64_t)t.width * (uint64_t)t.height;
+    }
+    if (d->environment_texture < 0 || (uint32_t)d->environment_texture > d->texture_count) {
+        delete sc;
+        return set_error(ctx, S3DG_ERR_INVALID, "environment texture out of range");
+    }
+
     std::vector<GMat> mats(d->material_count);
-    for (uint32_t i = 0; i < d->material_count; ++i) make_material(d->materials[i], mats[i]);
+    bool textured = false;
+    for (uint32_t i = 0; i < d->material_count; ++i) {
+        make_material(d->materials[i], mats[i]);
+        const int32_t tx = d->materials[i].texture;
+        if (tx < 0 || (uint32_t)tx > d->texture_count) {
+            delete sc;
+            return set_error(ctx, S3DG_ERR_INVALID, "material texture out of range");
+        }
+        mats[i].texture = tx - 1;
+        textured = textured || tx > 0;
+    }
 
     std::vector<GTri> tris;
     std::vector<GTriShade> shade;
     std::vector<uint32_t> rank;   // depth-first (tie) rank of every stored triangle
+    std::vector<GTriUV> uvs;      // texture coordinates, kept when a material is textured
     Compressed comp;
     std::vector<Aabb> mesh_box(d->mesh_count);
     int32_t blas_depth = 0;
@@ -308,6 +342,14 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
             }
             tris.push_back(t);
             shade.push_back(s);
+            if (textured) {
+                GTriUV w;
+                for (int k = 0; k < 3; ++k) {
+                    w.uv[2 * k] = v[k].texcoord[0];
+                    w.uv[2 * k + 1] = v[k].texcoord[1];
+                }
+                uvs.push_back(w);
+            }
         }
         sc->meshes.push_back(std::move(info));
     }
@@ -488,10 +530,9 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         g.focal_distance = sqrtf((tp.x * tp.x) + (tp.y * tp.y) + (tp.z * tp.z));
     }
     for (int a = 0; a < 3; ++a) sc->rec.env[a] = d->environment[a];
+    sc->rec.env_tex = d->environment_texture - 1;
     sc->rec.light_count = (int32_t)d->light_count;
     sc->rec.light_base = 0;
-    // per BLAS level: one child group and one leaf group; per TLAS level: one
-    // child group and up to eight shapes; one restore entry
     // at most two entries per node on the path (a group of inner children and
     // a group of leaves or shapes) plus the return from an instance
     dv.stack_need = 2 * blas_depth + 2 * tlas_depth + 4;
@@ -508,6 +549,8 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     sc->counts.mats = (uint32_t)mats.size();
     sc->counts.lights = (uint32_t)lights.size();
     sc->counts.shapes = (uint32_t)inst_pos.size();
+    sc->counts.textures = (uint32_t)texrecs.size();
+    sc->counts.texel_floats = texel_floats;
     size_t total = 0;
     int rc;
     if ((rc = upload(ctx, &sc->d_nodes, comp.nodes, total)) || (rc = upload(ctx, &sc->d_tris, tris, total)) ||
@@ -515,9 +558,25 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         (rc = upload(ctx, &sc->d_mats, mats, total)) || (rc = upload(ctx, &sc->d_lights, lights, total)) ||
         (rc = upload(ctx, &sc->d_shape_ref, shape_ref, total)) ||
         (rc = upload(ctx, &sc->d_rec, std::vector<SceneRec>{sc->rec}, total)) ||
-        (rc = upload(ctx, &sc->d_rank, rank, total))) {
+        (rc = upload(ctx, &sc->d_rank, rank, total)) ||
+        (!texrecs.empty() && (rc = upload(ctx, &sc->d_textures, texrecs, total))) ||
+        (!uvs.empty() && (rc = upload(ctx, &sc->d_uv, uvs, total)))) {
         s3dg_scene_destroy(sc);
         return rc;
+    }
+    if (texel_floats) {
+        // texels go up texture by texture, with no host copy of them all
+        hipError_t e = hipMalloc(&sc->d_texels, texel_floats * sizeof(float));
+        for (uint32_t i = 0; e == hipSuccess && i < d->texture_count; ++i)
+            e = hipMemcpy((float *)sc->d_texels + texrecs[i].offset, d->textures[i].rgb,
+                          3ull * (uint64_t)texrecs[i].width * (uint64_t)texrecs[i].height * sizeof(float),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            rc = hip_check(ctx, e, "texel upload");
+            s3dg_scene_destroy(sc);
+            return rc;
+        }
+        total += texel_floats * sizeof(float);
     }
     if ((rc = workspace_init(ctx, sc->ws))) {
         s3dg_scene_destroy(sc);
@@ -532,6 +591,9 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     dv.shape_ref = (const GShapeRef *)sc->d_shape_ref;
     dv.recs = (const SceneRec *)sc->d_rec;
     dv.tri_rank = (const uint32_t *)sc->d_rank;
+    dv.textures = (const GTexture *)sc->d_textures;
+    dv.texels = (const float *)sc->d_texels;
+    dv.tri_uv = (const GTriUV *)sc->d_uv;
 
     sc->stats.triangles = tris.size();
     sc->stats.blas_nodes = comp.nodes.size();
@@ -546,7 +608,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
 extern "C" int s3dg_scene_destroy(s3dg_scene *sc) {
     if (!sc) return S3DG_ERR_INVALID;
     void *ptrs[] = {sc->d_nodes, sc->d_tris, sc->d_shade, sc->d_insts, sc->d_mats, sc->d_lights,
-                    sc->d_shape_ref, sc->d_rec, sc->d_rank};
+                    sc->d_shape_ref, sc->d_rec, sc->d_rank, sc->d_textures, sc->d_texels, sc->d_uv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     workspace_free(sc->ws);

@@ -28,7 +28,7 @@ struct s3dg_scene_set {
     std::vector<SceneRec> recs;
     void *d_nodes = nullptr, *d_tris = nullptr, *d_shade = nullptr, *d_insts = nullptr;
     void *d_mats = nullptr, *d_lights = nullptr, *d_shape_ref = nullptr, *d_recs = nullptr;
-    void *d_rank = nullptr;
+    void *d_rank = nullptr, *d_textures = nullptr, *d_texels = nullptr, *d_uv = nullptr;
     Workspace ws;
 };
 
@@ -72,6 +72,16 @@ __global__ void k_rebase_lights(GLight *__restrict__ lights, uint32_t n, int32_t
     }
 }
 
+__global__ void k_rebase_mats(GMat *__restrict__ mats, uint32_t n, int32_t tex_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (mats[i].texture >= 0) mats[i].texture += tex_off;
+}
+
+__global__ void k_rebase_textures(GTexture *__restrict__ t, uint32_t n, uint64_t texel_off) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        t[i].offset += texel_off;
+}
+
 __global__ void k_add_u32(uint32_t *__restrict__ a, uint32_t n, uint32_t off) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] += off;
 }
@@ -87,7 +97,8 @@ unsigned grid_for(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 
 
 void free_set(s3dg_scene_set *set) {
     void *ptrs[] = {set->d_nodes, set->d_tris,   set->d_shade,          set->d_insts, set->d_mats,
-                    set->d_lights, set->d_shape_ref, set->d_recs, set->d_rank};
+                    set->d_lights, set->d_shape_ref, set->d_recs, set->d_rank,
+                    set->d_textures, set->d_texels, set->d_uv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     workspace_free(set->ws);
@@ -108,10 +119,12 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
     set->ctx = ctx;
     // element offsets of every member
     struct Off {
-        uint32_t nodes, tris, insts, mats, lights, shapes;
+        uint32_t nodes, tris, insts, mats, lights, shapes, textures;
+        uint64_t texels;
     };
     std::vector<Off> off(n);
-    Off tot{0, 0, 0, 0, 0, 0};
+    Off tot{0, 0, 0, 0, 0, 0, 0, 0};
+    bool any_uv = false;
     int32_t stack_need = 4;
     for (uint32_t k = 0; k < n; ++k) {
         const auto &c = scenes[k]->counts;
@@ -122,6 +135,9 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         tot.mats += c.mats;
         tot.lights += c.lights;
         tot.shapes += c.shapes;
+        tot.textures += c.textures;
+        tot.texels += c.texel_floats;
+        any_uv = any_uv || scenes[k]->d_uv != nullptr;
         stack_need = std::max(stack_need, scenes[k]->dev.stack_need);
     }
     auto alloc = [&](void **p, size_t bytes) -> hipError_t { return hipMalloc(p, std::max<size_t>(bytes, 256)); };
@@ -134,7 +150,10 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         (e = alloc(&set->d_lights, (size_t)tot.lights * sizeof(GLight))) != hipSuccess ||
         (e = alloc(&set->d_shape_ref, (size_t)tot.shapes * sizeof(GShapeRef))) != hipSuccess ||
         (e = alloc(&set->d_recs, (size_t)n * sizeof(SceneRec))) != hipSuccess ||
-        (e = alloc(&set->d_rank, (size_t)tot.tris * sizeof(uint32_t))) != hipSuccess) {
+        (e = alloc(&set->d_rank, (size_t)tot.tris * sizeof(uint32_t))) != hipSuccess ||
+        (tot.textures && (e = alloc(&set->d_textures, (size_t)tot.textures * sizeof(GTexture))) != hipSuccess) ||
+        (tot.texels && (e = alloc(&set->d_texels, (size_t)tot.texels * sizeof(float))) != hipSuccess) ||
+        (any_uv && (e = alloc(&set->d_uv, (size_t)tot.tris * sizeof(GTriUV))) != hipSuccess)) {
         free_set(set);
         return hip_check(ctx, e, "hipMalloc(scene set)");
     }
@@ -166,7 +185,10 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
                       {(GMat *)set->d_mats + o.mats, sc->d_mats, (size_t)c.mats * sizeof(GMat)},
                       {lights, sc->d_lights, (size_t)c.lights * sizeof(GLight)},
                       {refs, sc->d_shape_ref, (size_t)c.shapes * sizeof(GShapeRef)},
-                      {rank, sc->d_rank, (size_t)c.tris * sizeof(uint32_t)}};
+                      {rank, sc->d_rank, (size_t)c.tris * sizeof(uint32_t)},
+                      {(GTexture *)set->d_textures + o.textures, sc->d_textures, (size_t)c.textures * sizeof(GTexture)},
+                      {(float *)set->d_texels + o.texels, sc->d_texels, (size_t)c.texel_floats * sizeof(float)},
+                      {(GTriUV *)set->d_uv + o.tris, sc->d_uv, sc->d_uv ? (size_t)c.tris * sizeof(GTriUV) : 0}};
         for (auto &cp : copies)
             if (cp.bytes) {
                 e = hipMemcpyAsync(cp.dst, cp.src, cp.bytes, hipMemcpyDeviceToDevice, s);
@@ -189,6 +211,12 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         if (c.lights)
             hipLaunchKernelGGL(k_rebase_lights, dim3(grid_for(c.lights)), dim3(256), 0, s, lights, c.lights,
                                (int32_t)o.tris, (int32_t)o.insts, (int32_t)o.shapes);
+        if (c.mats)
+            hipLaunchKernelGGL(k_rebase_mats, dim3(grid_for(c.mats)), dim3(256), 0, s, (GMat *)set->d_mats + o.mats,
+                               c.mats, (int32_t)o.textures);
+        if (c.textures)
+            hipLaunchKernelGGL(k_rebase_textures, dim3(grid_for(c.textures)), dim3(256), 0, s,
+                               (GTexture *)set->d_textures + o.textures, c.textures, o.texels);
         if (c.shapes)
             hipLaunchKernelGGL(k_rebase_shape_refs, dim3(grid_for(c.shapes)), dim3(256), 0, s, refs, c.shapes,
                                (int32_t)o.insts, (int32_t)o.mats);
@@ -196,6 +224,7 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
         r.root += o.nodes;
         if (r.single_shape >= 0) r.single_shape += (int32_t)o.shapes;
         r.light_base = (int32_t)o.lights;
+        if (r.env_tex >= 0) r.env_tex += (int32_t)o.textures;
         set->recs[k] = r;
     }
     e = hipMemcpyAsync(set->d_recs, set->recs.data(), n * sizeof(SceneRec), hipMemcpyHostToDevice, s);
@@ -215,6 +244,9 @@ extern "C" int s3dg_scene_set_create(s3dg_context *ctx, uint32_t n, s3dg_scene *
     dv.shape_ref = (const GShapeRef *)set->d_shape_ref;
     dv.recs = (const SceneRec *)set->d_recs;
     dv.tri_rank = (const uint32_t *)set->d_rank;
+    dv.textures = (const GTexture *)set->d_textures;
+    dv.texels = (const float *)set->d_texels;
+    dv.tri_uv = (const GTriUV *)set->d_uv;
     dv.stack_need = stack_need;
     dv.tri_batch = ctx->tri_batch;
     *out = set;

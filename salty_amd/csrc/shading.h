@@ -112,6 +112,68 @@ __device__ __forceinline__ V3 base_color(const GMat &m) {
     return (m.type == S3DG_MAT_LAMBERT || m.type == S3DG_MAT_PLASTIC) ? ld3(m.diffuse) : ld3(m.specular);
 }
 
+constexpr float k1DivPi = 0.31830988618379067153776752674503f;    // F_1DIVPI
+constexpr float k1Div2Pi = 0.15915494309189533576888376337251f;   // F_1DIV2PI
+
+// static_cast<s32>(floor(x)) with NaN taken to 0 and the range held to
+// +-2^30, so that the corner arithmetic below cannot overflow.
+__device__ __forceinline__ int32_t floor_i32(float x) {
+    const float f = floorf(x);
+    if (!(f == f)) return 0;
+    return (int32_t)fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);
+}
+
+// Texture::SampleColor(const Vector2&) (src/src/s3d_texture.cpp:925-952):
+// bilinear weights from the corners around uv * (size - 1), the corners then
+// wrapped by abs(i % size) (a truncating remainder: negative coordinates
+// reflect rather than repeat).
+__device__ __forceinline__ V3 texel(const float *texels, const GTexture &t, int32_t x, int32_t y) {
+    const float *p = texels + t.offset + 3ull * ((uint64_t)(uint32_t)t.width * (uint32_t)y + (uint32_t)x);
+    return v3(p[0], p[1], p[2]);
+}
+__device__ __forceinline__ V3 tex_sample(const float *texels, const GTexture &t, float u, float v) {
+    const float fx = u * (float)(t.width - 1);
+    const float fy = v * (float)(t.height - 1);
+    int32_t x0 = floor_i32(fx), y0 = floor_i32(fy);
+    int32_t x1 = x0 + 1, y1 = y0 + 1;
+    const float xw0 = (float)x1 - fx, xw1 = fx - (float)x0;
+    const float yw0 = (float)y1 - fy, yw1 = fy - (float)y0;
+    x0 = abs(x0 % t.width);
+    x1 = abs(x1 % t.width);
+    y0 = abs(y0 % t.height);
+    y1 = abs(y1 % t.height);
+    return (texel(texels, t, x0, y0) * yw0 + texel(texels, t, x0, y1) * yw1) * xw0 +
+           (texel(texels, t, x1, y0) * yw0 + texel(texels, t, x1, y1) * yw1) * xw1;
+}
+
+// Texture::SampleColor(const Vector3&) (src/src/s3d_texture.cpp:957-972):
+// latitude-longitude lookup; u stays 0 when x or z is within FLT_EPSILON of 0.
+__device__ __forceinline__ V3 tex_sample_dir(const float *texels, const GTexture &t, V3 d) {
+    float u = 0.0f;
+    const float v = acosf(d.y) * k1DivPi;
+    if (!(fabsf(d.x) <= FLT_EPSILON) && !(fabsf(d.z) <= FLT_EPSILON)) {
+        const float phi = atan2f(d.z, d.x);
+        u = ((d.z < 0.0f) ? (phi + k2Pi) : phi) * k1Div2Pi;
+    }
+    return tex_sample(texels, t, u, v);
+}
+
+// Texture coordinates of a hit (the texcoord half of CalcParam): barycentric
+// blend of a triangle's vertex coordinates (src/src/s3d_triangle.cpp:107-121),
+// or a sphere's longitude / latitude (src/src/s3d_sphere.cpp:94-104).
+__device__ __forceinline__ void tri_texcoord(const GTriUV &w, float beta, float gamma, float &u, float &v) {
+    const float alpha = 1.0f - beta - gamma;
+    u = ((w.uv[0] * alpha) + (w.uv[2] * beta)) + (w.uv[4] * gamma);
+    v = ((w.uv[1] * alpha) + (w.uv[3] * beta)) + (w.uv[5] * gamma);
+}
+__device__ __forceinline__ void sphere_texcoord(V3 n, float &u, float &v) {
+    const float theta = acosf(n.y);
+    float phi = atan2f(n.x, n.z);
+    if (phi < 0.0f) phi += k2Pi;
+    u = phi * k1Div2Pi;
+    v = (kPi - theta) * k1DivPi;
+}
+
 // IMaterial::Shade of Lambert (src/src/s3d_lambert.cpp:64-86), Mirror
 // (s3d_mirror.cpp:59-71), Glass (s3d_glass.cpp:60-128), Phong (s3d_phong.cpp:65-88)
 // and Plastic (s3d_plastic.cpp:71-142).  Two draws passed as arguments of one

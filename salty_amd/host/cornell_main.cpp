@@ -5,9 +5,15 @@
 //
 // usage: s3d_cornell <width> <height> <passes> <out.ppm> [sum.bin [geometry.bin]]
 // geometry.bin: per mesh a u32 vertex count and the vertices, then the camera fov.
+//
+//        s3d_cornell --smd <file.smd> <width> <height> <passes> <out.ppm> <sum.bin> <setup.bin>
+// renders an SMD mesh (Mesh::Create(filename): its materials and BMP textures)
+// under a spherical light placed from its bounding box; setup.bin records the
+// light centre and radius, the camera position and target and the fov (f32).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "s3d_gpu.h"
@@ -38,7 +44,64 @@ void box(std::vector<Vertex> &v, f32 sx, f32 sy, f32 sz, f32 cx, f32 cz, f32 deg
 
 }  // namespace
 
+int render_smd(int argc, char **argv) {
+    if (argc < 9) {
+        fprintf(stderr, "usage: %s --smd file.smd width height passes out.ppm sum.bin setup.bin\n", argv[0]);
+        return 2;
+    }
+    const int W = atoi(argv[3]), H = atoi(argv[4]), passes = atoi(argv[5]);
+    IShape *mesh = Mesh::Create(argv[2]);
+    if (!mesh) {
+        fprintf(stderr, "cannot load %s\n", argv[2]);
+        return 1;
+    }
+    const std::vector<Vertex> &v = static_cast<Mesh *>(mesh)->vertices;
+    Vector3 lo = v[0].Position, hi = v[0].Position;
+    for (const Vertex &x : v) {
+        lo = Vector3(std::fmin(lo.x, x.Position.x), std::fmin(lo.y, x.Position.y), std::fmin(lo.z, x.Position.z));
+        hi = Vector3(std::fmax(hi.x, x.Position.x), std::fmax(hi.y, x.Position.y), std::fmax(hi.z, x.Position.z));
+    }
+    const Vector3 c((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+    const f32 ext = std::fmax(hi.x - lo.x, std::fmax(hi.y - lo.y, hi.z - lo.z));
+    const f32 setup[11] = {c.x, hi.y + 0.6f * ext, c.z + 0.3f * ext, 0.1f * ext, c.x, c.y, c.z + 1.6f * ext,
+                           c.x, c.y, c.z, 2.0f * std::tan(20.0f * 3.14159265358979f / 180.0f)};
+    IMaterial *glow = MaterialFactory::CreateLambert(Color3(0, 0, 0), nullptr, Color3(20, 20, 20));
+    IShape *lamp = Sphere::Create(setup[3], Vector3(setup[0], setup[1], setup[2]), glow);
+    Scene scene;
+    scene.AddShape(mesh);
+    scene.AddShape(lamp);
+    scene.AddLight(lamp);
+    scene.SetEnvironment(Color3(0.05f, 0.05f, 0.05f));
+    scene.GetCamera().Update(Vector3(setup[4], setup[5], setup[6]), Vector3(setup[7], setup[8], setup[9]),
+                             Vector3(0, 1, 0), setup[10], (f32)W / (f32)H, 1.0f);
+    PathTracer pt;
+    PathTracer::Config cfg;
+    cfg.Width = W;
+    cfg.Height = H;
+    cfg.MaxBounceCount = 6;
+    cfg.MaxPasses = passes;
+    if (!pt.Run(cfg, scene) || !pt.Capture(scene, argv[6])) {
+        fprintf(stderr, "render failed: %s\n", s3dg_last_error(scene.GetContext()));
+        return 1;
+    }
+    FILE *f = fopen(argv[7], "wb");
+    if (!f) return 1;
+    fwrite(pt.GetSum().data(), sizeof(Color3), pt.GetSum().size(), f);
+    fclose(f);
+    f = fopen(argv[8], "wb");
+    if (!f) return 1;
+    fwrite(setup, sizeof(f32), 11, f);
+    fclose(f);
+    printf("passes=%d rays=%llu seconds=%.3f\n", pt.GetPassCount(), (unsigned long long)pt.GetRayCount(),
+           pt.GetSeconds());
+    mesh->Release();
+    lamp->Release();
+    glow->Release();
+    return 0;
+}
+
 int main(int argc, char **argv) {
+    if (argc > 1 && std::string(argv[1]) == "--smd") return render_smd(argc, argv);
     if (argc < 5) {
         fprintf(stderr, "usage: %s width height passes out.ppm [sum.bin]\n", argv[0]);
         return 2;
@@ -48,7 +111,7 @@ int main(int argc, char **argv) {
     IMaterial *white = MaterialFactory::CreateLambert(Color3(0.75f, 0.75f, 0.75f));
     IMaterial *red = MaterialFactory::CreateLambert(Color3(0.75f, 0.25f, 0.25f));
     IMaterial *green = MaterialFactory::CreateLambert(Color3(0.25f, 0.75f, 0.25f));
-    IMaterial *light = MaterialFactory::CreateLambert(Color3(0, 0, 0), Color3(15, 15, 15));
+    IMaterial *light = MaterialFactory::CreateLambert(Color3(0, 0, 0), nullptr, Color3(15, 15, 15));
     std::vector<Vertex> vw, vr, vg, vl;
     quad(vw, {0, 0, 0}, {0, 0, 1}, {1, 0, 1}, {1, 0, 0});
     quad(vw, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1});

@@ -7,6 +7,8 @@
 //   IShape factories      Mesh::Create (s3d_mesh.h:42-47), Instance::Create
 //                         (s3d_instance.h:40), Sphere::Create (s3d_sphere.h:40)
 //   MaterialFactory       src/include/s3d_materialfactory.h:22-49
+//   Texture               src/include/s3d_texture.h (Init from BMP / HDR files)
+//   Mesh::Create(file)    src/src/s3d_mesh.cpp:250-465 (SMD meshes)
 //   Scene                 src/include/s3d_scene.h:22-109 (Intersect, GetRay)
 //   PathTracer::Config    src/include/s3d_pt.h:33-40, PathTracer::Run
 //   ToneMapper::Map       src/include/s3d_tonemapper.h:20-33
@@ -114,19 +116,62 @@ inline void SafeRelease(T *&p) {
 }
 
 // ---------------------------------------------------------------------------
+// Textures (s3d::Texture): RGB texels in memory, loaded from the reference's
+// BMP and HDR formats by the library's loaders (TGA is not supported here).
+// Like the reference's, a Texture is a plain object owned by its creator;
+// materials keep a pointer to it.
+// ---------------------------------------------------------------------------
+enum TEXTURE_FILE_FORMAT { TFF_BMP = 0, TFF_TGA, TFF_HDR };
+
+class Texture {
+public:
+    bool Init(const char *path, TEXTURE_FILE_FORMAT format) {
+        Term();
+        int32_t w = 0, h = 0;
+        float *rgb = nullptr;
+        int rc = S3DG_ERR_UNSUPPORTED;
+        if (format == TFF_BMP) rc = s3dg_image_load_bmp(path, &w, &h, &rgb);
+        else if (format == TFF_HDR) rc = s3dg_image_load_hdr(path, &w, &h, &rgb);
+        if (rc != S3DG_OK) return false;
+        m_Width = w;
+        m_Height = h;
+        m_Texels.assign(rgb, rgb + 3 * (size_t)w * h);
+        s3dg_image_free(rgb);
+        return true;
+    }
+    void Term() {
+        m_Width = m_Height = 0;
+        m_Texels.clear();
+    }
+    bool IsValid() const { return m_Width > 0 && m_Height > 0; }
+    s32 GetWidth() const { return m_Width; }
+    s32 GetHeight() const { return m_Height; }
+    const float *GetTexels() const { return m_Texels.data(); }
+
+private:
+    s32 m_Width = 0, m_Height = 0;
+    std::vector<float> m_Texels;
+};
+
+// ---------------------------------------------------------------------------
 // Materials
 // ---------------------------------------------------------------------------
 struct IMaterial : IReference {
     s3dg_material desc{};
+    const Texture *texture = nullptr;   // TexturedMaterial wrapper when set
     Color3 GetEmissive() const { return Color3(desc.emissive[0], desc.emissive[1], desc.emissive[2]); }
     bool HasDelta() const { return desc.type == S3DG_MAT_MIRROR || desc.type == S3DG_MAT_GLASS; }
 };
 
+// MaterialFactory::Create* with the reference's signatures: a texture wraps
+// the material in a TexturedMaterial (an invalid texture leaves it plain; the
+// reference would sample an empty texture).
 struct MaterialFactory {
     static IMaterial *Make(int type, const Color3 &diffuse, const Color3 &specular, f32 ior, f32 power,
-                           const Color3 &emissive) {
+                           const Texture *pTexture, const Color3 &emissive) {
         IMaterial *m = new (std::nothrow) IMaterial();
         if (!m) return nullptr;
+        if (pTexture && pTexture->IsValid()) m->texture = pTexture;
         m->desc.type = type;
         m->desc.diffuse[0] = diffuse.x; m->desc.diffuse[1] = diffuse.y; m->desc.diffuse[2] = diffuse.z;
         m->desc.specular[0] = specular.x; m->desc.specular[1] = specular.y; m->desc.specular[2] = specular.z;
@@ -135,21 +180,25 @@ struct MaterialFactory {
         m->desc.power = power;
         return m;
     }
-    static IMaterial *CreateLambert(const Color3 &diffuse, const Color3 &emissive = Color3()) {
-        return Make(S3DG_MAT_LAMBERT, diffuse, Color3(), 1.0f, 1.0f, emissive);
+    static IMaterial *CreateLambert(const Color3 &diffuse, const Texture *pTexture = nullptr,
+                                    const Color3 &emissive = Color3()) {
+        return Make(S3DG_MAT_LAMBERT, diffuse, Color3(), 1.0f, 1.0f, pTexture, emissive);
     }
-    static IMaterial *CreateMirror(const Color3 &specular, const Color3 &emissive = Color3()) {
-        return Make(S3DG_MAT_MIRROR, Color3(), specular, 1.0f, 1.0f, emissive);
+    static IMaterial *CreatePhong(const Color3 &specular, f32 power, const Texture *pTexture = nullptr,
+                                  const Color3 &emissive = Color3()) {
+        return Make(S3DG_MAT_PHONG, Color3(), specular, 1.0f, power, pTexture, emissive);
     }
-    static IMaterial *CreateGlass(const Color3 &specular, f32 ior, const Color3 &emissive = Color3()) {
-        return Make(S3DG_MAT_GLASS, Color3(), specular, ior, 1.0f, emissive);
+    static IMaterial *CreateMirror(const Color3 &specular, const Texture *pTexture = nullptr,
+                                   const Color3 &emissive = Color3()) {
+        return Make(S3DG_MAT_MIRROR, Color3(), specular, 1.0f, 1.0f, pTexture, emissive);
     }
-    static IMaterial *CreatePhong(const Color3 &specular, f32 power, const Color3 &emissive = Color3()) {
-        return Make(S3DG_MAT_PHONG, Color3(), specular, 1.0f, power, emissive);
+    static IMaterial *CreateGlass(const Color3 &specular, f32 ior, const Texture *pTexture = nullptr,
+                                  const Color3 &emissive = Color3()) {
+        return Make(S3DG_MAT_GLASS, Color3(), specular, ior, 1.0f, pTexture, emissive);
     }
     static IMaterial *CreatePlastic(const Color3 &diffuse, const Color3 &specular, f32 power,
-                                    const Color3 &emissive = Color3()) {
-        return Make(S3DG_MAT_PLASTIC, diffuse, specular, 1.0f, power, emissive);
+                                    const Texture *pTexture = nullptr, const Color3 &emissive = Color3()) {
+        return Make(S3DG_MAT_PLASTIC, diffuse, specular, 1.0f, power, pTexture, emissive);
     }
 };
 
@@ -163,10 +212,60 @@ struct IShape : IReference {
 
 struct Mesh : IShape {
     std::vector<Vertex> vertices;
-    IMaterial *material = nullptr;
+    IMaterial *material = nullptr;                 // one material for every triangle, or
+    std::vector<IMaterial *> materials;            // per-triangle materials (SMD files)
+    std::vector<int32_t> triangle_material;
+    std::vector<Texture *> textures;               // owned, as Mesh::m_Textures
     int bvh_kind = S3DG_BVH8;
     Mesh() : IShape(MESH) {}
-    ~Mesh() override { SafeRelease(material); }
+    ~Mesh() override {
+        SafeRelease(material);
+        for (IMaterial *m : materials) m->Release();
+        for (Texture *t : textures) delete t;
+    }
+    // Mesh::Create(const char* filename) (s3d_mesh.cpp:250-465): an SMD file
+    // with its materials and BMP textures.  A texture that fails to load leaves
+    // its material untextured (the reference logs a warning); a triangle
+    // without a material makes the mesh fail to build.
+    static IShape *Create(const char *filename) {
+        s3dg_smd *smd = nullptr;
+        if (s3dg_smd_load(filename, &smd) != S3DG_OK) return nullptr;
+        Mesh *m = new (std::nothrow) Mesh();
+        if (!m) {
+            s3dg_smd_free(smd);
+            return nullptr;
+        }
+        for (uint32_t i = 0; i < smd->texture_count; ++i) {
+            Texture *t = new Texture();
+            if (!t->Init(smd->texture_paths[i], TFF_BMP))
+                fprintf(stderr, "Warning : Texture Load Failed. filename = %s\n", smd->texture_paths[i]);
+            m->textures.push_back(t);
+        }
+        for (uint32_t i = 0; i < smd->material_count; ++i) {
+            const s3dg_material &d = smd->materials[i];
+            IMaterial *x = new (std::nothrow) IMaterial();
+            if (!x) break;
+            x->desc = d;
+            x->desc.texture = 0;
+            if (d.texture > 0 && m->textures[d.texture - 1]->IsValid()) x->texture = m->textures[d.texture - 1];
+            m->materials.push_back(x);
+        }
+        m->vertices.resize(smd->vertex_count);
+        for (uint32_t i = 0; i < smd->vertex_count; ++i) {
+            const s3dg_vertex &v = smd->vertices[i];
+            m->vertices[i] = Vertex{Vector3(v.position[0], v.position[1], v.position[2]),
+                                    Vector3(v.normal[0], v.normal[1], v.normal[2]),
+                                    Vector2(v.texcoord[0], v.texcoord[1])};
+        }
+        m->triangle_material.assign(smd->triangle_material, smd->triangle_material + smd->vertex_count / 3);
+        const bool ok = m->materials.size() == smd->material_count;
+        s3dg_smd_free(smd);
+        if (!ok) {
+            m->Release();
+            return nullptr;
+        }
+        return m;
+    }
     // Mesh::Create(u32 vertexCount, Vertex*, IMaterial*) (s3d_mesh.cpp:536-551)
     static IShape *Create(u32 vertexCount, const Vertex *pVertices, IMaterial *pMaterial) {
         if (vertexCount == 0 || vertexCount % 3 != 0 || !pVertices || !pMaterial) return nullptr;
@@ -271,6 +370,9 @@ public:
     }
     Camera &GetCamera() { return m_Camera; }
     void SetEnvironment(const Color3 &c) { m_Env = c; Invalidate(); }
+    // m_IBL: a lat-long texture whose SampleColor(dir) * 10 a missed ray sees
+    // (Scene::SampleIBL); nullptr returns to the constant environment.
+    void SetIBL(const Texture *t) { m_IBL = (t && t->IsValid()) ? t : nullptr; Invalidate(); }
 
     // Uploads the scene (builds every mesh tree with the reference SAH on the host).
     bool Build(int device = 0) {
@@ -315,12 +417,21 @@ public:
             }
         }
         std::vector<s3dg_mesh_desc> md(meshes.size());
+        std::vector<std::vector<int32_t>> tri_mats(meshes.size());
         for (size_t i = 0; i < meshes.size(); ++i) {
             md[i] = s3dg_mesh_desc{};
             md[i].vertex_count = (uint32_t)meshes[i]->vertices.size();
             md[i].vertices = reinterpret_cast<const s3dg_vertex *>(meshes[i]->vertices.data());
-            md[i].material = mat_index(meshes[i]->material);
             md[i].bvh_kind = meshes[i]->bvh_kind;
+            if (meshes[i]->material) {
+                md[i].material = mat_index(meshes[i]->material);
+            } else {
+                // per-triangle materials; a triangle without one is rejected
+                for (int32_t t : meshes[i]->triangle_material)
+                    tri_mats[i].push_back(t >= 0 && (size_t)t < meshes[i]->materials.size()
+                                              ? mat_index(meshes[i]->materials[t]) : -1);
+                md[i].triangle_material = tri_mats[i].data();
+            }
         }
         std::vector<s3dg_light_desc> lights;
         for (IShape *l : m_Lights) {
@@ -340,8 +451,22 @@ public:
             ld.edge2[0] = q.e2.x; ld.edge2[1] = q.e2.y; ld.edge2[2] = q.e2.z;
             lights.push_back(ld);
         }
+        std::vector<const Texture *> texs;
+        auto tex_slot = [&](const Texture *t) -> int32_t {
+            if (!t) return 0;
+            for (size_t i = 0; i < texs.size(); ++i)
+                if (texs[i] == t) return (int32_t)i + 1;
+            texs.push_back(t);
+            return (int32_t)texs.size();
+        };
         std::vector<s3dg_material> mdesc;
-        for (IMaterial *m : mats) mdesc.push_back(m->desc);
+        for (IMaterial *m : mats) {
+            mdesc.push_back(m->desc);
+            mdesc.back().texture = tex_slot(m->texture);
+        }
+        const int32_t ibl = tex_slot(m_IBL);
+        std::vector<s3dg_texture_desc> tdesc;
+        for (const Texture *t : texs) tdesc.push_back(s3dg_texture_desc{t->GetWidth(), t->GetHeight(), t->GetTexels()});
         s3dg_scene_desc sd{};
         sd.mesh_count = (uint32_t)md.size();
         sd.meshes = md.data();
@@ -353,6 +478,9 @@ public:
         sd.lights = lights.data();
         sd.camera = m_Camera.desc;
         sd.environment[0] = m_Env.x; sd.environment[1] = m_Env.y; sd.environment[2] = m_Env.z;
+        sd.texture_count = (uint32_t)tdesc.size();
+        sd.textures = tdesc.data();
+        sd.environment_texture = ibl;
         return s3dg_scene_create(m_Context, &sd, &m_Scene) == S3DG_OK;
     }
 
@@ -385,6 +513,7 @@ private:
     std::vector<QuadLight> m_Quads;
     Camera m_Camera;
     Color3 m_Env;
+    const Texture *m_IBL = nullptr;
     s3dg_context *m_Context = nullptr;
     s3dg_scene *m_Scene = nullptr;
 

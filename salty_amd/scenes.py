@@ -1,6 +1,10 @@
 """Seeded procedural scenes for the rendering hot path.
 
 * ``cornell_box``      -- 36-triangle Cornell box, quad light (radiance 15).
+* ``textured_cornell`` -- the same box with textured materials, a textured
+                          sphere and a lat-long IBL.
+* ``smd_scene``        -- an SMD mesh file with its materials and textures,
+                          lit by a spherical light.
 * ``mesh_scene``       -- ~1.05 M triangles: displaced level-7 icosphere, 64
                           instanced level-3 icospheres, 640 k-triangle ground
                           grid, one spherical light.
@@ -95,6 +99,93 @@ def cornell_box() -> SceneDesc:
     desc.camera = Camera(position=(0.5, 0.5, 2.4), target=(0.5, 0.5, 0.0), upward=(0.0, 1.0, 0.0), fov=FOV_40,
                          aspect=1.0, near_clip=1.0)
     desc.environment = (0.0, 0.0, 0.0)
+    return desc
+
+
+def textured_cornell(seed: int = 0) -> SceneDesc:
+    """The Cornell box with textures and an IBL (TexturedMaterial and
+    Scene::SampleIBL): a seeded 16 x 16 checker on the floor with texture
+    coordinates running over [-1.25, 2.75] (so both wrap directions and the
+    negative-coordinate quirk are hit), a 37 x 23 noise texture on the back wall
+    and on the tall box (a textured mirror), a textured Lambert sphere, and a
+    64 x 32 lat-long environment seen through the open front."""
+    desc = cornell_box()
+    rng = np.random.default_rng(seed + 1000)
+    v = desc.meshes[0].vertices
+    mats = desc.meshes[0].triangle_material.copy()
+
+    def quad_uv(t0, u0, v0, u1, v1):
+        # triangles (a, b, c), (a, c, d) of _quad with uv corners a..d
+        uv = np.array([[u0, v0], [u1, v0], [u1, v1], [u0, v1]], dtype=np.float32)
+        for k, idx in enumerate(((0, 1, 2), (0, 2, 3))):
+            v["texcoord"][3 * (t0 + k):3 * (t0 + k) + 3] = uv[list(idx)]
+
+    quad_uv(0, -1.25, -1.25, 2.75, 2.75)     # floor
+    quad_uv(4, 0.0, 0.0, 1.0, 1.0)           # back wall
+    for t in range(10, 22):                  # tall box faces
+        v["texcoord"][3 * t:3 * t + 3] = [[0.1, 0.2], [0.9, 0.3], [0.5, 0.8]]
+    mats[0:2] = 4                            # floor: textured white Lambert
+    mats[4:6] = 5                            # back wall: textured Lambert
+    mats[10:22] = 6                          # tall box: textured mirror
+    desc.meshes[0].triangle_material = mats
+    checker = np.zeros((16, 16, 3), dtype=np.float32)
+    yy, xx = np.mgrid[0:16, 0:16]
+    on = ((xx // 2 + yy // 2) % 2 == 0)
+    checker[on] = rng.uniform(0.7, 1.0, 3)
+    checker[~on] = rng.uniform(0.05, 0.3, 3)
+    noise = rng.uniform(0.1, 1.0, (23, 37, 3)).astype(np.float32)
+    env = np.zeros((32, 64, 3), dtype=np.float32)
+    env[..., 0] = np.linspace(0.0, 0.6, 64)[None, :]
+    env[..., 1] = np.linspace(0.5, 0.0, 32)[:, None]
+    env[..., 2] = 0.2
+    desc.textures = [checker, noise, env]
+    desc.environment_texture = 2
+    desc.materials += [dict(type=MAT_LAMBERT, diffuse=(0.8, 0.8, 0.8), texture=0),
+                       dict(type=MAT_LAMBERT, diffuse=(0.9, 0.85, 0.8), texture=1),
+                       dict(type=MAT_MIRROR, specular=(0.95, 0.95, 0.95), texture=1),
+                       dict(type=MAT_LAMBERT, diffuse=(0.9, 0.9, 0.9), texture=0)]
+    desc.shapes.append(Shape(SHAPE_SPHERE, material=7, center=(0.72, 0.45, 0.55), radius=0.12))
+    return desc
+
+
+def smd_scene(path: str, light_radiance: float = 20.0, environment=(0.05, 0.05, 0.05)) -> SceneDesc:
+    """A scene around an SMD mesh file (Mesh::Create(filename), its materials
+    and BMP textures): the mesh, a spherical light above it and a camera
+    framing its bounding box.  A texture that does not load leaves its material
+    untextured (the reference logs a warning and keeps the empty texture)."""
+    from . import S3dgError, load_bmp, load_smd
+    m = load_smd(path)
+    textures, slot = [], {}
+    for i, tp in enumerate(m["texture_paths"]):
+        try:
+            textures.append(load_bmp(tp))
+            slot[i] = len(textures) - 1
+        except S3dgError:
+            pass
+    mats = []
+    for mt in m["materials"]:
+        mt = dict(mt)
+        mt["texture"] = slot.get(mt.get("texture", -1), -1)
+        mats.append(mt)
+    light = len(mats)
+    mats.append(dict(type=MAT_LAMBERT, diffuse=(0.0, 0.0, 0.0), emissive=(light_radiance,) * 3))
+    v = m["vertices"]
+    lo, hi = v["position"].min(axis=0), v["position"].max(axis=0)
+    c, ext = (lo + hi) / 2, float(np.max(hi - lo))
+    desc = SceneDesc()
+    tm = m["triangle_material"].copy()
+    tm[tm < 0] = 0
+    desc.meshes = [Mesh(v, triangle_material=tm)]
+    desc.materials = mats
+    desc.textures = textures
+    r = 0.1 * ext
+    centre = (float(c[0]), float(hi[1] + 0.6 * ext), float(c[2] + 0.3 * ext))
+    desc.shapes = [Shape(SHAPE_MESH, mesh=0), Shape(SHAPE_SPHERE, material=light, center=centre, radius=r)]
+    desc.lights = [Light(LIGHT_SHAPE, shape=1)]
+    desc.camera = Camera(position=(float(c[0]), float(c[1]), float(c[2] + 1.6 * ext)),
+                         target=(float(c[0]), float(c[1]), float(c[2])), upward=(0.0, 1.0, 0.0), fov=FOV_40,
+                         aspect=1.0, near_clip=1.0)
+    desc.environment = tuple(environment)
     return desc
 
 
