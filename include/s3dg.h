@@ -278,6 +278,14 @@ int s3dg_scene_get_stats(const s3dg_scene *scene, s3dg_scene_stats *out);
  * holds vertex_count/3 entries. */
 int s3dg_scene_mesh_order(const s3dg_scene *scene, uint32_t mesh, int32_t *order);
 
+/* Instance::UpdateMatrix (src/src/s3d_instance.cpp:95-101) for `count`
+ * INSTANCE shapes: worlds holds 16 floats per shape (row-vector convention,
+ * like s3dg_shape_desc.world).  The top-level tree is rebuilt over the
+ * unchanged mesh trees; renders after the call see the new matrices (a render
+ * equals that of a scene created with them).  Scene sets made earlier keep
+ * their copy. */
+int s3dg_scene_update_instances(s3dg_scene *scene, uint32_t count, const uint32_t *shapes, const float *worlds);
+
 /* Host-only: the depth-first permutation that BVH2/4/8::Create would apply to
  * the triangles of Mesh::Create(vertex_count, vertices, ...) (src/src/s3d_mesh.cpp:470-498),
  * and the number of tree nodes, without a device.  kind is S3DG_BVH2/4/8. */

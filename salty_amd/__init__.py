@@ -37,6 +37,7 @@ _SIGNATURES = {
     "s3dg_scene_destroy": [C.c_void_p],
     "s3dg_scene_get_stats": [C.c_void_p, C.POINTER(abi.SceneStats)],
     "s3dg_scene_mesh_order": [C.c_void_p, C.c_uint32, C.c_void_p],
+    "s3dg_scene_update_instances": [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p],
     "s3dg_build_order": [C.c_uint32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p],
     "s3dg_image_load_bmp": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_void_p)],
     "s3dg_image_load_hdr": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_void_p)],
@@ -287,6 +288,13 @@ class Scene:
         out = np.empty(n, dtype=np.int32)
         self.ctx.check(self.ctx.lib.s3dg_scene_mesh_order(self.handle, mesh, _ptr(out)), "mesh_order")
         return out
+
+    def update_instances(self, shapes, worlds):
+        """New world matrices (4x4 each, row-vector convention) for INSTANCE shapes."""
+        sh = np.ascontiguousarray(shapes, dtype=np.uint32)
+        w = np.ascontiguousarray(worlds, dtype=np.float32).reshape(len(sh), 16)
+        self.ctx.check(self.ctx.lib.s3dg_scene_update_instances(self.handle, len(sh), _ptr(sh), _ptr(w)),
+                       "s3dg_scene_update_instances")
 
     def trace(self, rays: np.ndarray, mode: int = abi.TRACE_CLOSEST, with_shapes: bool = False):
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)

@@ -467,7 +467,7 @@ public:
                 work.push_back({children_of(item.kids[by_slot[s]]), next++, item.depth + 1});
             }
         }
-        return root;
+        return out_.node_offset + root;
     }
 
 private:
@@ -608,7 +608,7 @@ private:
             g.exp[a] = (uint8_t)(e + 127);
             step[a] = std::ldexp(1.0, e);
         }
-        g.child_base = base | (top_ ? kTopLevelBit : 0u);
+        g.child_base = (out_.node_offset + base) | (top_ ? kTopLevelBit : 0u);
         g.leaf_base = tri_offset_ + (uint32_t)leaf_order_.size();
         for (int s = 0; s < 8; ++s) {
             for (int a = 0; a < 3; ++a) { g.qlo[a][s] = 255; g.qhi[a][s] = 0; }

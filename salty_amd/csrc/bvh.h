@@ -49,13 +49,15 @@ void build_traversal_tree(const std::vector<Aabb> &boxes, int max_leaf, RefTree 
 struct Compressed {
     std::vector<GNode> nodes;
     int32_t max_depth = 0;
+    uint32_t node_offset = 0;   // device index of nodes[0] (node indices written are offset by it)
 };
 
 // Appends the compressed form of `tree` (whose prims have the boxes
 // prim_boxes, in input order) to out.nodes.  The prims are given a new order
 // in which every node's leaves are contiguous: leaf_order[j] is the
 // depth-first position of the prim stored at tri_offset + j (triangles, or
-// shape records for top-level trees).  Returns the root node index.  `margin`
+// shape records for top-level trees).  Returns the root node index (offset by
+// out.node_offset, like every node index written).  `margin`
 // is the absolute amount by which every child box is grown before
 // quantisation so that rounding in the kernels' slab tests can never reject a
 // box containing an accepted hit.

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/s3dg.h"
+#include "bvh.h"
 #include "gpu_types.h"
 
 struct s3dg_context {
@@ -77,7 +78,7 @@ struct Workspace {
     void *d_batch = nullptr;        // batch descriptor and segment table of the render
     size_t batch_bytes = 0;
     hipGraphExec_t graph = nullptr; // captured bounce loop
-    uint64_t graph_key[8] = {};     // launch parameters the graph was captured with
+    uint64_t graph_key[10] = {};     // launch parameters the graph was captured with
 };
 
 struct SceneMeshInfo {
@@ -107,6 +108,16 @@ struct s3dg_scene {
         uint64_t texel_floats;                                // floats of the texel array
     } counts{};
     s3dg::Workspace ws;
+    // host copies an instance update rebuilds the top level from
+    struct {
+        std::vector<s3dg::GInst> inst;     // shape records in shape order
+        std::vector<int32_t> shape_mesh;   // mesh of every shape (-1: sphere)
+        std::vector<s3dg::Aabb> wbox;      // world box of every shape
+        std::vector<s3dg::Aabb> mesh_box;  // box of every mesh
+        std::vector<s3dg::GLight> lights;
+        uint32_t tlas_base = 0, tlas_cap = 0;   // first top-level node, nodes reserved for the top level
+        int32_t blas_depth = 0;
+    } host;
 };
 
 namespace s3dg {

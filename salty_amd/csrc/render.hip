@@ -725,7 +725,8 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     // read are rewritten before each replay).
     const bool use_graph = !timed && !count;
     if (use_graph) {
-        const uint64_t key[8] = {(uint64_t)(uintptr_t)ws.d_render,
+        const uint64_t key[10] = {(uint64_t)(uintptr_t)ws.d_gstack, (uint64_t)(uint32_t)gcap,
+                                  (uint64_t)(uintptr_t)ws.d_render,
                                  slots | ((uint64_t)(uint32_t)(P.single_scene + 1) << 32), (uint64_t)max_bounce,
                                  ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
                                  (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,

@@ -188,6 +188,65 @@ float scale_margin(const Aabb &b) {
     return 4e-6f * m + 1e-30f;
 }
 
+// Inverse matrix and world box of an instance of a mesh whose box is mb
+// (Instance ctor / UpdateMatrix, src/src/s3d_instance.cpp:22-32, 95-101):
+// the inverse is the reference's Matrix::Invert; the box is the region the
+// inverse actually used maps into mb, grown to cover rounding.  False when the
+// matrix is singular.
+bool instance_transform(const float world[16], const Aabb &mb, GInst &g, Aabb &wbox) {
+    float w[4][4];
+    memcpy(w, world, sizeof(w));
+    invert_ref(w, g.inv);
+    // a singular matrix divides by a zero determinant: NaN or inf entries
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c)
+            if (!std::isfinite(g.inv[r][c])) return false;
+    double back[4][4];
+    if (!inverse_double(g.inv, back)) return false;
+    for (int a = 0; a < 3; ++a) { wbox.lo[a] = 3.4e38f; wbox.hi[a] = -3.4e38f; }
+    for (int c = 0; c < 8; ++c) {
+        const double p[3] = {(c & 1) ? mb.hi[0] : mb.lo[0], (c & 2) ? mb.hi[1] : mb.lo[1],
+                             (c & 4) ? mb.hi[2] : mb.lo[2]};
+        double q[4];
+        for (int k = 0; k < 4; ++k) q[k] = p[0] * back[0][k] + p[1] * back[1][k] + p[2] * back[2][k] + back[3][k];
+        for (int a = 0; a < 3; ++a) {
+            const double v = q[a] / q[3];
+            wbox.lo[a] = std::min(wbox.lo[a], (float)(v - 1e-6 * fabs(v)));
+            wbox.hi[a] = std::max(wbox.hi[a], (float)(v + 1e-6 * fabs(v)));
+        }
+    }
+    // a small extra margin covers the rounding of the transformed ray
+    const float m = scale_margin(wbox) * 4.0f;
+    for (int a = 0; a < 3; ++a) { wbox.lo[a] -= m; wbox.hi[a] += m; }
+    return true;
+}
+
+// Top-level tree over the shapes' world boxes, appended to comp (whose
+// node_offset places it in the device node array).  Shape records come out in
+// the tree's leaf order: inst_dev[k] is the record of shape inst_dev[k].shape,
+// inst_pos[shape] its position.  Ties between shapes go by shape index,
+// whatever the tree.
+uint32_t build_tlas(const s3dg_context *ctx, const std::vector<GInst> &inst, const std::vector<Aabb> &wbox,
+                    Compressed &comp, std::vector<GInst> &inst_dev, std::vector<int32_t> &inst_pos,
+                    int32_t &depth) {
+    RefTree top;
+    if (ctx->leaf_size) build_traversal_tree(wbox, 1, top);
+    else build_reference_tree(wbox, 8, top);
+    const Aabb all = top.nodes[top.root].box;
+    comp.max_depth = 0;
+    std::vector<int32_t> top_order;
+    const uint32_t root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp, top_order);
+    inst_dev.resize(inst.size());
+    inst_pos.assign(inst.size(), 0);
+    for (size_t k = 0; k < inst.size(); ++k) {
+        const int32_t shape = top.order[top_order[k]];
+        inst_dev[k] = inst[shape];
+        inst_pos[shape] = (int32_t)k;
+    }
+    depth = comp.max_depth;
+    return root;
+}
+
 template <class T>
 int upload(s3dg_context *ctx, void **dst, const std::vector<T> &src, size_t &total) {
     const size_t bytes = std::max<size_t>(src.size() * sizeof(T), 256);
@@ -393,31 +452,10 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
                     return set_error(ctx, S3DG_ERR_INVALID, "instance material override out of range");
                 }
                 g.material = s.material >= 0 ? s.material : -1;
-                float w[4][4];
-                memcpy(w, s.world, sizeof(w));
-                invert_ref(w, g.inv);
-                // World region mapped into the mesh box by the inverse actually used.
-                double back[4][4];
-                if (!inverse_double(g.inv, back)) {
+                if (!instance_transform(s.world, mb, g, wbox[i])) {
                     delete sc;
                     return set_error(ctx, S3DG_ERR_INVALID, "singular instance matrix");
                 }
-                for (int a = 0; a < 3; ++a) { wbox[i].lo[a] = 3.4e38f; wbox[i].hi[a] = -3.4e38f; }
-                for (int c = 0; c < 8; ++c) {
-                    const double p[3] = {(c & 1) ? mb.hi[0] : mb.lo[0], (c & 2) ? mb.hi[1] : mb.lo[1],
-                                         (c & 4) ? mb.hi[2] : mb.lo[2]};
-                    double q[4];
-                    for (int k = 0; k < 4; ++k)
-                        q[k] = p[0] * back[0][k] + p[1] * back[1][k] + p[2] * back[2][k] + back[3][k];
-                    for (int a = 0; a < 3; ++a) {
-                        const double v = q[a] / q[3];
-                        wbox[i].lo[a] = std::min(wbox[i].lo[a], (float)(v - 1e-6 * fabs(v)));
-                        wbox[i].hi[a] = std::max(wbox[i].hi[a], (float)(v + 1e-6 * fabs(v)));
-                    }
-                }
-                // a small extra margin covers the rounding of the transformed ray
-                const float m = scale_margin(wbox[i]) * 4.0f;
-                for (int a = 0; a < 3; ++a) { wbox[i].lo[a] -= m; wbox[i].hi[a] += m; }
             }
         } else {
             delete sc;
@@ -437,21 +475,13 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         inst_dev = inst;
         inst_pos[0] = 0;
     } else {
-        // ties between shapes go by shape index, whatever the tree
-        RefTree top;
-        if (ctx->leaf_size) build_traversal_tree(wbox, 1, top);
-        else build_reference_tree(wbox, 8, top);
-        Aabb all = top.nodes[top.root].box;
-        comp.max_depth = 0;
-        std::vector<int32_t> top_order;
-        sc->rec.root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp, top_order);
-        inst_dev.resize(d->shape_count);
-        for (uint32_t k = 0; k < d->shape_count; ++k) {
-            const int32_t shape = top.order[top_order[k]];
-            inst_dev[k] = inst[shape];
-            inst_pos[shape] = (int32_t)k;
-        }
-        tlas_depth = comp.max_depth;
+        // the top-level tree follows the meshes' nodes, with room for every
+        // rebuild an instance update can make (at most one node per shape)
+        const uint32_t base = (uint32_t)comp.nodes.size();
+        sc->rec.root = build_tlas(ctx, inst, wbox, comp, inst_dev, inst_pos, tlas_depth);
+        sc->host.tlas_base = base;
+        sc->host.tlas_cap = std::max<uint32_t>((uint32_t)(comp.nodes.size() - base), d->shape_count + 1);
+        comp.nodes.resize(base + sc->host.tlas_cap);
         sc->rec.single_shape = -1;
         sc->stats.tlas_nodes = 0;
     }
@@ -595,6 +625,14 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     dv.texels = (const float *)sc->d_texels;
     dv.tri_uv = (const GTriUV *)sc->d_uv;
 
+    sc->host.inst = inst;
+    sc->host.shape_mesh.resize(d->shape_count);
+    for (uint32_t i = 0; i < d->shape_count; ++i) sc->host.shape_mesh[i] = d->shapes[i].mesh;
+    sc->host.wbox = wbox;
+    sc->host.mesh_box = mesh_box;
+    sc->host.lights = lights;
+    sc->host.blas_depth = blas_depth;
+
     sc->stats.triangles = tris.size();
     sc->stats.blas_nodes = comp.nodes.size();
     sc->stats.device_bytes = total;
@@ -650,5 +688,58 @@ extern "C" int s3dg_scene_mesh_order(const s3dg_scene *sc, uint32_t mesh, int32_
     if (!sc || !order || mesh >= sc->meshes.size()) return S3DG_ERR_INVALID;
     const auto &o = sc->meshes[mesh].order;
     std::copy(o.begin(), o.end(), order);
+    return S3DG_OK;
+}
+
+// Instance::UpdateMatrix (src/src/s3d_instance.cpp:95-101) for several
+// instances at once: new inverse matrices and world boxes, then the top-level
+// tree is rebuilt on the host over the unchanged mesh trees and uploaded into
+// the node range reserved for it, with the reordered shape records, shape
+// references and light records.
+extern "C" int s3dg_scene_update_instances(s3dg_scene *sc, uint32_t count, const uint32_t *shapes,
+                                           const float *worlds) {
+    if (!sc || (count && (!shapes || !worlds))) return S3DG_ERR_INVALID;
+    s3dg_context *ctx = sc->ctx;
+    if (sc->rec.single_shape >= 0) return set_error(ctx, S3DG_ERR_INVALID, "scene has no instances");
+    std::vector<GInst> inst = sc->host.inst;
+    std::vector<Aabb> wbox = sc->host.wbox;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t sh = shapes[k];
+        if (sh >= inst.size() || inst[sh].type != INST_TRANSFORMED)
+            return set_error(ctx, S3DG_ERR_INVALID, "shape is not an instance");
+        const int32_t mesh = sc->host.shape_mesh[sh];
+        if (!instance_transform(worlds + 16 * (size_t)k, sc->host.mesh_box[mesh], inst[sh], wbox[sh]))
+            return set_error(ctx, S3DG_ERR_INVALID, "singular instance matrix");
+    }
+    Compressed comp;
+    comp.node_offset = sc->host.tlas_base;
+    std::vector<GInst> inst_dev;
+    std::vector<int32_t> inst_pos;
+    int32_t tlas_depth = 0;
+    const uint32_t root = build_tlas(ctx, inst, wbox, comp, inst_dev, inst_pos, tlas_depth);
+    if (comp.nodes.size() > sc->host.tlas_cap || root != sc->rec.root)
+        return set_error(ctx, S3DG_ERR_INVALID, "top-level tree outgrew its reserved nodes");
+    std::vector<GShapeRef> refs(inst.size());
+    for (size_t k = 0; k < inst.size(); ++k) {
+        refs[k].inst = inst_pos[k];
+        refs[k].material = inst[k].material;
+    }
+    std::vector<GLight> lights = sc->host.lights;
+    for (GLight &l : lights)
+        if (l.type == LIGHT_SPHERE) l.tri[0] = inst_pos[l.shape];
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    // the context's stream may still be rendering with the old records
+    S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    S3DG_HIP(ctx, hipMemcpy((GNode *)sc->d_nodes + sc->host.tlas_base, comp.nodes.data(),
+                            comp.nodes.size() * sizeof(GNode), hipMemcpyHostToDevice));
+    S3DG_HIP(ctx, hipMemcpy(sc->d_insts, inst_dev.data(), inst_dev.size() * sizeof(GInst), hipMemcpyHostToDevice));
+    S3DG_HIP(ctx, hipMemcpy(sc->d_shape_ref, refs.data(), refs.size() * sizeof(GShapeRef), hipMemcpyHostToDevice));
+    if (!lights.empty())
+        S3DG_HIP(ctx, hipMemcpy(sc->d_lights, lights.data(), lights.size() * sizeof(GLight), hipMemcpyHostToDevice));
+    sc->host.inst = std::move(inst);
+    sc->host.wbox = std::move(wbox);
+    sc->host.lights = std::move(lights);
+    sc->dev.stack_need = std::max(sc->dev.stack_need, 2 * sc->host.blas_depth + 2 * tlas_depth + 4);
+    sc->stats.max_depth = sc->dev.stack_need;
     return S3DG_OK;
 }

@@ -282,6 +282,12 @@ struct Instance : IShape {
     IShape *shape = nullptr;
     Matrix world;
     IMaterial *override_material = nullptr;
+    u32 version = 0;   // bumped by UpdateMatrix; a built Scene catches up on its next use
+    // Instance::UpdateMatrix (s3d_instance.cpp:95-101)
+    void UpdateMatrix(const Matrix &m) {
+        world = m;
+        ++version;
+    }
     Instance() : IShape(INSTANCE) {}
     ~Instance() override {
         SafeRelease(shape);
@@ -376,7 +382,7 @@ public:
 
     // Uploads the scene (builds every mesh tree with the reference SAH on the host).
     bool Build(int device = 0) {
-        if (m_Scene) return true;
+        if (m_Scene) return SyncInstances();
         if (!m_Context && s3dg_create(&m_Context, device) != S3DG_OK) return false;
         std::vector<IMaterial *> mats;
         auto mat_index = [&](IMaterial *m) -> int32_t {
@@ -481,6 +487,9 @@ public:
         sd.texture_count = (uint32_t)tdesc.size();
         sd.textures = tdesc.data();
         sd.environment_texture = ibl;
+        m_Versions.assign(m_Shapes.size(), 0);
+        for (size_t i = 0; i < m_Shapes.size(); ++i)
+            if (m_Shapes[i]->kind == IShape::INSTANCE) m_Versions[i] = static_cast<Instance *>(m_Shapes[i])->version;
         return s3dg_scene_create(m_Context, &sd, &m_Scene) == S3DG_OK;
     }
 
@@ -503,6 +512,10 @@ public:
     s3dg_context *GetContext() { return m_Context; }
     s3dg_scene *GetDeviceScene() { return Build() ? m_Scene : nullptr; }
 
+    // Virtual like Scene::Update (s3d_scene.h:92): a subclass moves its
+    // instances here with Instance::UpdateMatrix.
+    virtual void Update(float) {}
+
 private:
     struct QuadLight {
         IShape *mesh;
@@ -516,6 +529,28 @@ private:
     const Texture *m_IBL = nullptr;
     s3dg_context *m_Context = nullptr;
     s3dg_scene *m_Scene = nullptr;
+
+    std::vector<u32> m_Versions;   // instance versions the device scene holds
+
+    // Sends the matrices of instances moved since the last use (one top-level
+    // rebuild for all of them).
+    bool SyncInstances() {
+        std::vector<uint32_t> ids;
+        std::vector<float> worlds;
+        for (size_t i = 0; i < m_Shapes.size(); ++i) {
+            if (m_Shapes[i]->kind != IShape::INSTANCE) continue;
+            const Instance *in = static_cast<const Instance *>(m_Shapes[i]);
+            if (in->version == m_Versions[i]) continue;
+            ids.push_back((uint32_t)i);
+            for (int r = 0; r < 4; ++r)
+                for (int c = 0; c < 4; ++c) worlds.push_back(in->world.m[r][c]);
+        }
+        if (ids.empty()) return true;
+        if (s3dg_scene_update_instances(m_Scene, (uint32_t)ids.size(), ids.data(), worlds.data()) != S3DG_OK)
+            return false;
+        for (uint32_t i : ids) m_Versions[i] = static_cast<const Instance *>(m_Shapes[i])->version;
+        return true;
+    }
 
     int32_t shape_index(IShape *s) const {
         for (size_t i = 0; i < m_Shapes.size(); ++i)
