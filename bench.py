@@ -60,6 +60,9 @@ def parse():
     p.add_argument("--passes-in-flight", type=int, default=0,
                    help="passes traced together per wavefront (0: the library's automatic choice)")
     p.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel settings")
+    p.add_argument("--materials", choices=("c2", "all"), default="c2",
+                   help="mesh workload materials: Lambert/mirror/glass, or all five (Phong and Plastic too)")
+    p.add_argument("--lens-radius", type=float, default=0.0, help="mesh workload: thin-lens camera when > 0")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-denoise", action="store_true",
                    help="skip the time-to-relMSE comparison (learned denoiser vs plain path tracing)")
@@ -429,7 +432,7 @@ def main():
     from salty_amd import abi, scenes
 
     t = time.perf_counter()
-    desc = scenes.mesh_scene(seed=args.seed)
+    desc = scenes.mesh_scene(seed=args.seed, materials=args.materials, lens_radius=args.lens_radius)
     gen_s = time.perf_counter() - t
     ctx = salty_amd.Context(0)
     t = time.perf_counter()
@@ -497,7 +500,9 @@ def main():
         "config": {
             "workload": f"procedural mesh scene, {sstats.triangles} triangles (displaced L7 icosphere + 64 "
                         f"instanced L3 icospheres + 640k-triangle ground), {W}x{H}, {args.spp} spp, depth "
-                        f"{args.depth}, wavefront path tracer with NEE",
+                        f"{args.depth}, wavefront path tracer with NEE"
+                        + ("" if args.materials == "c2" else ", all five materials (Phong and Plastic included)")
+                        + ("" if args.lens_radius <= 0 else f", thin-lens camera (lens radius {args.lens_radius})"),
             "paths_per_s": paths / elapsed,
             "extension_rays": rays_ext,
             "shadow_rays": rays_sh,

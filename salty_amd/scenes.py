@@ -20,7 +20,8 @@ import math
 
 import numpy as np
 
-from .abi import (BVH8, CAMERA_PINHOLE, F_HIT_MAX, LIGHT_QUAD, LIGHT_SHAPE, MAT_GLASS, MAT_LAMBERT, MAT_MIRROR,
+from .abi import (BVH8, CAMERA_PINHOLE, CAMERA_THINLENS, F_HIT_MAX, LIGHT_QUAD, LIGHT_SHAPE, MAT_GLASS,
+                  MAT_LAMBERT, MAT_MIRROR, MAT_PHONG, MAT_PLASTIC,
                   RAY_DTYPE, SHAPE_INSTANCE, SHAPE_MESH, SHAPE_SPHERE, VERTEX_DTYPE, Camera, Light, Mesh,
                   SceneDesc, Shape)
 
@@ -332,8 +333,21 @@ def _grid(x0, x1, z0, z1, nx, nz, y=0.0):
     return v, f, n
 
 
-def _random_material(rng) -> dict:
+def _random_material(rng, mix: str = "c2") -> dict:
     u = rng.random()
+    if mix == "all":
+        # every material of the reference: Lambert, Mirror, Glass, Phong, Plastic
+        if u < 0.40:
+            return dict(type=MAT_LAMBERT, diffuse=tuple(float(x) for x in rng.uniform(0.2, 0.9, 3)))
+        if u < 0.50:
+            return dict(type=MAT_MIRROR, specular=(0.95, 0.95, 0.95))
+        if u < 0.60:
+            return dict(type=MAT_GLASS, specular=(0.99, 0.99, 0.99), ior=1.5)
+        if u < 0.80:
+            return dict(type=MAT_PHONG, specular=tuple(float(x) for x in rng.uniform(0.3, 0.9, 3)),
+                        power=float(rng.uniform(5.0, 200.0)))
+        return dict(type=MAT_PLASTIC, diffuse=tuple(float(x) for x in rng.uniform(0.2, 0.9, 3)),
+                    specular=(0.8, 0.8, 0.8), power=float(rng.uniform(10.0, 500.0)))
     if u < 0.70:
         return dict(type=MAT_LAMBERT, diffuse=tuple(float(x) for x in rng.uniform(0.2, 0.9, 3)))
     if u < 0.85:
@@ -349,17 +363,22 @@ def translate(x, y, z) -> np.ndarray:
 
 
 def mesh_scene(seed: int = 0, big_level: int = 7, small_level: int = 3, ground=(800, 400),
-               instances: int = 64) -> SceneDesc:
+               instances: int = 64, materials: str = "c2", lens_radius: float = 0.0) -> SceneDesc:
     """The ~1.05 M-triangle scene: a level-7 icosphere (327,680 triangles) of
     radius 1 displaced by 3-octave value noise of amplitude 0.05; `instances`
     translated copies of a level-3 icosphere of radius 0.4 (1,280 triangles
     each) at seeded uniform positions on a 20 x 20 ground grid of 800 x 400
     quads (640,000 triangles); one spherical light.  Materials per object:
     70 % Lambert (albedo U[0.2,0.9]), 15 % mirror, 15 % glass (IOR 1.5); the
-    ground is Lambert 0.7."""
+    ground is Lambert 0.7.  materials="all" draws from all five of the
+    reference's materials instead (Phong and Plastic included, the ground
+    Plastic), and lens_radius > 0 makes the camera a ThinLensCamera focused
+    on the target."""
     rng = np.random.default_rng(seed)
     desc = SceneDesc()
     mats = [dict(type=MAT_LAMBERT, diffuse=(0.7, 0.7, 0.7))]
+    if materials == "all":
+        mats = [dict(type=MAT_PLASTIC, diffuse=(0.6, 0.6, 0.6), specular=(0.5, 0.5, 0.5), power=80.0)]
     # ground
     gv, gf, gn = _grid(-10.0, 10.0, -10.0, 10.0, ground[0], ground[1])
     desc.meshes.append(Mesh(_mesh_vertices(gv, gf, gn), material=0))
@@ -367,7 +386,7 @@ def mesh_scene(seed: int = 0, big_level: int = 7, small_level: int = 3, ground=(
     v, f = icosphere(big_level)
     disp = 0.05 * value_noise(v, seed, octaves=3, base_freq=2.0)
     vd = v * (1.0 + disp)[:, None] + np.array([0.0, 1.05, 0.0])
-    mats.append(_random_material(rng))
+    mats.append(_random_material(rng, materials))
     desc.meshes.append(Mesh(_mesh_vertices(vd, f, _smooth_normals(vd, f)), material=1))
     # small icosphere mesh shared by the instances
     sv, sf = icosphere(small_level)
@@ -380,7 +399,7 @@ def mesh_scene(seed: int = 0, big_level: int = 7, small_level: int = 3, ground=(
         x, z = rng.uniform(-9.5, 9.5, 2)
         if x * x + z * z < 1.6 ** 2:
             continue
-        mats.append(_random_material(rng))
+        mats.append(_random_material(rng, materials))
         desc.shapes.append(Shape(SHAPE_INSTANCE, mesh=2, material=len(mats) - 1,
                                  world=translate(float(x), r_small, float(z))))
         placed += 1
@@ -389,7 +408,8 @@ def mesh_scene(seed: int = 0, big_level: int = 7, small_level: int = 3, ground=(
     desc.lights = [Light(LIGHT_SHAPE, shape=len(desc.shapes) - 1)]
     desc.materials = mats
     desc.camera = Camera(position=(0.0, 3.5, 11.0), target=(0.0, 0.8, 0.0), upward=(0.0, 1.0, 0.0), fov=FOV_40,
-                         aspect=16.0 / 9.0, near_clip=1.0)
+                         aspect=16.0 / 9.0, near_clip=1.0,
+                         type=CAMERA_THINLENS if lens_radius > 0 else CAMERA_PINHOLE, lens_radius=lens_radius)
     desc.environment = (0.15, 0.18, 0.22)
     return desc
 
