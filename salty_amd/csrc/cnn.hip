@@ -302,16 +302,23 @@ __global__ void k_relu_back(bf16 *__restrict__ d, const bf16 *__restrict__ y, lo
 }
 
 // 2x2 average pooling, NHWC.
+// 2x2 average pooling; one thread per output pixel and group of 8 channels
+// (16-byte loads and stores), 32-bit indexing (the host checks the range).
 __global__ void k_pool2(const bf16 *__restrict__ x, int32_t N, int32_t H, int32_t W, int32_t C, bf16 *__restrict__ y) {
-    const int Ho = H / 2, Wo = W / 2;
-    const long long n_out = (long long)N * Ho * Wo * C;
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n_out; i += (long long)gridDim.x * 256) {
-        const int c = (int)(i % C);
-        const long long p = i / C;
-        const int xo = (int)(p % Wo), yo = (int)((p / Wo) % Ho), n = (int)(p / ((long long)Wo * Ho));
-        const size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + c;
-        const float s = b2f(x[b]) + b2f(x[b + C]) + b2f(x[b + (size_t)W * C]) + b2f(x[b + (size_t)W * C + C]);
-        y[i] = f2b(0.25f * s);
+    const uint32_t Ho = (uint32_t)H / 2, Wo = (uint32_t)W / 2, G = (uint32_t)C / 8;
+    const uint32_t n_out = (uint32_t)N * Ho * Wo * G;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n_out; i += gridDim.x * 256u) {
+        const uint32_t g = i % G, p = i / G;
+        const uint32_t xo = p % Wo, yo = (p / Wo) % Ho, n = p / (Wo * Ho);
+        const size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + 8 * g;
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(x + b);
+        const bf16x8 bb = *reinterpret_cast<const bf16x8 *>(x + b + C);
+        const bf16x8 c = *reinterpret_cast<const bf16x8 *>(x + b + (size_t)W * C);
+        const bf16x8 d = *reinterpret_cast<const bf16x8 *>(x + b + (size_t)W * C + C);
+        bf16x8 o;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) o[k] = f2b(0.25f * (b2f(a[k]) + b2f(bb[k]) + b2f(c[k]) + b2f(d[k])));
+        *reinterpret_cast<bf16x8 *>(y + (size_t)p * C + 8 * g) = o;
     }
 }
 // dX += upsample(dY) / 4
@@ -359,15 +366,21 @@ __global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H
 
 // Input features -> padded bf16: log1p radiance, albedo, normal, log1p depth.
 __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *__restrict__ x) {
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < npix * kInPadded; i += (long long)gridDim.x * 256) {
-        const long long p = i / kInPadded;
-        const int c = (int)(i % kInPadded);
-        float v = 0.0f;
-        if (c < kInChannels) {
-            v = f[p * kInChannels + c];
-            if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
+    // one thread per pixel: its 10 features in, 32 bf16 out as four 16-byte stores
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
+        bf16x8 o[kInPadded / 8];
+#pragma unroll
+        for (int c = 0; c < kInPadded; ++c) {
+            float v = 0.0f;
+            if (c < kInChannels) {
+                v = f[p * kInChannels + c];
+                if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
+            }
+            o[c / 8][c % 8] = f2b(v);
         }
-        x[i] = f2b(v);
+        bf16x8 *dst = reinterpret_cast<bf16x8 *>(x + p * kInPadded);
+#pragma unroll
+        for (int q = 0; q < kInPadded / 8; ++q) dst[q] = o[q];
     }
 }
 
@@ -405,11 +418,6 @@ __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long l
             o[k] = s;
         }
         const bool miss = feat[p * kInChannels + 9] == 0.0f;
-        if (!target) {
-            for (int k = 0; k < kOut; ++k)
-                out[p * kOut + k] = miss ? feat[p * kInChannels + k] : fmaxf(expm1f(o[k]), 0.0f);
-            continue;
-        }
         if (miss) {
             for (int k = 0; k < kOut; ++k)
                 if (out) out[p * kOut + k] = feat[p * kInChannels + k];
@@ -438,7 +446,6 @@ __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long l
         }
         for (int k = 0; k < kOut; ++k) gw[k * 33 + 32] += g[k];
     }
-    if (!target) return;
     for (int k = 0; k < kOut * 33; ++k) sg[threadIdx.x][k] = gw[k];
     sl[threadIdx.x] = l;
     __syncthreads();
@@ -453,6 +460,39 @@ __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long l
         double s = 0.0;
         for (int t = 0; t < 256; ++t) s += sl[t];
         atomicAdd(loss, s);
+    }
+}
+
+// Inference head: the 1x1 convolution to radiance (the same sums as k_head),
+// expm1 back from log space, misses passed through; one thread per pixel with
+// 16-byte loads of its 32 channels.
+__global__ __launch_bounds__(256) void k_head_infer(const bf16 *__restrict__ x, long long npix,
+                                                    const float *__restrict__ wh, const float *__restrict__ bh,
+                                                    const float *__restrict__ feat, float *__restrict__ out) {
+    __shared__ float sw[kOut * 32 + kOut];
+    for (int i = threadIdx.x; i < kOut * 32; i += 256) sw[i] = wh[i];
+    if (threadIdx.x < kOut) sw[kOut * 32 + threadIdx.x] = bh[threadIdx.x];
+    __syncthreads();
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
+        if (feat[p * kInChannels + 9] == 0.0f) {
+            for (int k = 0; k < kOut; ++k) out[p * kOut + k] = feat[p * kInChannels + k];
+            continue;
+        }
+        const bf16x8 *src = reinterpret_cast<const bf16x8 *>(x + p * 32);
+        float xi[32];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bf16x8 v = src[q];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) xi[8 * q + j] = b2f(v[j]);
+        }
+#pragma unroll
+        for (int k = 0; k < kOut; ++k) {
+            float sum = sw[kOut * 32 + k];
+#pragma unroll
+            for (int c = 0; c < 32; ++c) sum += sw[k * 32 + c] * xi[c];
+            out[p * kOut + k] = fmaxf(expm1f(sum), 0.0f);
+        }
     }
 }
 
@@ -618,18 +658,18 @@ void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
     hipStream_t s = c->ctx->stream;
     const long long n0 = (long long)N * H * W;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-    hipLaunchKernelGGL(k_pack_input, dim3(grid1(n0 * kInPadded)), dim3(256), 0, s, feat, n0, a.x);
+    hipLaunchKernelGGL(k_pack_input, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
     ConvLayer *L = c->layer;
     auto B = [&](int l) { return c->p + L[l].b_off; };
     conv(c, cin1(a.x, kInPadded), N, H, W, L[L_E0A].wf, B(L_E0A), 32, true, a.e0a);
     conv(c, cin1(a.e0a, 32), N, H, W, L[L_E0B].wf, B(L_E0B), 32, true, a.e0b);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H1 * W1 * 32)), dim3(256), 0, s, a.e0b, N, H, W, 32, a.p0);
+    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H1 * W1 * 4)), dim3(256), 0, s, a.e0b, N, H, W, 32, a.p0);
     conv(c, cin1(a.p0, 32), N, H1, W1, L[L_E1A].wf, B(L_E1A), 64, true, a.e1a);
     conv(c, cin1(a.e1a, 64), N, H1, W1, L[L_E1B].wf, B(L_E1B), 64, true, a.e1b);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H2 * W2 * 64)), dim3(256), 0, s, a.e1b, N, H1, W1, 64, a.p1);
+    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H2 * W2 * 8)), dim3(256), 0, s, a.e1b, N, H1, W1, 64, a.p1);
     conv(c, cin1(a.p1, 64), N, H2, W2, L[L_E2A].wf, B(L_E2A), 96, true, a.e2a);
     conv(c, cin1(a.e2a, 96), N, H2, W2, L[L_E2B].wf, B(L_E2B), 96, true, a.e2b);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H3 * W3 * 96)), dim3(256), 0, s, a.e2b, N, H2, W2, 96, a.p2);
+    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H3 * W3 * 12)), dim3(256), 0, s, a.e2b, N, H2, W2, 96, a.p2);
     conv(c, cin1(a.p2, 96), N, H3, W3, L[L_B0].wf, B(L_B0), 128, true, a.b0);
     conv(c, cin1(a.b0, 128), N, H3, W3, L[L_B1].wf, B(L_B1), 128, true, a.b1);
     conv(c, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, L[L_D2A].wf, B(L_D2A), 96, true, a.d2a);
@@ -833,10 +873,8 @@ extern "C" int s3dg_cnn_infer_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t 
     rc = prepare(c, n0, a);
     if (rc) return rc;
     forward(c, a, d_feat, n, h, w);
-    hipLaunchKernelGGL(k_head, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 2048)), dim3(256), 0,
-                       c->ctx->stream, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, d_feat, nullptr, d_out, nullptr,
-                       nullptr,
-                       nullptr, nullptr, 0.0f);
+    hipLaunchKernelGGL(k_head_infer, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 8192)), dim3(256), 0,
+                       c->ctx->stream, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, d_feat, d_out);
     S3DG_HIP(c->ctx, hipGetLastError());
     return S3DG_OK;
 }
