@@ -42,7 +42,7 @@ __device__ __forceinline__ bf16 f2b(float x) { return (bf16)x; }
 struct ConvIn {
     const bf16 *in1;
     const bf16 *in2;
-    int32_t C1, C2, s1;
+    int32_t C1, C2, s1;   // s1 is 1 or 2
 };
 
 // 8 consecutive channels c..c+7 (all in the same source) at pixel (n, y, x);
@@ -54,8 +54,10 @@ __device__ __forceinline__ bf16x8 load8(const ConvIn &I, int n, int y, int x, in
         return v;
     }
     if (c < I.C1) {
-        const int h1 = (H + I.s1 - 1) / I.s1, w1 = (W + I.s1 - 1) / I.s1;
-        const size_t off = (((size_t)n * h1 + y / I.s1) * w1 + x / I.s1) * I.C1 + c;
+        // upsampled input: coordinates halved when s1 = 2 (y, x >= 0 here)
+        const int sh = I.s1 >> 1;
+        const int h1 = (H + sh) >> sh, w1 = (W + sh) >> sh;
+        const size_t off = (((size_t)n * h1 + (y >> sh)) * w1 + (x >> sh)) * I.C1 + c;
         return *reinterpret_cast<const bf16x8 *>(I.in1 + off);
     }
     const size_t off = (((size_t)n * H + y) * W + x) * I.C2 + (c - I.C1);
@@ -113,18 +115,40 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
     for (int j = 0; j < 8; ++j) zero[j] = f2b(0.0f);
     const int koff = 8 * (lane >> 5);
     const int p = lane & 31;
+    // staging: every load of a slice is issued before the first LDS store, so
+    // that the slice costs one memory round trip
+    constexpr int kHalfCh = kHalo * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
+    constexpr int kWCh = 9 * COT * 4, kWPer = (kWCh + 255) / 256;
     for (int c0 = 0; c0 < Cin; c0 += 32) {
-        for (int e = threadIdx.x; e < kHalo * kHalo * 4; e += 256) {
-            const int pix = e >> 2, q = e & 3;
-            const int y = y0 + pix / kHalo - 1, x = x0 + pix % kHalo - 1;
-            *reinterpret_cast<bf16x8 *>(&s_in[pix * kRow + q * 8]) = load8(I, n, y, x, H, W, c0 + q * 8);
+        bf16x8 hv[kHalfPer], wv[kWPer];
+#pragma unroll
+        for (int k = 0; k < kHalfPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kHalfCh) {
+                const int pix = e >> 2, q = e & 3;
+                const int y = y0 + pix / kHalo - 1, x = x0 + pix % kHalo - 1;
+                hv[k] = load8(I, n, y, x, H, W, c0 + q * 8);
+            }
         }
-        for (int e = threadIdx.x; e < 9 * COT * 4; e += 256) {
-            const int row = e >> 2, q = e & 3;
-            const int tap = row / COT, co = co0 + row % COT;
-            bf16x8 v = zero;
-            if (co < Cout) v = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + co) * Cin + c0 + q * 8);
-            *reinterpret_cast<bf16x8 *>(&s_w[row * kRow + q * 8]) = v;
+#pragma unroll
+        for (int k = 0; k < kWPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kWCh) {
+                const int row = e >> 2, q = e & 3;
+                const int tap = row / COT, co = co0 + row % COT;
+                wv[k] = zero;
+                if (co < Cout) wv[k] = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + co) * Cin + c0 + q * 8);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kHalfPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kHalfCh) *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = hv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < kWPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kWCh) *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = wv[k];
         }
         __syncthreads();
 #pragma unroll 1
@@ -150,24 +174,36 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
         }
         __syncthreads();
     }
-    // D layout: row (pixel of the 32-pixel tile) = 8*(i/4) + 4*(lane/32) + i%4,
-    // column (channel) = lane % 32
+    // Epilogue through LDS: bias and ReLU applied, the tile is transposed into
+    // [pixel][channel] rows in the weight buffer (free after the last slice),
+    // then written out 8 channels (16 bytes) per store.  D layout of a block:
+    // row (pixel of the 32-pixel tile) = 8*(i/4) + 4*(lane/32) + i%4, column
+    // (channel) = lane % 32.
+    constexpr int kOutRow = COT + 8;
+    static_assert(kTile * kTile * kOutRow <= 9 * COT * kRow, "output tile fits the weight buffer");
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int co = co0 + 32 * t + p;
-        if (co >= Cout) continue;
-        const float bv = bias ? bias[co] : 0.0f;
+        const float bv = (bias && co < Cout) ? bias[co] : 0.0f;
 #pragma unroll
         for (int m = 0; m < 2; ++m)
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int row = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-                const int y = y0 + wave * 4 + m * 2 + (row >> 4), x = x0 + (row & 15);
-                if (y >= H || x >= W) continue;
+                const int pix = (wave * 4 + m * 2 + (row >> 4)) * kTile + (row & 15);
                 float v = acc[m][t][i] + bv;
                 if (relu) v = fmaxf(v, 0.0f);
-                out[(((size_t)n * H + y) * W + x) * Cout + co] = f2b(v);
+                s_w[pix * kOutRow + 32 * t + p] = f2b(v);
             }
+    }
+    __syncthreads();
+    constexpr int kChunks = kTile * kTile * (COT / 8);
+    for (int e = threadIdx.x; e < kChunks; e += 256) {
+        const int pix = e / (COT / 8), q = e % (COT / 8);
+        const int y = y0 + pix / kTile, x = x0 + pix % kTile, co = co0 + 8 * q;
+        if (y >= H || x >= W || co >= Cout) continue;
+        *reinterpret_cast<bf16x8 *>(out + (((size_t)n * H + y) * W + x) * Cout + co) =
+            *reinterpret_cast<const bf16x8 *>(&s_w[pix * kOutRow + 8 * q]);
     }
 }
 
