@@ -90,7 +90,7 @@ constexpr int kHalo = kTile + 2;
 constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 padding
 
 template <int COT>
-__global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
+__global__ __launch_bounds__(256, COT == 32 ? 3 : 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                      const bf16 *__restrict__ wt, const float *__restrict__ bias,
                                                      int32_t Cout, int32_t relu, bf16 *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) bf16 s_in[kHalo * kHalo * kRow];
@@ -103,13 +103,19 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
     const int n = blockIdx.y / co_blocks;
     const int co0 = (blockIdx.y % co_blocks) * COT;
     const int Cin = I.C1 + I.C2;
-    f32x16 acc[2][NT];
+    // KS accumulators per block: with one B tile (COT = 32) the two K halves
+    // of a slice go to separate accumulators, so that four independent MFMA
+    // chains keep the matrix pipeline busy; they are summed in the epilogue
+    constexpr int KS = NT == 1 ? 2 : 1;
+    f32x16 acc[2][NT][KS];
 #pragma unroll
     for (int m = 0; m < 2; ++m)
 #pragma unroll
         for (int t = 0; t < NT; ++t)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) acc[m][t][i] = 0.0f;
+            for (int k = 0; k < KS; ++k)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[m][t][k][i] = 0.0f;
     bf16x8 zero;
 #pragma unroll
     for (int j = 0; j < 8; ++j) zero[j] = f2b(0.0f);
@@ -169,7 +175,8 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                 for (int m = 0; m < 2; ++m)
 #pragma unroll
                     for (int t = 0; t < NT; ++t)
-                        acc[m][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[t], acc[m][t], 0, 0, 0);
+                        acc[m][t][kk / 16 % KS] =
+                            __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[t], acc[m][t][kk / 16 % KS], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -191,7 +198,9 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
             for (int i = 0; i < 16; ++i) {
                 const int row = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
                 const int pix = (wave * 4 + m * 2 + (row >> 4)) * kTile + (row & 15);
-                float v = acc[m][t][i] + bv;
+                float v = acc[m][t][0][i];
+                if (KS == 2) v += acc[m][t][KS - 1][i];
+                v += bv;
                 if (relu) v = fmaxf(v, 0.0f);
                 s_w[pix * kOutRow + 32 * t + p] = f2b(v);
             }
