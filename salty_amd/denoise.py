@@ -50,6 +50,15 @@ class FrameRenderer:
         img = self.ctx.download(self.d_acc, (self.h, self.w, 3), np.float32) / np.float32(spp)
         return img, st.milliseconds
 
+    def render_ms(self, spp, first_pass=0):
+        """Device milliseconds of one render of spp passes (the second of two,
+        so that the first has set up the launch graph)."""
+        ms = 0.0
+        for _ in range(2):
+            self.ctx.copy_to_device(self.d_acc, self.zero)
+            ms = self.scene.render_device(self.cfg(spp, first_pass), self.d_acc).milliseconds
+        return ms
+
     def ladder(self, levels, first_pass=0):
         """Means after each of the increasing sample counts `levels` of one
         progressive render (passes first_pass, first_pass + 1, ...):
@@ -208,7 +217,9 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65
                    plain_max=32768, den_levels=INPUT_LEVELS, nlm=True):
     """Device time for plain path tracing to reach relMSE <= target against
     a gt_spp render (passes disjoint from the others), and for path tracing +
-    guides + U-Net inference.  Times are interpolated between the measured
+    guides + U-Net inference.  Images come from one progressive render; the
+    time of a sample count is that of one render of that many samples (timed
+    after an untimed one), and times are interpolated between the measured
     sample counts (relMSE of plain path tracing falls as 1/spp)."""
     fr = FrameRenderer(ctx, desc, width, height, max_depth, seed=seed)
     gt, gt_ms = fr.mean(gt_spp, first_pass=1 << 24)
@@ -219,16 +230,17 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65
     while spp <= plain_max:
         levels.append(spp)
         spp *= 2
-    for spp, img, ms in fr.ladder(levels):
+    for spp, img, _ in fr.ladder(levels):
         r = rel_mse(img, gt)
-        plain.append((spp, ms, r))
+        plain.append((spp, fr.render_ms(spp), r))
         if r < target / 2:
             break
     plain_ms = crossing([(ms, r) for _, ms, r in plain], target)
     d_f = ctx.malloc(width * height * 10 * 4)
     d_o = ctx.malloc(width * height * 3 * 4)
     den = []
-    for spp, img, ms in fr.ladder(den_levels):
+    for spp, img, _ in fr.ladder(den_levels):
+        ms = fr.render_ms(spp)
         ctx.copy_to_device(d_f, features(img, a, n, d)[None])
         ctx.synchronize()
         t = time.perf_counter()
