@@ -224,6 +224,15 @@ __device__ __forceinline__ uint32_t slot_first(uint32_t leaf_base, uint32_t lo, 
     return __builtin_amdgcn_sad_u8(mlo, 0u, __builtin_amdgcn_sad_u8(mhi, 0u, leaf_base));
 }
 
+// Bit b (b < 4) set when byte b of w is non-zero: bit 7 of each byte of t
+// is set iff the byte is non-zero, and the multiply gathers bits 7, 15, 23,
+// 31 into bits 28..31 (the other partial products land on distinct lower
+// bits, so no carry reaches them).
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
+    const uint32_t t = ((w & 0x7f7f7f7fu) + 0x7f7f7f7fu) | w;
+    return ((t & 0x80808080u) * 0x00204081u) >> 28;
+}
+
 __device__ __forceinline__ uint32_t slot_count(uint32_t lo, uint32_t hi, uint32_t s) {
     return ((s < 4u ? lo : hi) >> ((s & 3u) * 8u)) & 0xffu;
 }
@@ -435,9 +444,7 @@ struct Tracer {
             if (tn <= tf) hits |= 1u << s;
         }
         // valid slots: internal children or leaves with a non-zero count
-        uint32_t valid = imask;
-#pragma unroll
-        for (int s = 0; s < 8; ++s) valid |= (slot_count(cnt_lo, cnt_hi, (uint32_t)s) != 0u ? 1u : 0u) << s;
+        const uint32_t valid = imask | nonzero_bytes(cnt_lo) | (nonzero_bytes(cnt_hi) << 4);
         hits &= valid;
         const uint32_t leaves = hits & ~imask;
         if (leaves) {
