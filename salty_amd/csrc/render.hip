@@ -299,12 +299,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
         if (active) {
             p = q[i];
             V3 W = v3(P.wr[p], P.wg[p], P.wb[p]);
-            V3 L = v3(P.lr[p], P.lg[p], P.lb[p]);
+            V3 Ladd;   // what this vertex adds to the path's radiance L
             const int32_t prim = P.hprim[p];
             if (prim == -1) {
                 // miss: L += W * SampleIBL(dir) (s3d_pt.cpp:199-203), or the
                 // constant environment when the scene has no IBL texture
-                L = L + W * environment(S, S.recs[scene_of(P, p)], v3(P.dx[p], P.dy[p], P.dz[p]));
+                Ladd = W * environment(S, S.recs[scene_of(P, p)], v3(P.dx[p], P.dy[p], P.dz[p]));
             } else {
                 const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
                 const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                     if (mat < 0) mat = sh.material;
                 }
                 const GMat &m = S.mats[mat];
-                L = L + W * ld3(m.emissive);
+                Ladd = W * ld3(m.emissive);
                 // TexturedMaterial: Shade and GetBaseColor times the texture
                 // at the hit's texcoord (1 when untextured, an exact no-op)
                 V3 tcol = v3(1.0f, 1.0f, 1.0f);
@@ -412,7 +412,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
                 }
                 P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
             }
-            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+            // L += Ladd, skipped when Ladd is zero in every channel: L starts
+            // at +0 and a sum is -0 only when both terms are, so adding a zero
+            // leaves L's bits as they are (a NaN compares unequal and is added)
+            if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
+                P.lr[p] = P.lr[p] + Ladd.x;
+                P.lg[p] = P.lg[p] + Ladd.y;
+                P.lb[p] = P.lb[p] + Ladd.z;
+            }
         }
         stage.flush_if_full(&P.cnt[2], P.qs, &P.cnt[cur ^ 1], P.q[cur ^ 1], false);
         stage.add(shadow, cont, p);
