@@ -485,6 +485,13 @@ struct Tracer {
         if (!have_node && !tris) {
             if (pop(S, st)) return true;
             tris = tri_cur < tri_end;
+        } else if (!have_node && st.sp > 0) {
+            // a lane with triangles pending also visits the next node, if it
+            // is a group entry (one pushed in the current ray's space),
+            // instead of idling through the visit; results do not depend on
+            // the order of tests
+            const uint2 e = st.top();
+            if ((e.y >> 30) == ENT_GROUP) pop(S, st);
         }
         NodeLines nl;
         if (have_node) nl = fetch(S);
