@@ -485,13 +485,6 @@ struct Tracer {
         if (!have_node && !tris) {
             if (pop(S, st)) return true;
             tris = tri_cur < tri_end;
-        } else if (!have_node && st.sp > 0) {
-            // a lane with triangles pending also visits the next node, if it
-            // is a group entry (one pushed in the current ray's space),
-            // instead of idling through the visit; results do not depend on
-            // the order of tests
-            const uint2 e = st.top();
-            if ((e.y >> 30) == ENT_GROUP) pop(S, st);
         }
         NodeLines nl;
         if (have_node) nl = fetch(S);
@@ -552,10 +545,12 @@ __device__ __forceinline__ DHit traverse(const SceneDev &S, const SceneRec &rec,
 }
 
 // Persistent-kernel work distribution with lane refill.  Each wave owns a
-// chunk of queue entries [cb, ce) taken from the global cursor `chunk` at a
-// time; whenever at least `refill` lanes are idle they get the next entries
-// of the chunk (the chunk is renewed with one atomic when it runs short), so
-// the cursor sees one atomic per chunk instead of one per refill.
+// chunk of queue entries [cb, ce): its first chunk is fixed by its index in
+// the grid, later ones are taken from the global cursor `chunk` at a time
+// (past the first chunks of all waves); whenever at least `refill` lanes are
+// idle they get the next entries of the chunk (the chunk is renewed with one
+// atomic when it runs short), so the cursor sees one atomic per chunk instead
+// of one per refill, and a launch with little work sees none at all.
 // `begin(i)` sets lane state up for queue entry i and returns false when the
 // entry needs no traversal; `finish()` consumes the result.  The wave leaves
 // once the cursor has passed n, its chunk is used up and no lane is active.
@@ -568,8 +563,15 @@ template <class Begin, class Step, class Finish>
 __device__ __forceinline__ void refill_loop(uint32_t *cursor, uint32_t n, FetchParams fp, Begin begin, Step step,
                                             Finish finish) {
     bool active = false;
-    bool exhausted = false;   // the global cursor has passed n
-    uint32_t cb = 0, ce = 0;  // this wave's chunk (entries < n only)
+    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // first chunks: wave w owns [w * chunk, (w + 1) * chunk); the cursor hands
+    // out entries from waves * chunk on (64-bit: the product can pass 2^32)
+    const uint64_t first_end = (uint64_t)waves * fp.chunk;
+    const uint32_t dyn0 = first_end < n ? (uint32_t)first_end : n;
+    uint32_t cb = (uint64_t)wave * fp.chunk < n ? wave * fp.chunk : n;   // this wave's chunk (entries < n only)
+    uint32_t ce = n - cb < fp.chunk ? n : cb + fp.chunk;
+    bool exhausted = dyn0 >= n;   // the global cursor has passed n
     const uint64_t below = (1ull << __lane_id()) - 1ull;
     for (;;) {
         const uint64_t idle = __ballot(!active);
@@ -581,9 +583,11 @@ __device__ __forceinline__ void refill_loop(uint32_t *cursor, uint32_t n, FetchP
             if (avail < want && !exhausted) {
                 uint32_t base = 0;
                 if (__lane_id() == 0) base = atomicAdd(cursor, fp.chunk);
-                nb = __shfl(base, 0);
-                ne = nb < n ? (n - nb < fp.chunk ? n : nb + fp.chunk) : nb;
-                exhausted = nb + fp.chunk >= n;
+                // 64-bit: the cursor keeps counting past n by up to a chunk per wave
+                const uint64_t nb64 = (uint64_t)dyn0 + __shfl(base, 0);
+                nb = nb64 < n ? (uint32_t)nb64 : n;
+                ne = n - nb < fp.chunk ? n : nb + fp.chunk;
+                exhausted = nb64 + fp.chunk >= n;
             }
             if (!active) {
                 const uint32_t r = (uint32_t)__popcll(idle & below);
