@@ -24,7 +24,7 @@ def main():
     ap.add_argument("--train-scenes", type=int, default=1024)
     ap.add_argument("--held-out", type=int, default=4)
     ap.add_argument("--spp-ref", type=int, default=1024)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--crop", type=int, default=0, help="random crop side (0: whole frames)")
     ap.add_argument("--lr", type=float, default=1e-3)
