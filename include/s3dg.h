@@ -206,9 +206,13 @@ typedef struct s3dg_render_config {
 } s3dg_render_config;
 
 /* flags: time every traversal launch with HIP events on the context stream;
- * count compressed nodes visited and triangles tested by the traversals. */
+ * count compressed nodes visited and triangles tested by the traversals;
+ * always use the per-bounce wavefront pipeline (otherwise a render whose
+ * paths fit the device's resident threads traces each path in one thread,
+ * in one launch, with the same result).  The first two imply the third. */
 #define S3DG_RENDER_TIME_KERNELS   1u
 #define S3DG_RENDER_COUNT_WORK     2u
+#define S3DG_RENDER_WAVEFRONT      4u
 
 typedef struct s3dg_render_stats {
     uint64_t extension_rays;   /* closest-hit rays traced                      */

@@ -101,6 +101,7 @@ class FrameJob(C.Structure):
 
 RENDER_TIME_KERNELS = 1
 RENDER_COUNT_WORK = 2
+RENDER_WAVEFRONT = 4
 
 
 class RenderStats(C.Structure):

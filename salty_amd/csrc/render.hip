@@ -206,6 +206,29 @@ __device__ __forceinline__ V3 environment(const SceneDev &S, const SceneRec &rec
     return tex_sample_dir(S.texels, S.textures[rec.env_tex], dir) * 10.0f;
 }
 
+// Camera ray, throughput, radiance and RNG of path slot p.
+__device__ __forceinline__ void raygen_path(const SceneDev &S, const PathBuf &P, uint32_t p, uint32_t nseg,
+                                            const Seg *__restrict__ segs) {
+    const Seg &g = find_seg<false>(segs, nseg, p);
+    const uint32_t local = p - g.path0;
+    const uint32_t pix = slot_pixel(local % g.npix, (uint32_t)g.width, (uint32_t)g.height);
+    const uint32_t pass = g.pass0 + local / g.npix;
+    const uint32_t x = pix % (uint32_t)g.width, y = pix / (uint32_t)g.width;
+    Pcg rng;
+    rng.seed(path_seed(g.seed, pix, pass, 0u));
+    // TracePath pixel-centre coordinates (s3d_pt.cpp:326-328)
+    const float fx = (0.5f + (float)x) / (float)g.width - 0.5f;
+    const float fy = (0.5f + (float)y) / (float)g.height - 0.5f;
+    V3 o, d;
+    camera_ray(S.recs[g.scene].cam, fx, fy, rng, o, d);
+    if (P.single_scene < 0) P.scene[p] = g.scene;
+    P.ox[p] = o.x; P.oy[p] = o.y; P.oz[p] = o.z;
+    P.dx[p] = d.x; P.dy[p] = d.y; P.dz[p] = d.z;
+    P.wr[p] = 1.0f; P.wg[p] = 1.0f; P.wb[p] = 1.0f;
+    P.lr[p] = 0.0f; P.lg[p] = 0.0f; P.lb[p] = 0.0f;
+    P.rng[p] = rng.state;
+}
+
 __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const BatchDesc *__restrict__ bd,
                                                     const Seg *__restrict__ segs) {
     const uint32_t npaths = bd->paths, nseg = bd->nseg;
@@ -218,24 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const 
         P.cnt[5] = 0;
     }
     for (uint32_t p = gtid; p < npaths; p += gridDim.x * blockDim.x) {
-        const Seg &g = find_seg<false>(segs, nseg, p);
-        const uint32_t local = p - g.path0;
-        const uint32_t pix = slot_pixel(local % g.npix, (uint32_t)g.width, (uint32_t)g.height);
-        const uint32_t pass = g.pass0 + local / g.npix;
-        const uint32_t x = pix % (uint32_t)g.width, y = pix / (uint32_t)g.width;
-        Pcg rng;
-        rng.seed(path_seed(g.seed, pix, pass, 0u));
-        // TracePath pixel-centre coordinates (s3d_pt.cpp:326-328)
-        const float fx = (0.5f + (float)x) / (float)g.width - 0.5f;
-        const float fy = (0.5f + (float)y) / (float)g.height - 0.5f;
-        V3 o, d;
-        camera_ray(S.recs[g.scene].cam, fx, fy, rng, o, d);
-        if (P.single_scene < 0) P.scene[p] = g.scene;
-        P.ox[p] = o.x; P.oy[p] = o.y; P.oz[p] = o.z;
-        P.dx[p] = d.x; P.dy[p] = d.y; P.dz[p] = d.z;
-        P.wr[p] = 1.0f; P.wg[p] = 1.0f; P.wb[p] = 1.0f;
-        P.lr[p] = 0.0f; P.lg[p] = 0.0f; P.lb[p] = 0.0f;
-        P.rng[p] = rng.state;
+        raygen_path(S, P, p, nseg, segs);
         P.q[0][p] = p;
     }
 }
@@ -281,6 +287,136 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
     tr.flush(P.tot + 2);
 }
 
+// One path vertex (Radiance, s3d_pt.cpp:182-246, with the sampling half of
+// NextEventEstimation): emission or environment, the light sample and its NEE
+// term (kept for the shadow ray), the material sample and Russian roulette.
+// shadow: a shadow ray is queued; cont: the path goes on.
+__device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, uint32_t p, int32_t depth,
+                                           int32_t rr_start, bool &shadow, bool &cont) {
+    V3 W = v3(P.wr[p], P.wg[p], P.wb[p]);
+    V3 Ladd;   // what this vertex adds to the path's radiance L
+    const int32_t prim = P.hprim[p];
+    if (prim == -1) {
+        // miss: L += W * SampleIBL(dir) (s3d_pt.cpp:199-203), or the
+        // constant environment when the scene has no IBL texture
+        Ladd = W * environment(S, S.recs[scene_of(P, p)], v3(P.dx[p], P.dy[p], P.dz[p]));
+    } else {
+        const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
+        const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
+        const float t = P.ht[p];
+        const V3 pos = o + d * t;
+        V3 normal;
+        const GShapeRef sr = S.shape_ref[P.hshape[p]];
+        int32_t mat = sr.material;
+        if (prim == -2) {
+            normal = unit3(pos - ld3(S.insts[sr.inst].center));
+        } else {
+            const GTriShade &sh = S.shade[prim];
+            normal = tri_normal(sh, P.hu[p], P.hv[p]);
+            if (mat < 0) mat = sh.material;
+        }
+        const GMat &m = S.mats[mat];
+        Ladd = W * ld3(m.emissive);
+        // TexturedMaterial: Shade and GetBaseColor times the texture
+        // at the hit's texcoord (1 when untextured, an exact no-op)
+        V3 tcol = v3(1.0f, 1.0f, 1.0f);
+        if (m.texture >= 0) {
+            float tu, tv;
+            if (prim == -2) sphere_texcoord(normal, tu, tv);
+            else tri_texcoord(S.tri_uv[prim], P.hu[p], P.hv[p], tu, tv);
+            tcol = tex_sample(S.texels, S.textures[m.texture], tu, tv);
+        }
+        Pcg rng;
+        rng.state = P.rng[p];
+        const SceneRec &rec = S.recs[scene_of(P, p)];
+        if (!m.delta && rec.light_count > 0) {
+            // NextEventEstimation sampling half (s3d_pt.cpp:258-268)
+            const uint32_t li = (uint32_t)rec.light_base + rng.u32() % (uint32_t)rec.light_count;
+            const GLight &lt = S.lights[li];
+            V3 lpos;
+            if (lt.type == LIGHT_SPHERE) {
+                // Sphere::Sample (s3d_sphere.cpp:106-115)
+                const float r1 = k2Pi * rng.f32();
+                const float r2 = 1.0f - 2.0f * rng.f32();
+                const float r3 = sqrtf(1.0f - r2 * r2);
+                lpos = ld3(lt.corner) + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (lt.e1[0] + 1e-1f);
+            } else {
+                const float u = rng.f32();
+                const float v = rng.f32();
+                lpos = ld3(lt.corner) + ld3(lt.e1) * u + ld3(lt.e2) * v;
+            }
+            V3 ldir = lpos - pos;
+            const float d2 = dot3(ldir, ldir);
+            ldir = safe_unit3(ldir);
+            // Where the shadow ray meets the sampled light, by that
+            // shape's own test, and the NextEventEstimation term
+            // (s3d_pt.cpp:274-289) it adds if nothing is hit before.
+            DRay r;
+            r.ox = pos.x; r.oy = pos.y; r.oz = pos.z;
+            r.dx = ldir.x; r.dy = ldir.y; r.dz = ldir.z;
+            float tl = -1.0f, lu = 0.0f, lv = 0.0f;
+            int32_t ltri = -1;
+            if (lt.type == LIGHT_SPHERE) {
+                tl = sphere_test(lt.corner[0], lt.corner[1], lt.corner[2], lt.e1[0], r, kRefHitMin);
+            } else {
+                float best = kFMaxF;
+                for (int k = 0; k < lt.count; ++k) {
+                    float t, u, v;
+                    if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) &&
+                        t < best) {
+                        best = t; lu = u; lv = v; ltri = lt.tri[k];
+                    }
+                }
+                if (ltri >= 0) tl = best;
+            }
+            V3 c = v3(0.0f, 0.0f, 0.0f);
+            if (tl >= 0.0f) {
+                const V3 ph = pos + ldir * tl;
+                V3 ln, le;
+                if (lt.type == LIGHT_SPHERE) {
+                    ln = unit3(ph - ld3(lt.corner));
+                    le = ld3(lt.e2);
+                } else {
+                    ln = tri_normal(S.shade[ltri], lu, lv);
+                    le = ld3(S.mats[S.shade[ltri].material].emissive);
+                }
+                const V3 fs = (base_color(m) * tcol) / kPi;
+                const float cos_shadow = fabsf(dot3(normal, ldir));
+                const float cos_light = fabsf(dot3(ln, -ldir));
+                const float G = (cos_shadow * cos_light) / d2;
+                const float brdf_pdf = fabsf(ldir.z / kPi) * cos_light / d2;
+                const float mis = lt.pdf / (brdf_pdf + lt.pdf);
+                const V3 nee = le * fs * mis * (G / lt.pdf);
+                c = W * nee;
+            }
+            P.sx[p] = ldir.x; P.sy[p] = ldir.y; P.sz[p] = ldir.z;
+            P.stl[p] = tl;
+            P.scr[p] = c.x; P.scg[p] = c.y; P.scb[p] = c.z;
+            shadow = true;
+        }
+        V3 out;
+        bool dice = false;
+        W = W * (shade(m, d, normal, rng, out, dice) * tcol);
+        P.rng[p] = rng.state;
+        P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
+        const bool stop = (dice && depth >= rr_start) ||
+                          (W.x < FLT_EPSILON && W.y < FLT_EPSILON && W.z < FLT_EPSILON);
+        if (!stop) {
+            P.dx[p] = out.x; P.dy[p] = out.y; P.dz[p] = out.z;
+            cont = true;
+        }
+        P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
+    }
+    // L += Ladd, skipped when Ladd is zero in every channel: L starts
+    // at +0 and a sum is -0 only when both terms are, so adding a zero
+    // leaves L's bits as they are (a NaN compares unequal and is added)
+    if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
+        P.lr[p] = P.lr[p] + Ladd.x;
+        P.lg[p] = P.lg[p] + Ladd.y;
+        P.lb[p] = P.lb[p] + Ladd.z;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t cur, int32_t depth,
                                                    int32_t rr_start) {
     const uint32_t n = P.cnt[cur];
@@ -298,128 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
         uint32_t p = 0;
         if (active) {
             p = q[i];
-            V3 W = v3(P.wr[p], P.wg[p], P.wb[p]);
-            V3 Ladd;   // what this vertex adds to the path's radiance L
-            const int32_t prim = P.hprim[p];
-            if (prim == -1) {
-                // miss: L += W * SampleIBL(dir) (s3d_pt.cpp:199-203), or the
-                // constant environment when the scene has no IBL texture
-                Ladd = W * environment(S, S.recs[scene_of(P, p)], v3(P.dx[p], P.dy[p], P.dz[p]));
-            } else {
-                const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
-                const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
-                const float t = P.ht[p];
-                const V3 pos = o + d * t;
-                V3 normal;
-                const GShapeRef sr = S.shape_ref[P.hshape[p]];
-                int32_t mat = sr.material;
-                if (prim == -2) {
-                    normal = unit3(pos - ld3(S.insts[sr.inst].center));
-                } else {
-                    const GTriShade &sh = S.shade[prim];
-                    normal = tri_normal(sh, P.hu[p], P.hv[p]);
-                    if (mat < 0) mat = sh.material;
-                }
-                const GMat &m = S.mats[mat];
-                Ladd = W * ld3(m.emissive);
-                // TexturedMaterial: Shade and GetBaseColor times the texture
-                // at the hit's texcoord (1 when untextured, an exact no-op)
-                V3 tcol = v3(1.0f, 1.0f, 1.0f);
-                if (m.texture >= 0) {
-                    float tu, tv;
-                    if (prim == -2) sphere_texcoord(normal, tu, tv);
-                    else tri_texcoord(S.tri_uv[prim], P.hu[p], P.hv[p], tu, tv);
-                    tcol = tex_sample(S.texels, S.textures[m.texture], tu, tv);
-                }
-                Pcg rng;
-                rng.state = P.rng[p];
-                const SceneRec &rec = S.recs[scene_of(P, p)];
-                if (!m.delta && rec.light_count > 0) {
-                    // NextEventEstimation sampling half (s3d_pt.cpp:258-268)
-                    const uint32_t li = (uint32_t)rec.light_base + rng.u32() % (uint32_t)rec.light_count;
-                    const GLight &lt = S.lights[li];
-                    V3 lpos;
-                    if (lt.type == LIGHT_SPHERE) {
-                        // Sphere::Sample (s3d_sphere.cpp:106-115)
-                        const float r1 = k2Pi * rng.f32();
-                        const float r2 = 1.0f - 2.0f * rng.f32();
-                        const float r3 = sqrtf(1.0f - r2 * r2);
-                        lpos = ld3(lt.corner) + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (lt.e1[0] + 1e-1f);
-                    } else {
-                        const float u = rng.f32();
-                        const float v = rng.f32();
-                        lpos = ld3(lt.corner) + ld3(lt.e1) * u + ld3(lt.e2) * v;
-                    }
-                    V3 ldir = lpos - pos;
-                    const float d2 = dot3(ldir, ldir);
-                    ldir = safe_unit3(ldir);
-                    // Where the shadow ray meets the sampled light, by that
-                    // shape's own test, and the NextEventEstimation term
-                    // (s3d_pt.cpp:274-289) it adds if nothing is hit before.
-                    DRay r;
-                    r.ox = pos.x; r.oy = pos.y; r.oz = pos.z;
-                    r.dx = ldir.x; r.dy = ldir.y; r.dz = ldir.z;
-                    float tl = -1.0f, lu = 0.0f, lv = 0.0f;
-                    int32_t ltri = -1;
-                    if (lt.type == LIGHT_SPHERE) {
-                        tl = sphere_test(lt.corner[0], lt.corner[1], lt.corner[2], lt.e1[0], r, kRefHitMin);
-                    } else {
-                        float best = kFMaxF;
-                        for (int k = 0; k < lt.count; ++k) {
-                            float t, u, v;
-                            if (tri_test(S.tris, (uint32_t)lt.tri[k], r, kRefHitMin, kRefHitMax, t, u, v) &&
-                                t < best) {
-                                best = t; lu = u; lv = v; ltri = lt.tri[k];
-                            }
-                        }
-                        if (ltri >= 0) tl = best;
-                    }
-                    V3 c = v3(0.0f, 0.0f, 0.0f);
-                    if (tl >= 0.0f) {
-                        const V3 ph = pos + ldir * tl;
-                        V3 ln, le;
-                        if (lt.type == LIGHT_SPHERE) {
-                            ln = unit3(ph - ld3(lt.corner));
-                            le = ld3(lt.e2);
-                        } else {
-                            ln = tri_normal(S.shade[ltri], lu, lv);
-                            le = ld3(S.mats[S.shade[ltri].material].emissive);
-                        }
-                        const V3 fs = (base_color(m) * tcol) / kPi;
-                        const float cos_shadow = fabsf(dot3(normal, ldir));
-                        const float cos_light = fabsf(dot3(ln, -ldir));
-                        const float G = (cos_shadow * cos_light) / d2;
-                        const float brdf_pdf = fabsf(ldir.z / kPi) * cos_light / d2;
-                        const float mis = lt.pdf / (brdf_pdf + lt.pdf);
-                        const V3 nee = le * fs * mis * (G / lt.pdf);
-                        c = W * nee;
-                    }
-                    P.sx[p] = ldir.x; P.sy[p] = ldir.y; P.sz[p] = ldir.z;
-                    P.stl[p] = tl;
-                    P.scr[p] = c.x; P.scg[p] = c.y; P.scb[p] = c.z;
-                    shadow = true;
-                }
-                V3 out;
-                bool dice = false;
-                W = W * (shade(m, d, normal, rng, out, dice) * tcol);
-                P.rng[p] = rng.state;
-                P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
-                const bool stop = (dice && depth >= rr_start) ||
-                                  (W.x < FLT_EPSILON && W.y < FLT_EPSILON && W.z < FLT_EPSILON);
-                if (!stop) {
-                    P.dx[p] = out.x; P.dy[p] = out.y; P.dz[p] = out.z;
-                    cont = true;
-                }
-                P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
-            }
-            // L += Ladd, skipped when Ladd is zero in every channel: L starts
-            // at +0 and a sum is -0 only when both terms are, so adding a zero
-            // leaves L's bits as they are (a NaN compares unequal and is added)
-            if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
-                P.lr[p] = P.lr[p] + Ladd.x;
-                P.lg[p] = P.lg[p] + Ladd.y;
-                P.lb[p] = P.lb[p] + Ladd.z;
-            }
+            shade_path(S, P, p, depth, rr_start, shadow, cont);
         }
         stage.flush_if_full(&P.cnt[2], P.qs, &P.cnt[cur ^ 1], P.q[cur ^ 1], false);
         stage.add(shadow, cont, p);
@@ -463,6 +478,68 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, Fetch
             P.lb[p] = P.lb[p] + P.scb[p];
         });
     tr.flush(P.tot + 4);
+}
+
+// Renders of few paths: each thread carries one path through all of its
+// bounces in a single launch (raygen, then per bounce the closest hit,
+// shade_path and the shadow ray), the per-path arithmetic of the wavefront
+// kernels in the same order, so the sums are the same bit for bit.  A small
+// render then costs one launch instead of three per bounce; the wavefront
+// pipeline wins once the paths outnumber the threads the device holds.
+template <int LDS>
+__global__ __launch_bounds__(kBlock) void k_paths(SceneDev S, PathBuf P, const BatchDesc *__restrict__ bd,
+                                                  const Seg *__restrict__ segs, int32_t max_bounce,
+                                                  int32_t rr_start, uint2 *gstack, int32_t gcap, int32_t *err) {
+    __shared__ uint2 lds[LDS * kBlock];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    Stack<LDS> st;
+    st.init(lds, threadIdx.x, gstack + p, gridDim.x * blockDim.x, gcap, err);
+    unsigned long long ext = 0, shadows = 0;
+    if (p < bd->paths) {
+        raygen_path(S, P, p, bd->nseg, segs);
+        const SceneRec &rec = S.recs[scene_of(P, p)];
+        for (int32_t depth = 0; depth < max_bounce; ++depth) {
+            DRay r;
+            r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
+            r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
+            const DHit h = traverse<false, false>(S, rec, r, kRefHitMin, kRefHitMax, st, nullptr);
+            P.ht[p] = h.t;
+            P.hu[p] = h.u;
+            P.hv[p] = h.v;
+            P.hprim[p] = h.prim;
+            P.hshape[p] = h.shape;
+            ++ext;
+            bool shadow = false, cont = false;
+            shade_path(S, P, p, depth, rr_start, shadow, cont);
+            if (shadow) {
+                ++shadows;
+                const float tl = P.stl[p];
+                if (tl >= 0.0f) {
+                    DRay sr;
+                    sr.ox = P.ox[p]; sr.oy = P.oy[p]; sr.oz = P.oz[p];
+                    sr.dx = P.sx[p]; sr.dy = P.sy[p]; sr.dz = P.sz[p];
+                    // "closest hit is the light" == no other hit strictly before it
+                    const DHit hs = traverse<true, false>(S, rec, sr, kRefHitMin, nextafterf(tl, 0.0f), st, nullptr);
+                    if (hs.prim == -1) {
+                        P.lr[p] = P.lr[p] + P.scr[p];
+                        P.lg[p] = P.lg[p] + P.scg[p];
+                        P.lb[p] = P.lb[p] + P.scb[p];
+                    }
+                }
+            }
+            if (!cont) break;
+        }
+    }
+    // ray counts of the render statistics: one atomic per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ext += __shfl_xor(ext, off);
+        shadows += __shfl_xor(shadows, off);
+    }
+    if (__lane_id() == 0) {
+        if (ext) atomicAdd(&P.tot[0], ext);
+        if (shadows) atomicAdd(&P.tot[1], shadows);
+    }
 }
 
 // m_RenderTarget[idx] += L for each pass in turn (s3d_pt.cpp:330-332).
@@ -544,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_guides(SceneDev S, int32_t W, int32_
 struct TraceKernels {
     const void *ext[2];
     const void *sh[2];
+    const void *paths;
     int lds;
 };
 
@@ -554,7 +632,7 @@ TraceKernels trace_kernels_of() {
                          reinterpret_cast<const void *>(&k_extend<true, L, W>)},
                         {reinterpret_cast<const void *>(&k_shadow<false, L, W>),
                          reinterpret_cast<const void *>(&k_shadow<true, L, W>)},
-                        L};
+                        reinterpret_cast<const void *>(&k_paths<L>), L};
 }
 
 TraceKernels trace_kernels(uint32_t v) {
@@ -709,7 +787,17 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
 
     // one batch's kernels; li = index of its first bounce launch (timing)
+    // One batch that the device holds thread per path: each thread traces its
+    // path through every bounce in one launch (same sums as the pipeline).
+    const bool per_path = !timed && !count && !(flags & S3DG_RENDER_WAVEFRONT) && batches.size() == 1 &&
+                          (uint64_t)slots <= (uint64_t)tgrid * kBlock;
     auto launch_batch = [&](size_t li) {
+        if (per_path) {
+            void *args[] = {&Sx, &P, &d_desc, &d_segs, &max_bounce, &rr_start, &gs, &gcap, &d_err};
+            (void)hipLaunchKernel(K.paths, dim3((slots + kBlock - 1) / kBlock), dim3(kBlock), args, 0, ctx->stream);
+            hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_desc, d_segs);
+            return;
+        }
         hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(kBlock), 0, ctx->stream, Sx, P, d_desc, d_segs);
         int32_t cur = 0;
         for (int32_t depth = 0; depth < max_bounce; ++depth, ++li) {
@@ -734,7 +822,8 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     if (use_graph) {
         const uint64_t key[10] = {(uint64_t)(uintptr_t)ws.d_gstack, (uint64_t)(uint32_t)gcap,
                                   (uint64_t)(uintptr_t)ws.d_render,
-                                 slots | ((uint64_t)(uint32_t)(P.single_scene + 1) << 32), (uint64_t)max_bounce,
+                                 slots | ((uint64_t)(uint32_t)(P.single_scene + 1) << 32),
+                                 (uint64_t)(uint32_t)max_bounce | ((uint64_t)per_path << 32),
                                  ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
                                  (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
                                  ((uint64_t)fetch.refill << 48) | ((uint64_t)fetch.chunk << 16) | Sx.tri_batch};

@@ -42,10 +42,12 @@ def test_variants_render_identically(ctx):
         ref = None
         for t in _settings():
             ctx.set_tuning(t)
-            img, st = sc.render(64, 48, spp=3, max_depth=6, seed=2)
-            if ref is None:
-                ref = (img, st.extension_rays, st.shadow_rays)
-            assert np.array_equal(img, ref[0]) and (st.extension_rays, st.shadow_rays) == ref[1:]
+            # the small render runs thread per path unless the wavefront pipeline is asked for
+            for flags in (0, abi.RENDER_WAVEFRONT):
+                img, st = sc.render(64, 48, spp=3, max_depth=6, seed=2, flags=flags)
+                if ref is None:
+                    ref = (img, st.extension_rays, st.shadow_rays)
+                assert np.array_equal(img, ref[0]) and (st.extension_rays, st.shadow_rays) == ref[1:]
     finally:
         ctx.set_tuning(start)
 
