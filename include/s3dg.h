@@ -207,12 +207,16 @@ typedef struct s3dg_render_config {
 
 /* flags: time every traversal launch with HIP events on the context stream;
  * count compressed nodes visited and triangles tested by the traversals;
- * always use the per-bounce wavefront pipeline (otherwise a render whose
- * paths fit the device's resident threads traces each path in one thread,
- * in one launch, with the same result).  The first two imply the third. */
+ * always use the per-bounce wavefront pipeline; trace each path in one
+ * thread, in one launch, whatever the number of paths.  Without the last two
+ * a render of one batch of up to four times the device's resident traversal
+ * threads (about 1.5 M paths) is traced thread per path, a larger one by the
+ * pipeline; both give the same sums and counts.  The first two flags, and a
+ * render of several batches, imply the pipeline. */
 #define S3DG_RENDER_TIME_KERNELS   1u
 #define S3DG_RENDER_COUNT_WORK     2u
 #define S3DG_RENDER_WAVEFRONT      4u
+#define S3DG_RENDER_PER_PATH       8u
 
 typedef struct s3dg_render_stats {
     uint64_t extension_rays;   /* closest-hit rays traced                      */

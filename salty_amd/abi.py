@@ -102,6 +102,7 @@ class FrameJob(C.Structure):
 RENDER_TIME_KERNELS = 1
 RENDER_COUNT_WORK = 2
 RENDER_WAVEFRONT = 4
+RENDER_PER_PATH = 8
 
 
 class RenderStats(C.Structure):

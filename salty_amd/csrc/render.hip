@@ -480,22 +480,24 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, Fetch
     tr.flush(P.tot + 4);
 }
 
-// Renders of few paths: each thread carries one path through all of its
-// bounces in a single launch (raygen, then per bounce the closest hit,
-// shade_path and the shadow ray), the per-path arithmetic of the wavefront
-// kernels in the same order, so the sums are the same bit for bit.  A small
-// render then costs one launch instead of three per bounce; the wavefront
-// pipeline wins once the paths outnumber the threads the device holds.
+// Renders of few paths: each thread carries its paths (grid-stride) through
+// all of their bounces in a single launch (raygen, then per bounce the
+// closest hit, shade_path and the shadow ray), the per-path arithmetic of the
+// wavefront kernels in the same order, so the sums are the same bit for bit.
+// A small render then costs one launch instead of three per bounce; with many
+// paths the wavefront pipeline's coherent, compacted launches win.
 template <int LDS>
 __global__ __launch_bounds__(kBlock) void k_paths(SceneDev S, PathBuf P, const BatchDesc *__restrict__ bd,
                                                   const Seg *__restrict__ segs, int32_t max_bounce,
                                                   int32_t rr_start, uint2 *gstack, int32_t gcap, int32_t *err) {
     __shared__ uint2 lds[LDS * kBlock];
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
     Stack<LDS> st;
-    st.init(lds, threadIdx.x, gstack + p, gridDim.x * blockDim.x, gcap, err);
+    st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     unsigned long long ext = 0, shadows = 0;
-    if (p < bd->paths) {
+    const uint32_t npaths = bd->paths;
+    for (uint32_t p = gtid; p < npaths; p += stride) {
         raygen_path(S, P, p, bd->nseg, segs);
         const SceneRec &rec = S.recs[scene_of(P, p)];
         for (int32_t depth = 0; depth < max_bounce; ++depth) {
@@ -787,14 +789,21 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
 
     // one batch's kernels; li = index of its first bounce launch (timing)
-    // One batch that the device holds thread per path: each thread traces its
-    // path through every bounce in one launch (same sums as the pipeline).
+    // One batch of up to kPerPathRounds x the resident traversal threads is
+    // traced thread per path: each thread carries its paths through every
+    // bounce in one launch (same sums as the pipeline).  Measured on the
+    // Cornell box at 256x256 the two break even near 16 spp (1 M paths); on
+    // the mesh scene at 480x270 thread per path is still 1.8x faster at 16 spp.
+    constexpr uint64_t kPerPathRounds = 4;
     const bool per_path = !timed && !count && !(flags & S3DG_RENDER_WAVEFRONT) && batches.size() == 1 &&
-                          (uint64_t)slots <= (uint64_t)tgrid * kBlock;
+                          ((flags & S3DG_RENDER_PER_PATH) || (uint64_t)slots <= kPerPathRounds * tgrid * kBlock);
+    // its grid: one thread per path slot, at most as many threads as the
+    // traversal stack area was sized for
+    const uint32_t pgrid = std::min<uint32_t>((slots + kBlock - 1) / kBlock, tgrid);
     auto launch_batch = [&](size_t li) {
         if (per_path) {
             void *args[] = {&Sx, &P, &d_desc, &d_segs, &max_bounce, &rr_start, &gs, &gcap, &d_err};
-            (void)hipLaunchKernel(K.paths, dim3((slots + kBlock - 1) / kBlock), dim3(kBlock), args, 0, ctx->stream);
+            (void)hipLaunchKernel(K.paths, dim3(pgrid), dim3(kBlock), args, 0, ctx->stream);
             hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_desc, d_segs);
             return;
         }

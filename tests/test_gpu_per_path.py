@@ -75,3 +75,18 @@ def test_per_path_repeated_renders_and_sizes(ctx):
         b, sb = sc.render(w, h, spp, 8, seed=4, flags=abi.RENDER_WAVEFRONT)
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
         assert (sa.extension_rays, sa.shadow_rays) == (sb.extension_rays, sb.shadow_rays)
+
+
+def test_forced_per_path_with_several_paths_per_thread(ctx):
+    """2 M paths: more than the per-path grid holds, so its threads take
+    several paths each (grid-stride); still the wavefront's sums."""
+    import salty_amd
+    sc = salty_amd.Scene(ctx, scenes.cornell_box())
+    w, h, spp = 512, 512, 8
+    a, sa = sc.render(w, h, spp, 8, seed=6, flags=abi.RENDER_PER_PATH)
+    b, sb = sc.render(w, h, spp, 8, seed=6, flags=abi.RENDER_WAVEFRONT)
+    c, sc_ = sc.render(w, h, spp, 8, seed=6)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(c.view(np.uint32), b.view(np.uint32))
+    assert (sa.extension_rays, sa.shadow_rays) == (sb.extension_rays, sb.shadow_rays) == \
+        (sc_.extension_rays, sc_.shadow_rays)
