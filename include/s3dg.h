@@ -449,7 +449,10 @@ int s3dg_nlm_device(s3dg_context *ctx, int32_t width, int32_t height, float coef
  * radiance) run with BF16 MFMA kernels and FP32 accumulation.
  * Input features: N x H x W x 10 floats (radiance 3, albedo 3, normal 3,
  * depth 1), H and W multiples of 8; output N x H x W x 3 linear radiance.
- * Training minimises the L1 loss on log(1 + radiance) with Adam.
+ * Training minimises the relative squared error
+ * (pred - target)^2 / (pred^2 + 0.01), denominator held constant, with Adam;
+ * pixels whose depth guide is 0 (camera ray missed) pass their input
+ * radiance through and take no part in the loss.
  * ------------------------------------------------------------------------- */
 typedef struct s3dg_cnn s3dg_cnn;
 
@@ -459,9 +462,11 @@ int s3dg_cnn_param_count(const s3dg_cnn *cnn, uint64_t *count);
 /* Flat FP32 parameters: per layer W[9][cout][cin] then b[cout], head last. */
 int s3dg_cnn_get_params(s3dg_cnn *cnn, float *host, uint64_t count);
 int s3dg_cnn_set_params(s3dg_cnn *cnn, const float *host, uint64_t count);
+/* The second call with the same shape and buffers captures the launches
+ * into a HIP graph, later ones replay it (same results). */
 int s3dg_cnn_infer_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
                           float *d_out_rgb);
-/* Mean L1 loss and the gradient of every parameter (host copy optional). */
+/* Mean loss and the gradient of every parameter (host copy optional). */
 int s3dg_cnn_gradient_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
                              const float *d_target_rgb, double *loss, float *host_grad, uint64_t count);
 int s3dg_cnn_train_step_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,

@@ -600,6 +600,18 @@ struct s3dg_cnn {
     void *ws = nullptr;
     size_t ws_bytes = 0;
     double *d_loss = nullptr;
+    // inference launches: run directly the first time a (shape, buffers) key
+    // is seen, captured into a HIP graph the second time, replayed after that
+    struct InferGraph {
+        uint64_t key[6];
+        hipGraphExec_t exec;
+        uint64_t used;   // tick of the last launch (least recently used is replaced)
+    };
+    static constexpr int kGraphs = 4, kSeen = 8;
+    InferGraph graphs[kGraphs] = {};
+    uint64_t seen[kSeen][6] = {};   // keys run directly (ring)
+    int seen_next = 0;
+    uint64_t tick = 0;
 };
 
 namespace {
@@ -879,6 +891,8 @@ extern "C" int s3dg_cnn_create(s3dg_context *ctx, uint64_t seed, s3dg_cnn **out)
 
 extern "C" int s3dg_cnn_destroy(s3dg_cnn *c) {
     if (!c) return S3DG_ERR_INVALID;
+    for (auto &g : c->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
     void *ptrs[] = {c->p, c->g, c->m, c->v, c->wbuf, c->ws, c->d_loss};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -917,9 +931,49 @@ extern "C" int s3dg_cnn_infer_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t 
     Acts a;
     rc = prepare(c, n0, a);
     if (rc) return rc;
-    forward(c, a, d_feat, n, h, w);
-    hipLaunchKernelGGL(k_head_infer, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 8192)), dim3(256), 0,
-                       c->ctx->stream, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, d_feat, d_out);
+    hipStream_t s = c->ctx->stream;
+    auto launch = [&]() {
+        forward(c, a, d_feat, n, h, w);
+        hipLaunchKernelGGL(k_head_infer, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 8192)), dim3(256), 0,
+                           s, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, d_feat, d_out);
+    };
+    // The ~20 launches of one inference as one graph launch once the same
+    // shape and buffers come back (a denoiser runs frame after frame on the
+    // same buffers).  The kernels read the weights through fixed pointers,
+    // so a graph stays valid when training or set_params changes them.
+    const uint64_t key[6] = {(uint64_t)n, (uint64_t)h, (uint64_t)w, (uint64_t)(uintptr_t)d_feat,
+                             (uint64_t)(uintptr_t)d_out, (uint64_t)(uintptr_t)c->ws};
+    ++c->tick;
+    s3dg_cnn::InferGraph *hit = nullptr, *lru = &c->graphs[0];
+    for (auto &g : c->graphs) {
+        if (g.exec && memcmp(g.key, key, sizeof(key)) == 0) hit = &g;
+        if (!g.exec || (lru->exec && g.used < lru->used)) lru = &g;
+    }
+    bool seen = false;
+    for (const auto &k : c->seen) seen = seen || memcmp(k, key, sizeof(key)) == 0;
+    if (hit) {
+        hit->used = c->tick;
+        S3DG_HIP(c->ctx, hipGraphLaunch(hit->exec, s));
+    } else if (seen) {
+        if (lru->exec) {
+            (void)hipGraphExecDestroy(lru->exec);
+            lru->exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        S3DG_HIP(c->ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        launch();
+        S3DG_HIP(c->ctx, hipStreamEndCapture(s, &g));
+        const hipError_t ie = hipGraphInstantiate(&lru->exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        S3DG_HIP(c->ctx, ie);
+        memcpy(lru->key, key, sizeof(key));
+        lru->used = c->tick;
+        S3DG_HIP(c->ctx, hipGraphLaunch(lru->exec, s));
+    } else {
+        launch();
+        memcpy(c->seen[c->seen_next], key, sizeof(key));
+        c->seen_next = (c->seen_next + 1) % s3dg_cnn::kSeen;
+    }
     S3DG_HIP(c->ctx, hipGetLastError());
     return S3DG_OK;
 }

@@ -238,6 +238,13 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65
     plain_ms = crossing([(ms, r) for _, ms, r in plain], target)
     d_f = ctx.malloc(width * height * 10 * 4)
     d_o = ctx.malloc(width * height * 3 * 4)
+    # two untimed inferences on these buffers: the first runs the launches
+    # directly, the second captures them into the graph the timed ones replay
+    # (as the renders are timed after an untimed one)
+    ctx.copy_to_device(d_f, np.zeros(width * height * 10, np.float32))
+    for _ in range(2):
+        net.infer_device(1, height, width, d_f, d_o)
+    ctx.synchronize()
     den = []
     for spp, img, _ in fr.ladder(den_levels):
         ms = fr.render_ms(spp)

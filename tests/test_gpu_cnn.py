@@ -90,3 +90,38 @@ def test_miss_pixels_pass_through(ctx):
     m = feat[..., 9] == 0
     assert np.array_equal(out[m], feat[..., :3][m])
     assert not np.array_equal(out[~m], feat[..., :3][~m])
+
+
+def test_inference_graph_replays_match_direct_launches(ctx):
+    """The first inference of a shape runs its launches directly, the second
+    captures them into a graph, later ones replay it: every result equals the
+    direct one bit for bit, with other inferences, training steps (which
+    change the weights the graph reads) and renders in between."""
+    import salty_amd
+    from salty_amd import scenes
+    net = salty_amd.Denoiser(ctx, seed=9)
+    feat, target = case(3, n=1, h=32, w=40)
+    other, _ = case(4, n=2, h=16, w=16)
+    d_f, d_t = ctx.upload(feat), ctx.upload(target)
+    d_o = ctx.malloc(32 * 40 * 3 * 4)
+    sc = salty_amd.Scene(ctx, scenes.cornell_box())
+    try:
+        outs = []
+        for k in range(5):
+            net.infer_device(1, 32, 40, d_f, d_o)
+            outs.append(ctx.download(d_o, (1, 32, 40, 3), np.float32))
+            net.infer(other)
+            sc.render(16, 16, 1, 4, seed=k)
+        for o in outs[1:]:
+            assert np.array_equal(o.view(np.uint32), outs[0].view(np.uint32))
+        # new weights: the replayed graph reads them (first direct run of the
+        # training shape, then replays of the inference graph)
+        net.train_step_device(1, 32, 40, d_f, d_t, 1e-3)
+        direct = net.infer(feat)   # new buffers: launched directly
+        net.infer_device(1, 32, 40, d_f, d_o)   # graph replay
+        assert np.array_equal(ctx.download(d_o, (1, 32, 40, 3), np.float32).view(np.uint32), direct.view(np.uint32))
+        assert not np.array_equal(direct.view(np.uint32), outs[0].view(np.uint32))
+    finally:
+        ctx.free(d_f)
+        ctx.free(d_t)
+        ctx.free(d_o)
