@@ -1246,6 +1246,35 @@ void orc_camera_ray(const void *sp, float x, float y, float org[3], float dir[3]
     store3(dir, d);
 }
 
+// The camera ray with ThinLensCamera's lens sample drawn from a PCG state
+// (advanced in place), as a path's raygen does.
+void orc_camera_ray_lens(const void *sp, float x, float y, uint64_t *state, float org[3], float dir[3]) {
+    const Scene *sc = (const Scene *)sp;
+    PCG rng;
+    rng.state = *state;
+    V3 o, d;
+    camera_ray(*sc, x, y, &rng, o, d);
+    *state = rng.state;
+    store3(org, o);
+    store3(dir, d);
+}
+
+// Texture::SampleColor by texcoord and by direction on a w x h RGB texel array.
+static Texture make_texture(int32_t w, int32_t h, const float *rgb) {
+    Texture t;
+    t.w = w;
+    t.h = h;
+    t.px.resize((size_t)w * h);
+    for (size_t i = 0; i < t.px.size(); ++i) t.px[i] = load3(rgb + 3 * i);
+    return t;
+}
+void orc_texture_sample(int32_t w, int32_t h, const float *rgb, float u, float v, float out[3]) {
+    store3(out, make_texture(w, h, rgb).sample(u, v));
+}
+void orc_texture_sample_dir(int32_t w, int32_t h, const float *rgb, const float dir[3], float out[3]) {
+    store3(out, make_texture(w, h, rgb).sample_dir(load3(dir)));
+}
+
 // PathTracer::TracePath (src/src/s3d_pt.cpp:297-353): pixel-centre camera rays
 // ((0.5+x)/W - 0.5, (0.5+y)/H - 0.5), one sample per pass added to a sum buffer.
 int orc_render(const void *sp, const s3dg_render_config *cfg, int32_t row_begin, int32_t row_end,

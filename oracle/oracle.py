@@ -53,6 +53,9 @@ def lib() -> C.CDLL:
             "orc_scene_destroy": (None, [vp]),
             "orc_scene_trace": (C.c_int, [vp, vp, C.c_uint32, vp, vp, C.c_int32]),
             "orc_camera_ray": (None, [vp, C.c_float, C.c_float, vp, vp]),
+            "orc_camera_ray_lens": (None, [vp, C.c_float, C.c_float, vp, vp, vp]),
+            "orc_texture_sample": (None, [C.c_int32, C.c_int32, vp, C.c_float, C.c_float, vp]),
+            "orc_texture_sample_dir": (None, [C.c_int32, C.c_int32, vp, vp, vp]),
             "orc_render": (C.c_int, [vp, C.POINTER(abi.RenderConfig), C.c_int32, C.c_int32, vp, vp, C.c_int32]),
             "orc_render_guides": (C.c_int, [vp, C.POINTER(abi.RenderConfig), vp, vp, vp]),
             "orc_tonemap": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, vp]),
@@ -131,6 +134,23 @@ def shade(material: dict, inp, normal, state: int):
     return o, w, bool(dice), int(st[0])
 
 
+def texture_sample(texels: np.ndarray, u: float, v: float) -> np.ndarray:
+    """Texture::SampleColor(Vector2) on an H x W x 3 texel array."""
+    t = np.ascontiguousarray(texels, dtype=np.float32)
+    o = np.empty(3, np.float32)
+    lib().orc_texture_sample(t.shape[1], t.shape[0], _p(t), u, v, _p(o))
+    return o
+
+
+def texture_sample_dir(texels: np.ndarray, d) -> np.ndarray:
+    """Texture::SampleColor(Vector3): the latitude-longitude lookup."""
+    t = np.ascontiguousarray(texels, dtype=np.float32)
+    dd = np.ascontiguousarray(d, dtype=np.float32)
+    o = np.empty(3, np.float32)
+    lib().orc_texture_sample_dir(t.shape[1], t.shape[0], _p(t), _p(dd), _p(o))
+    return o
+
+
 def invert(m) -> np.ndarray:
     a = np.ascontiguousarray(m, dtype=np.float32).reshape(16)
     out = np.empty(16, np.float32)
@@ -195,6 +215,19 @@ class OracleScene:
         shapes = np.empty(len(rays), dtype=np.int32)
         lib().orc_scene_trace(self.h, _p(rays), len(rays), _p(hits), _p(shapes), threads)
         return hits, shapes
+
+    def camera_ray(self, x: float, y: float, state: int | None = None):
+        """(origin, direction, state) of the camera ray through image-plane
+        coordinates x, y in [-0.5, 0.5]; with a PCG state the thin lens is
+        sampled from it (the state returned advanced)."""
+        o = np.empty(3, np.float32)
+        d = np.empty(3, np.float32)
+        if state is None:
+            lib().orc_camera_ray(self.h, x, y, _p(o), _p(d))
+            return o, d, None
+        st = np.array([state], np.uint64)
+        lib().orc_camera_ray_lens(self.h, x, y, _p(st), _p(o), _p(d))
+        return o, d, int(st[0])
 
     def render(self, width, height, spp, max_depth, seed=0, rr_start_depth=0, first_pass=0, rows=None,
                accum=None, threads: int = 0):

@@ -130,3 +130,102 @@ def test_shade_consumes_the_documented_number_of_draws():
     assert draws(GLASS) == 1
     assert draws(dict(type=abi.MAT_PHONG, specular=(0.5, 0.5, 0.5), power=20.0)) == 3
     assert draws(dict(type=abi.MAT_PLASTIC, diffuse=(0.5, 0.5, 0.5), specular=(0.5, 0.5, 0.5), power=20)) == 4
+
+
+def _normal_incidence_samples(mat, count, state=4242):
+    n = np.float32([0, 0, 1])
+    inp = np.float32([0, 0, -1])
+    out = []
+    for _ in range(count):
+        o, w, d, state = O.shade(mat, inp, n, state)
+        out.append((o, w, d))
+    return out
+
+
+@pytest.mark.parametrize("power", [1.0, 10.0])
+def test_phong_lobe_at_normal_incidence(power):
+    """At normal incidence the Phong lobe is centred on the normal, with
+    cos(theta) = (1 - v)^(1/(n+1)): E[cos] = (n+1)/(n+2); the weight is the
+    specular colour times cos; dice = rand >= max(specular)."""
+    spec = (0.5, 0.7, 0.2)
+    samples = _normal_incidence_samples(dict(type=abi.MAT_PHONG, specular=spec, power=power), 20000)
+    cos = np.array([o[2] for o, _, _ in samples], np.float64)
+    assert (cos >= 0).all()
+    want = (power + 1) / (power + 2)
+    sd = math.sqrt((power + 1) / (power + 3) - want ** 2)   # E[cos^2] = (n+1)/(n+3)
+    assert abs(cos.mean() - want) < 4 * sd / math.sqrt(len(cos))
+    w = np.array([w for _, w, _ in samples], np.float64)
+    assert np.allclose(w.mean(0), np.array(spec) * want, rtol=0.02)
+    p = 1 - 0.7
+    dice = np.mean([d for _, _, d in samples])
+    assert abs(dice - p) < 4 * math.sqrt(p * (1 - p) / len(samples))
+
+
+def test_phong_high_power_is_a_mirror():
+    n = np.float32([0, 0, 1])
+    inp = np.float32([0.6, 0.0, -0.8])
+    mirror = np.float32([0.6, 0.0, 0.8])
+    state = 5
+    for _ in range(200):
+        o, w, _, state = O.shade(dict(type=abi.MAT_PHONG, specular=(1, 1, 1), power=1e6), inp, n, state)
+        assert np.linalg.norm(o - mirror) < 1e-2
+        assert abs(w[0] - 0.8) < 1e-2
+
+
+def test_plastic_lobes_at_normal_incidence():
+    """Schlick with R0 = 1/2: at normal incidence R = P = 1/2, so the diffuse
+    lobe (uniform hemisphere) is taken half the time with weight exactly the
+    diffuse colour, the specular lobe otherwise with weight specular * cos,
+    E[cos] = (n+1)/(n+2); both lobes roll their dice against the diffuse
+    threshold (the reference copies it into the specular one)."""
+    D, S, power = (0.3, 0.0, 0.0), (0.0, 0.9, 0.0), 3.0
+    samples = _normal_incidence_samples(dict(type=abi.MAT_PLASTIC, diffuse=D, specular=S, power=power), 20000)
+    diffuse = [s for s in samples if s[1][0] > 0]
+    specular = [s for s in samples if s[1][0] == 0]
+    assert len(diffuse) + len(specular) == len(samples)
+    frac = len(diffuse) / len(samples)
+    assert abs(frac - 0.5) < 4 * math.sqrt(0.25 / len(samples))
+    assert all(np.array_equal(w, np.float32(D)) for _, w, _ in diffuse)
+    cos_d = np.array([o[2] for o, _, _ in diffuse])
+    assert abs(cos_d.mean() - 0.5) < 4 * math.sqrt(1 / 12 / len(cos_d))       # uniform hemisphere
+    cos_s = np.array([o[2] for o, _, _ in specular])
+    assert abs(cos_s.mean() - (power + 1) / (power + 2)) < 0.01
+    assert np.allclose([w[1] / o[2] for o, w, _ in specular], 0.9, rtol=1e-5)
+    for group in (diffuse, specular):
+        rate = np.mean([d for _, _, d in group])
+        assert abs(rate - 0.7) < 4 * math.sqrt(0.21 / len(group))           # 1 - max(diffuse)
+
+
+def test_plastic_grazing_incidence_prefers_the_diffuse_lobe_by_schlick():
+    n = np.float32([0, 0, 1])
+    c = 0.05
+    inp = np.float32([math.sqrt(1 - c * c), 0.0, -c])
+    R = 0.5 + 0.5 * (1 - c) ** 5
+    P = (R + 0.5) / 2
+    state, hits, count = 31, 0, 20000
+    for _ in range(count):
+        o, w, _, state = O.shade(dict(type=abi.MAT_PLASTIC, diffuse=(0.4, 0, 0), specular=(0, 0.5, 0), power=2.0),
+                                 inp, n, state)
+        if w[0] > 0:
+            hits += 1
+            assert abs(w[0] - 0.4 * R / P) < 1e-5
+    assert abs(hits / count - P) < 4 * math.sqrt(P * (1 - P) / count)
+
+
+def test_phong_and_plastic_draw_counts():
+    n = np.float32([0, 0, 1])
+    inp = np.float32([0, 0, -1])
+    mul, inc, mask = 6364136223846793005, 1442695040888963407, (1 << 64) - 1
+
+    def draws(mat):
+        s0 = 991
+        _, _, _, s1 = O.shade(mat, inp, n, s0)
+        s, k = s0, 0
+        while s != s1:
+            s = (s * mul + inc) & mask
+            k += 1
+            assert k < 16
+        return k
+
+    assert draws(dict(type=abi.MAT_PHONG, specular=(1, 1, 1), power=5.0)) == 3
+    assert draws(dict(type=abi.MAT_PLASTIC, diffuse=(0.5, 0.5, 0.5), specular=(0.5, 0.5, 0.5), power=5.0)) == 4
