@@ -425,7 +425,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path shards by pixel tiles x "
-              f"sample ranges with a replicated BVH, which this build does not launch.", file=sys.stderr)
+              f"sample ranges with a replicated BVH, and only the one-GPU share is implemented.", file=sys.stderr)
         sys.exit(2)
 
     import salty_amd
