@@ -90,3 +90,16 @@ def test_forced_per_path_with_several_paths_per_thread(ctx):
     assert np.array_equal(c.view(np.uint32), b.view(np.uint32))
     assert (sa.extension_rays, sa.shadow_rays) == (sb.extension_rays, sb.shadow_rays) == \
         (sc_.extension_rays, sc_.shadow_rays)
+
+
+def test_per_path_on_the_full_mesh_scene(ctx):
+    """The 1080p benchmark scene's geometry at a small size: two-level trees
+    deep enough that a traversal stack can outgrow its LDS part and continue
+    in the global overflow area."""
+    import salty_amd
+    sc = salty_amd.Scene(ctx, scenes.mesh_scene())
+    a, sa = sc.render(96, 54, 2, 12, seed=8)
+    b, sb = sc.render(96, 54, 2, 12, seed=8, flags=abi.RENDER_WAVEFRONT)
+    assert np.isfinite(a).all() and a.any()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (sa.extension_rays, sa.shadow_rays) == (sb.extension_rays, sb.shadow_rays)
