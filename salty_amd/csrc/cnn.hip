@@ -89,8 +89,8 @@ constexpr int kTile = 16;                 // output tile edge (pixels)
 constexpr int kHalo = kTile + 2;
 constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 padding
 
-template <int COT>
-__global__ __launch_bounds__(256, COT == 32 ? 3 : 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
+template <int COT, bool PF>
+__global__ __launch_bounds__(256, PF ? 1 : (COT == 32 ? 3 : 2)) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                      const bf16 *__restrict__ wt, const float *__restrict__ bias,
                                                      int32_t Cout, int32_t relu, bf16 *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) bf16 s_in[kHalo * kHalo * kRow];
@@ -125,8 +125,9 @@ __global__ __launch_bounds__(256, COT == 32 ? 3 : 2) void k_conv3x3(ConvIn I, in
     // that the slice costs one memory round trip
     constexpr int kHalfCh = kHalo * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
     constexpr int kWCh = 9 * COT * 4, kWPer = (kWCh + 255) / 256;
-    for (int c0 = 0; c0 < Cin; c0 += 32) {
-        bf16x8 hv[kHalfPer], wv[kWPer];
+    bf16x8 hv[kHalfPer], wv[kWPer];
+    // global loads of the slice at channel c0 into registers
+    const auto issue = [&](int c0) {
 #pragma unroll
         for (int k = 0; k < kHalfPer; ++k) {
             const int e = threadIdx.x + 256 * k;
@@ -146,6 +147,13 @@ __global__ __launch_bounds__(256, COT == 32 ? 3 : 2) void k_conv3x3(ConvIn I, in
                 if (co < Cout) wv[k] = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + co) * Cin + c0 + q * 8);
             }
         }
+    };
+    // With PF (grids of at most one workgroup per CU, where latency rather
+    // than occupancy decides) the next slice's loads are in flight while the
+    // current slice's MFMAs run; the registers this takes are free there.
+    if (PF) issue(0);
+    for (int c0 = 0; c0 < Cin; c0 += 32) {
+        if (!PF) issue(c0);
 #pragma unroll
         for (int k = 0; k < kHalfPer; ++k) {
             const int e = threadIdx.x + 256 * k;
@@ -155,6 +163,10 @@ __global__ __launch_bounds__(256, COT == 32 ? 3 : 2) void k_conv3x3(ConvIn I, in
         for (int k = 0; k < kWPer; ++k) {
             const int e = threadIdx.x + 256 * k;
             if (e < kWCh) *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = wv[k];
+        }
+        if (PF && c0 + 32 < Cin) {
+            issue(c0 + 32);
+            __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the taps
         }
         __syncthreads();
 #pragma unroll 1
@@ -666,16 +678,37 @@ Acts carve_acts(void *base, long long n0) {
     return a;
 }
 
+// Workgroups of a convolution up to which the prefetching kernel (one
+// workgroup per CU) is used: the device's CUs.
+int conv_prefetch_limit(s3dg_cnn *c) {
+    static thread_local int cached_device = -1, cached_cus = 0;
+    if (cached_device != c->ctx->device) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->ctx->device) != hipSuccess) cus = 0;
+        cached_device = c->ctx->device;
+        cached_cus = cus;
+    }
+    return cached_cus;
+}
+
 void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
           bf16 *out) {
     hipStream_t s = c->ctx->stream;
     const unsigned tiles = (unsigned)(((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile));
+    const int r = relu ? 1 : 0;
+    const long long limit = conv_prefetch_limit(c);
     if (Cout % 64 == 0) {
         dim3 grid(tiles, N * (Cout / 64));
-        hipLaunchKernelGGL(k_conv3x3<64>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+        if ((long long)grid.x * grid.y <= limit)
+            hipLaunchKernelGGL((k_conv3x3<64, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
+        else
+            hipLaunchKernelGGL((k_conv3x3<64, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
     } else {
         dim3 grid(tiles, N * ((Cout + 31) / 32));
-        hipLaunchKernelGGL(k_conv3x3<32>, grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, relu ? 1 : 0, out);
+        if ((long long)grid.x * grid.y <= limit)
+            hipLaunchKernelGGL((k_conv3x3<32, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
+        else
+            hipLaunchKernelGGL((k_conv3x3<32, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
     }
 }
 

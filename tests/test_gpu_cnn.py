@@ -125,3 +125,18 @@ def test_inference_graph_replays_match_direct_launches(ctx):
         ctx.free(d_f)
         ctx.free(d_t)
         ctx.free(d_o)
+
+
+def test_batched_inference_equals_single_images(ctx):
+    """Images are independent: a batch of two gives each image's own result
+    bit for bit.  At this size one image's full-resolution convolutions fit
+    one workgroup per CU (the kernel that prefetches the next channel slice)
+    and the batch's do not (the occupancy-oriented kernel), so the two
+    kernels are checked against each other."""
+    import salty_amd
+    net = salty_amd.Denoiser(ctx, seed=6)
+    a, _ = case(11, n=1, h=240, w=256)   # 15 x 16 = 240 tiles of 16 x 16
+    b, _ = case(12, n=1, h=240, w=256)
+    both = net.infer(np.concatenate([a, b]))
+    assert np.array_equal(both[0:1].view(np.uint32), net.infer(a).view(np.uint32))
+    assert np.array_equal(both[1:2].view(np.uint32), net.infer(b).view(np.uint32))
