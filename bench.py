@@ -66,10 +66,11 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-denoise", action="store_true",
                    help="skip the time-to-relMSE comparison (learned denoiser vs plain path tracing)")
-    p.add_argument("--workload", choices=("mesh", "soup", "unet", "batch"), default="mesh",
+    p.add_argument("--workload", choices=("mesh", "soup", "unet", "batch", "denoise"), default="mesh",
                    help="mesh: the path-traced mesh scene (default); soup: closest/any-hit rays against a "
                         "triangle soup; unet: denoiser inference at 1920x1080 and training steps; batch: 64 "
-                        "uneven scenes rendered together")
+                        "uneven scenes rendered together; denoise: only the time-to-relMSE comparison on the "
+                        "Cornell box (part of the default line)")
     p.add_argument("--soup-tris", type=int, default=4 << 20)
     p.add_argument("--soup-rays", type=int, default=16 << 20)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -332,6 +333,31 @@ def time_to_relmse_leg(ctx):
     return out
 
 
+def denoise_main(args):
+    """Only the time-to-relMSE comparison: speedup of path tracing + guides +
+    U-Net over plain path tracing in reaching relMSE 2e-3 (device time)."""
+    import salty_amd
+    ctx = salty_amd.Context(0)
+    t = time_to_relmse_leg(ctx)
+    line = {
+        "metric": "time to relMSE 2e-3, learned denoiser vs plain path tracing (speedup), 1 GPU",
+        "value": t.get("speedup"),
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": t.get("denoised_ms"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+bf16",
+        "data": "synthetic (procedural scene, trained weights in salty_amd/weights)",
+        "config": {"workload": t.get("scene"), "time_to_relmse": t},
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
 def batch_main(args):
     """64 seeded scenes (1k-500k triangles, log-uniform), each at a resolution
     from {320x240, 640x480, 1280x720, 1920x1080} and 4, 16 or 64 spp, depth 8,
@@ -422,6 +448,12 @@ def main():
             print("bench.py runs on one GPU only", file=sys.stderr)
             sys.exit(2)
         unet_main(args)
+        return
+    if args.workload == "denoise":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        denoise_main(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the path shards by pixel tiles x "
