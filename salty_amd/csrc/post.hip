@@ -145,9 +145,15 @@ __global__ __launch_bounds__(256) void k_tonemap(int32_t type, const float *__re
 constexpr int kTile = 16;
 constexpr int kMaxHalo = 12;   // search radius + patch radius
 
+// RP >= 0: the patch radius is RP, and the centre patch's (2 RP + 1)^2 texels
+// are held in registers across the search offsets, halving the LDS reads
+// (the sums are formed in the same order, so the result is the same);
+// RP < 0: the patch radius is the argument rp.
+template <int RP>
 __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, float frac_h2, float sigma2, int32_t rs,
-                                                      int32_t rp, const float4 *__restrict__ src,
+                                                      int32_t rp_arg, const float4 *__restrict__ src,
                                                       float4 *__restrict__ dst) {
+    const int rp = RP >= 0 ? RP : rp_arg;
     __shared__ float4 tile[(kTile + 2 * kMaxHalo) * (kTile + 2 * kMaxHalo)];
     const int halo = rs + rp;
     const int tw = kTile + 2 * halo;
@@ -163,17 +169,38 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
     const int x = blockIdx.x * kTile + lx, y = blockIdx.y * kTile + ly;
     if (x >= W || y >= H) return;
     const int cx = lx + halo, cy = ly + halo;   // pixel position inside the tile
+    constexpr int PS = RP >= 0 ? 2 * RP + 1 : 1;
+    float4 ctr[PS][PS];
+    if (RP >= 0) {
+#pragma unroll
+        for (int px = 0; px < PS; ++px)
+#pragma unroll
+            for (int py = 0; py < PS; ++py) ctr[px][py] = tile[(cy + py - RP) * tw + (cx + px - RP)];
+    }
     float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float sw = 0.0f;
     for (int ox = -rs; ox <= rs; ++ox) {
         for (int oy = -rs; oy <= rs; ++oy) {
             float acc = 0.0f;
-            for (int px = -rp; px <= rp; ++px) {
-                for (int py = -rp; py <= rp; ++py) {
-                    const float4 a = tile[(cy + py) * tw + (cx + px)];
-                    const float4 b = tile[(cy + oy + py) * tw + (cx + ox + px)];
-                    const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
-                    acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
+            if (RP >= 0) {
+#pragma unroll
+                for (int px = 0; px < PS; ++px) {
+#pragma unroll
+                    for (int py = 0; py < PS; ++py) {
+                        const float4 a = ctr[px][py];
+                        const float4 b = tile[(cy + oy + py - RP) * tw + (cx + ox + px - RP)];
+                        const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+                        acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
+                    }
+                }
+            } else {
+                for (int px = -rp; px <= rp; ++px) {
+                    for (int py = -rp; py <= rp; ++py) {
+                        const float4 a = tile[(cy + py) * tw + (cx + px)];
+                        const float4 b = tile[(cy + oy + py) * tw + (cx + ox + px)];
+                        const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+                        acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
+                    }
                 }
             }
             const float arg = -fmaxf(acc - 2.0f * sigma2, 0.0f) * frac_h2;
@@ -245,8 +272,13 @@ extern "C" int s3dg_nlm_device(s3dg_context *ctx, int32_t w, int32_t h, float co
     const float frac_h2 = 1.0f / (param_h * param_h);
     const float sigma2 = sigma * sigma;
     dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_nlm, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp,
-                       (const float4 *)d_src, (float4 *)d_dst);
+    // the reference's patch radius (2) has its own instantiation
+    if (rp == 2)
+        hipLaunchKernelGGL(k_nlm<2>, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp,
+                           (const float4 *)d_src, (float4 *)d_dst);
+    else
+        hipLaunchKernelGGL(k_nlm<-1>, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp,
+                           (const float4 *)d_src, (float4 *)d_dst);
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
 }
