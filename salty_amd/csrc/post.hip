@@ -149,11 +149,12 @@ constexpr int kMaxHalo = 12;   // search radius + patch radius
 // are held in registers across the search offsets, halving the LDS reads
 // (the sums are formed in the same order, so the result is the same);
 // RP < 0: the patch radius is the argument rp.
-template <int RP>
-__global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, float frac_h2, float sigma2, int32_t rs,
+template <int RP, int RS>
+__global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, float frac_h2, float sigma2, int32_t rs_arg,
                                                       int32_t rp_arg, const float4 *__restrict__ src,
                                                       float4 *__restrict__ dst) {
     const int rp = RP >= 0 ? RP : rp_arg;
+    const int rs = RS >= 0 ? RS : rs_arg;
     __shared__ float4 tile[(kTile + 2 * kMaxHalo) * (kTile + 2 * kMaxHalo)];
     const int halo = rs + rp;
     const int tw = kTile + 2 * halo;
@@ -180,6 +181,9 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
     float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float sw = 0.0f;
     for (int ox = -rs; ox <= rs; ++ox) {
+        // with RS known the offsets of one column are unrolled: a window row
+        // read for one offset is read once for all the offsets that use it
+#pragma unroll
         for (int oy = -rs; oy <= rs; ++oy) {
             float acc = 0.0f;
             if (RP >= 0) {
@@ -272,13 +276,19 @@ extern "C" int s3dg_nlm_device(s3dg_context *ctx, int32_t w, int32_t h, float co
     const float frac_h2 = 1.0f / (param_h * param_h);
     const float sigma2 = sigma * sigma;
     dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
-    // the reference's patch radius (2) has its own instantiation
-    if (rp == 2)
-        hipLaunchKernelGGL(k_nlm<2>, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp,
-                           (const float4 *)d_src, (float4 *)d_dst);
+    // the reference's patch radius (2) and search radii (6 there, 7 in the
+    // denoiser comparison) have their own instantiations
+    const float4 *s4 = (const float4 *)d_src;
+    float4 *d4 = (float4 *)d_dst;
+    const dim3 blk(kTile * kTile);
+    if (rp == 2 && rs == 6)
+        hipLaunchKernelGGL((k_nlm<2, 6>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
+    else if (rp == 2 && rs == 7)
+        hipLaunchKernelGGL((k_nlm<2, 7>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
+    else if (rp == 2)
+        hipLaunchKernelGGL((k_nlm<2, -1>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
     else
-        hipLaunchKernelGGL(k_nlm<-1>, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp,
-                           (const float4 *)d_src, (float4 *)d_dst);
+        hipLaunchKernelGGL((k_nlm<-1, -1>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
 }
