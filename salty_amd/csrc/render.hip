@@ -12,7 +12,9 @@
 // shadow queue) and shadow (visibility of the light sample, then the NEE term).
 // The order in which a path adds emission, NEE and miss radiance to L is the
 // reference's, and the passes are added to the accumulator in pass order, so
-// the sum is the same whatever the number of passes in flight.
+// the sum is the same whatever the number of passes in flight.  Renders of
+// few paths run the same per-path code in one launch instead (k_paths: one
+// thread carries a path through all its bounces), with the same sums.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
