@@ -165,8 +165,28 @@ def soup_main(args):
     scene.trace_counted_device(d_rays, n, abi.TRACE_CLOSEST, d_hits, d_cnt)
     cnt = ctx.download(d_cnt, (2,), np.uint64)
     nodes, tris = float(cnt[0]) / n, float(cnt[1]) / n
-    res = {}
+    res, picked = {}, {}
+    base = ctx.get_tuning()
     for mode, name in ((abi.TRACE_CLOSEST, "closest"), (abi.TRACE_ANY, "any")):
+        # the refill threshold is the traversal setting that matters for a
+        # batch of incoherent rays: one timed trace per candidate, the
+        # fastest kept (results do not depend on it)
+        if not args.no_autotune:
+            best = None
+            for refill in (16, 24, 32, 48):
+                ctx.set_tuning(abi.Tuning(base.traversal_variant, refill, base.tri_batch, base.passes_in_flight,
+                                          base.fetch_chunk, base.leaf_size))
+                scene.trace_device(d_rays, n, mode, d_hits)
+                ctx.synchronize()
+                t = time.perf_counter()
+                scene.trace_device(d_rays, n, mode, d_hits)
+                ctx.synchronize()
+                dt = time.perf_counter() - t
+                if best is None or dt < best[0]:
+                    best = (dt, refill)
+            ctx.set_tuning(abi.Tuning(base.traversal_variant, best[1], base.tri_batch, base.passes_in_flight,
+                                      base.fetch_chunk, base.leaf_size))
+            picked[name] = best[1]
         for _ in range(args.warmup):
             scene.trace_device(d_rays, n, mode, d_hits)
         t = time.perf_counter()
@@ -191,7 +211,8 @@ def soup_main(args):
         "config": {"workload": f"triangle soup, {args.soup_tris} triangles, {n} rays, tmin 1e-4, tmax inf",
                    "any_hit_grays_per_s": n / res["any"] / 1e9, "any_hit_ms": res["any"] * 1e3,
                    "scene_build_s": build_s, "generate_s": gen_s,
-                   "device_scene_bytes": scene.stats().device_bytes},
+                   "device_scene_bytes": scene.stats().device_bytes,
+                   "refill": picked or None},
         "roofline": {"kernel": "k_trace<closest> (batch traversal of the compressed BVH8)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_ray": bytes_per_ray, "nodes_per_ray": nodes,
