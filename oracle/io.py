@@ -67,6 +67,8 @@ def _old_line(r: _Reader, line: list, pos: int, count: int) -> None:
         if px[0] == 1 and px[1] == 1 and px[2] == 1:
             if pos == 0:
                 raise ValueError("run before the first texel")
+            if shift >= 32:
+                raise ValueError("run count past 32 bits")
             rep = px[3] << shift
             if pos + rep > count:
                 raise ValueError("run past the line")

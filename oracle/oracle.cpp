@@ -314,7 +314,7 @@ struct Builder {
             bcount[idx]++;
             bbox[idx] = merge(bbox[idx], tris[pp[i]].box);
         }
-        float cost[15];
+        float cost[15] = {};
         for (int i = 0; i < buckets - 1; ++i) {
             Box b0, b1;
             int c0 = 0, c1 = 0;

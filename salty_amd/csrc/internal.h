@@ -67,6 +67,9 @@ struct SceneDev {
     uint32_t tri_batch;     // lanes with pending triangles that keep a step testing
 };
 
+// Words of Workspace::graph_key.
+constexpr int kGraphKeyWords = 12;
+
 // Device buffers of the traversal and render kernels, owned by a scene or a
 // scene set and sized for the largest launch so far.
 struct Workspace {
@@ -78,7 +81,7 @@ struct Workspace {
     void *d_batch = nullptr;        // batch descriptor and segment table of the render
     size_t batch_bytes = 0;
     hipGraphExec_t graph = nullptr; // captured bounce loop
-    uint64_t graph_key[10] = {};     // launch parameters the graph was captured with
+    uint64_t graph_key[kGraphKeyWords] = {};   // launch parameters the graph was captured with
 };
 
 struct SceneMeshInfo {

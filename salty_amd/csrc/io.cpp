@@ -111,6 +111,10 @@ bool hdr_old(FILE *f, Rgbe *line, int32_t pos, int32_t count) {
         if (std::feof(f) || std::ferror(f)) return false;
         if (px.v[0] == 1 && px.v[1] == 1 && px.v[2] == 1) {
             if (pos == 0) return false;   // nothing to repeat
+            // a fifth consecutive repeat code would count in units of 2^32
+            // texels, more than any line holds (the reference shifts past
+            // the width of its int here)
+            if (shift >= 32) return false;
             const int64_t rep = (int64_t)px.v[3] << shift;
             if (pos + rep > count) return false;
             for (int64_t i = 0; i < rep; ++i, ++pos) line[pos] = line[pos - 1];

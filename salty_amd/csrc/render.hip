@@ -80,6 +80,24 @@ PathBuf carve(void *base, uint32_t n) {
     return b;
 }
 
+// HIP events owned for the duration of one call.
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    hipError_t create(size_t n) {
+        ev.reserve(n);
+        for (size_t i = 0; i < n; ++i) {
+            hipEvent_t e = nullptr;
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ev.push_back(e);
+        }
+        return hipSuccess;
+    }
+    ~EventSet() {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 __device__ __forceinline__ uint32_t scene_of(const PathBuf &P, uint32_t p) {
     return P.single_scene >= 0 ? (uint32_t)P.single_scene : P.scene[p];
 }
@@ -661,6 +679,7 @@ namespace s3dg {
 int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const RenderJob *jobs, size_t njobs,
                 int32_t max_bounce, int32_t rr_start, uint32_t flags, uint32_t passes_in_flight,
                 s3dg_render_stats *stats) {
+    if (njobs == 0 || !jobs) return set_error(ctx, S3DG_ERR_INVALID, "render of no frames");
     uint64_t total_paths = 0;
     uint32_t max_npix = 1;
     for (size_t j = 0; j < njobs; ++j) {
@@ -779,14 +798,12 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     const bool timed = (flags & S3DG_RENDER_TIME_KERNELS) != 0;
     const bool count = (flags & S3DG_RENDER_COUNT_WORK) != 0;
     const size_t launches = batches.size() * (size_t)max_bounce;
-    std::vector<hipEvent_t> ev;
-    if (timed) {
-        ev.resize(4 * launches);
-        for (auto &e : ev) S3DG_HIP(ctx, hipEventCreate(&e));
-    }
-    hipEvent_t e0, e1;
-    S3DG_HIP(ctx, hipEventCreate(&e0));
-    S3DG_HIP(ctx, hipEventCreate(&e1));
+    // events: [0], [1] around the render, then four per bounce launch when timed
+    // (destroyed on every return path)
+    EventSet events;
+    S3DG_HIP(ctx, events.create(2 + (timed ? 4 * launches : 0)));
+    const hipEvent_t e0 = events.ev[0], e1 = events.ev[1];
+    hipEvent_t *ev = events.ev.data() + 2;
     S3DG_HIP(ctx, hipEventRecord(e0, ctx->stream));
     hipLaunchKernelGGL(k_zero_tot, dim3(1), dim3(64), 0, ctx->stream, P);
 
@@ -831,13 +848,19 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     // read are rewritten before each replay).
     const bool use_graph = !timed && !count;
     if (use_graph) {
-        const uint64_t key[10] = {(uint64_t)(uintptr_t)ws.d_gstack, (uint64_t)(uint32_t)gcap,
-                                  (uint64_t)(uintptr_t)ws.d_render,
-                                 slots | ((uint64_t)(uint32_t)(P.single_scene + 1) << 32),
-                                 (uint64_t)(uint32_t)max_bounce | ((uint64_t)per_path << 32),
-                                 ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
-                                 (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
-                                 ((uint64_t)fetch.refill << 48) | ((uint64_t)fetch.chunk << 16) | Sx.tri_batch};
+        // every device address the captured launches bake in: the stack,
+        // path state, batch tables (d_desc and d_segs sit at fixed offsets in
+        // ws.d_batch, which a render needing more batches or segments
+        // reallocates), the error word and the scene arrays
+        const uint64_t key[kGraphKeyWords] = {(uint64_t)(uintptr_t)ws.d_gstack, (uint64_t)(uint32_t)gcap,
+                                              (uint64_t)(uintptr_t)ws.d_render,
+                                              slots | ((uint64_t)(uint32_t)(P.single_scene + 1) << 32),
+                                              (uint64_t)(uint32_t)max_bounce | ((uint64_t)per_path << 32),
+                                              ((uint64_t)ctx->variant << 32) | (uint32_t)rr_start, tgrid,
+                                              (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
+                                              ((uint64_t)fetch.refill << 48) | ((uint64_t)fetch.chunk << 16) |
+                                                  Sx.tri_batch,
+                                              (uint64_t)(uintptr_t)ws.d_batch, (uint64_t)(uintptr_t)ws.d_error};
         if (!ws.graph || memcmp(key, ws.graph_key, sizeof(key)) != 0) {
             if (ws.graph) {
                 (void)hipGraphExecDestroy(ws.graph);
@@ -878,10 +901,7 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
             ext_ms += a;
             sh_ms += b;
         }
-        for (auto &e : ev) (void)hipEventDestroy(e);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->extension_rays = tot[0];

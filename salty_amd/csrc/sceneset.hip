@@ -264,6 +264,12 @@ extern "C" int s3dg_render_batch_device(s3dg_scene_set *set, uint32_t njobs, con
                                         const s3dg_render_config *common, s3dg_render_stats *stats) {
     if (!set || !common || (njobs && !jobs)) return S3DG_ERR_INVALID;
     s3dg_context *ctx = set->ctx;
+    if (common->max_bounce_count < 0) return set_error(ctx, S3DG_ERR_INVALID, "bad bounce count");
+    if (njobs == 0) {
+        // an empty batch renders nothing
+        if (stats) memset(stats, 0, sizeof(*stats));
+        return S3DG_OK;
+    }
     S3DG_HIP(ctx, hipSetDevice(ctx->device));
     std::vector<RenderJob> rj(njobs);
     for (uint32_t j = 0; j < njobs; ++j) {

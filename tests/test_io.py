@@ -161,6 +161,14 @@ def test_hdr_rejects(tmp_path):
     open(f, "wb").write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 2\n\x05\x05\x05\x80")
     with pytest.raises(salty_amd.S3dgError):
         salty_amd.load_hdr(f)
+    # nine consecutive zero-length repeat codes: the run count would be
+    # shifted by 32 and more bits
+    open(f, "wb").write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 1 +X 2\n\x05\x05\x05\x80"
+                        + b"\x01\x01\x01\x00" * 9 + b"\x06\x06\x06\x80")
+    with pytest.raises(salty_amd.S3dgError):
+        salty_amd.load_hdr(f)
+    with pytest.raises(ValueError):
+        OIO.read_hdr(f)
 
 
 # ---------------------------------------------------------------- SMD ------
