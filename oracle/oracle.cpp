@@ -8,6 +8,17 @@
 // the reference root).  Written from the reference's observable semantics; the
 // data structures are index-based rather than the reference's virtual IShape
 // objects, which changes nothing that a caller can observe.
+//
+// Pins (tests/test_oracle_*.py): published PCG32 and Philox4x32-10 vectors;
+// closed forms and float64 recomputation for the triangle and sphere tests,
+// cameras, textures, materials, tonemap operators (hand-evaluated values with
+// the reference's constants) and NLM; hand-worked SAH trees (bucket choice,
+// the untruncated partition predicate, first-minimum ties, std::partition's
+// swap order, the BVH8 -> BVH4 fall-back, an eight-child node) plus tree
+// invariants; brute force for the traversal; rigid and scaled instances
+// against pre-transformed meshes.  Parity unpinned: outputs that depend on
+// the C runtime's cosf/sinf/powf/expf/logf (MSVC for the reference, glibc
+// here), which are compared with tolerances.
 #include "oracle.h"
 
 #include <algorithm>
