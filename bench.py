@@ -6,10 +6,15 @@ spherical light) rendered at 1920x1080, 64 samples per pixel, max depth 12,
 with the wavefront path tracer.  One step = one full 64-spp frame.
 
 Prints one JSON line: Grays/s (extension + shadow rays traced per second) as
-`value`, paths/s beside it, the roofline of the dominant kernel (closest-hit
-traversal) from HIP-event timings of every launch in the timed region, and the
-CPU oracle (the restatement of the reference's CPU path) timed on a bounded
-sample of the same frame on the host's cores.
+`value`, timed over whole frames whose bounce loops are replayed from the
+captured HIP graph (the library's default path); paths/s beside it; the
+roofline of the dominant kernel (closest-hit traversal) from HIP-event timings
+of every launch of one more, instrumented frame; the time to relMSE 2e-3 of the
+learned denoiser against plain path tracing; short runs of the other
+configurations under config.legs (4 M-triangle soup x 16 M rays closest and
+any hit, the 64-scene batch, the 1080p U-Net with NLM beside it); and the CPU
+oracle (the restatement of the reference's CPU path) timed on a bounded sample
+of the same frame on every host core the job may use.
 """
 from __future__ import annotations
 
@@ -64,6 +69,9 @@ def parse():
                    help="mesh workload materials: Lambert/mirror/glass, or all five (Phong and Plastic too)")
     p.add_argument("--lens-radius", type=float, default=0.0, help="mesh workload: thin-lens camera when > 0")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-legs", action="store_true",
+                   help="default workload only: skip the short triangle-soup, 64-scene batch and U-Net runs "
+                        "reported beside the headline")
     p.add_argument("--no-denoise", action="store_true",
                    help="skip the time-to-relMSE comparison (learned denoiser vs plain path tracing)")
     p.add_argument("--workload", choices=("mesh", "soup", "unet", "batch", "denoise"), default="mesh",
@@ -71,6 +79,9 @@ def parse():
                         "triangle soup; unet: denoiser inference at 1920x1080 and training steps; batch: 64 "
                         "uneven scenes rendered together; denoise: only the time-to-relMSE comparison on the "
                         "Cornell box (part of the default line)")
+    p.add_argument("--heldout", type=int, default=32,
+                   help="denoise workload: held-out scenes evaluated against NLM and plain 256-spp rendering")
+    p.add_argument("--gt-spp", type=int, default=65536, help="denoise workload: ground-truth samples per pixel")
     p.add_argument("--soup-tris", type=int, default=4 << 20)
     p.add_argument("--soup-rays", type=int, default=16 << 20)
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -91,12 +102,41 @@ def copy_calibration(mib: int) -> None:
     print(json.dumps({"copy_calibration_mib": mib, "checksum": float(b[::4096].sum().item())}), flush=True)
 
 
+def host_cpu():
+    """Host cores this process may run on and the CPU model.  The reference
+    runs its rows on every core it has (s3d_pt.cpp:311-321); the baseline uses
+    every core the job is given: the CPUs of its affinity mask, further
+    limited by OMP_NUM_THREADS when the environment sets it (a GPU box grants
+    one GPU's job a share of the host)."""
+    nproc = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = nproc
+    threads = allowed
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        threads = min(threads, int(omp))
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": max(1, threads), "nproc": nproc, "affinity_cpus": allowed,
+            "omp_num_threads": omp or None, "cpu_model": model}
+
+
 def cpu_baseline(desc, args):
     """Times the oracle (C++ restatement of PathTracer::Radiance with the
     reference traversal, OpenMP over rows as s3d_pt.cpp:311-313) on a band of
     rows at 1 spp, sized to about args.cpu_seconds of work."""
     from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    hc = host_cpu()
+    threads = hc["threads"]
     sc = O.OracleScene(desc)
     W, H = args.width, args.height
     mid = H // 2
@@ -123,6 +163,7 @@ def cpu_baseline(desc, args):
         "kind": "port",
         "sample": f"rows {r0}-{r0 + rows - 1} of the {W}x{H} frame, {spp} spp, depth {args.depth} "
                   f"({total_paths} paths, {total_rays} rays, {total_t:.1f} s on {threads} threads)",
+        "host": hc,
     }
 
 
@@ -130,7 +171,8 @@ def soup_cpu_baseline(desc, rays, seconds):
     """The oracle's reference traversal (no clipping, exact slab tests) on a
     prefix of the ray batch sized to about `seconds`, on the host's cores."""
     from oracle import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    hc = host_cpu()
+    threads = hc["threads"]
     mesh = O.OracleMesh(desc.meshes[0].vertices, desc.meshes[0].bvh_kind)
     t = time.perf_counter()
     mesh.trace(rays[:256], threads=threads)
@@ -140,14 +182,20 @@ def soup_cpu_baseline(desc, rays, seconds):
     mesh.trace(rays[:m], threads=threads)
     el = time.perf_counter() - t
     return {"value": m / el / 1e9, "unit": "Grays/s", "cores": threads, "kind": "port",
-            "sample": f"first {m} rays of the batch, closest hit ({el:.1f} s on {threads} threads)"}
+            "sample": f"first {m} rays of the batch, closest hit ({el:.1f} s on {threads} threads)",
+            "host": hc}
 
 
 def soup_main(args):
+    print(json.dumps(soup_run(args, args.steps, args.warmup, None if args.no_cpu_baseline else args.cpu_seconds)),
+          flush=True)
+
+
+def soup_run(args, steps, warmup, cpu_seconds):
     """Closest-hit and any-hit throughput of the batch traversal kernel on a
     triangle soup (v0 ~ U[0,1]^3, edges of length N(0.01, 0.003) in uniform
     directions) with rays from U[-0.1, 1.1]^3 in uniform directions,
-    tmin 1e-4, tmax infinity."""
+    tmin 1e-4, tmax infinity.  Returns the JSON line as a dict."""
     import salty_amd
     from salty_amd import abi, scenes
     t = time.perf_counter()
@@ -187,12 +235,14 @@ def soup_main(args):
             ctx.set_tuning(abi.Tuning(base.traversal_variant, best[1], base.tri_batch, base.passes_in_flight,
                                       base.fetch_chunk, base.leaf_size))
             picked[name] = best[1]
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             scene.trace_device(d_rays, n, mode, d_hits)
+        ctx.synchronize()
         t = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             scene.trace_device(d_rays, n, mode, d_hits)
-        res[name] = (time.perf_counter() - t) / args.steps
+        ctx.synchronize()
+        res[name] = (time.perf_counter() - t) / steps
     bytes_per_ray = nodes * NODE_BYTES + tris * TRI_BYTES + 32 + 16
     achieved = bytes_per_ray * n / res["closest"] / 1e9
     line = {
@@ -200,8 +250,8 @@ def soup_main(args):
         "value": n / res["closest"] / 1e9,
         "unit": "Grays/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": res["closest"] * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -218,11 +268,13 @@ def soup_main(args):
                      "traffic": None, "bytes_per_ray": bytes_per_ray, "nodes_per_ray": nodes,
                      "tris_per_ray": tris},
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = soup_cpu_baseline(desc, rays, args.cpu_seconds)
-    print(json.dumps(line), flush=True)
+    if cpu_seconds:
+        line["cpu_baseline"] = soup_cpu_baseline(desc, rays, cpu_seconds)
     for p in (d_rays, d_hits, d_cnt):
         ctx.free(p)
+    scene.close()
+    ctx.close()
+    return line
 
 
 BF16_PEAK_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
@@ -251,6 +303,11 @@ def unet_cpu_baseline(seconds):
 
 
 def unet_main(args):
+    print(json.dumps(unet_run(args, args.steps, args.warmup, None if args.no_cpu_baseline else args.cpu_seconds)),
+          flush=True)
+
+
+def unet_run(args, steps, warmup, cpu_seconds):
     """Denoiser inference on one 1920x1080 10-channel frame, training steps of
     16 x 256^2 crops, and NLM (search radius 7, patch radius 2) on the frame."""
     import salty_amd
@@ -264,14 +321,14 @@ def unet_main(args):
                            rng.normal(size=(1, H, W, 3)), rng.uniform(1, 5, (1, H, W, 1))], -1).astype(np.float32)
     d_f = ctx.upload(feat)
     d_o = ctx.malloc(H * W * 3 * 4)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         net.infer_device(1, H, W, d_f, d_o)
     ctx.synchronize()
     t = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         net.infer_device(1, H, W, d_f, d_o)
     ctx.synchronize()
-    inf_s = (time.perf_counter() - t) / args.steps
+    inf_s = (time.perf_counter() - t) / steps
     # NLM on the same frame (RGBA)
     rgba = np.concatenate([feat[0, ..., :3], np.ones((H, W, 1), np.float32)], -1)
     d_in = ctx.upload(rgba)
@@ -279,10 +336,10 @@ def unet_main(args):
     ctx.nlm_device(W, H, 0.2, 7, 2, d_in, d_nl)
     ctx.synchronize()
     t = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         ctx.nlm_device(W, H, 0.2, 7, 2, d_in, d_nl)
     ctx.synchronize()
-    nlm_s = (time.perf_counter() - t) / args.steps
+    nlm_s = (time.perf_counter() - t) / steps
     # training steps at 16 x 256^2
     B, S = 16, 256
     tf = np.ascontiguousarray(feat[0, :S, :S][None].repeat(B, 0))
@@ -291,10 +348,10 @@ def unet_main(args):
     net.train_step_device(B, S, S, d_tf, d_tt, 1e-6)
     ctx.synchronize()
     t = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         net.train_step_device(B, S, S, d_tf, d_tt, 1e-6)
     ctx.synchronize()
-    train_s = (time.perf_counter() - t) / args.steps
+    train_s = (time.perf_counter() - t) / steps
     fl = denoise.forward_flops(1, H, W)
     achieved = fl / inf_s / 1e12
     line = {
@@ -302,8 +359,8 @@ def unet_main(args):
         "value": achieved,
         "unit": "TFLOP/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": inf_s * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -319,11 +376,13 @@ def unet_main(args):
                      "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None},
     }
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = unet_cpu_baseline(min(args.cpu_seconds, 10.0))
-    print(json.dumps(line), flush=True)
+    if cpu_seconds:
+        line["cpu_baseline"] = unet_cpu_baseline(min(cpu_seconds, 10.0))
     for p in (d_f, d_o, d_in, d_nl, d_tf, d_tt):
         ctx.free(p)
+    net.close()
+    ctx.close()
+    return line
 
 
 def time_to_relmse_leg(ctx):
@@ -337,7 +396,7 @@ def time_to_relmse_leg(ctx):
         net.close()
         return {"error": "no trained weights"}
     r = denoise.time_to_relmse(ctx, net, scenes.cornell_box(), target=2e-3, width=256, height=256,
-                               gt_spp=65536, max_depth=8, nlm=False)
+                               gt_spp=65536, max_depth=8, nlm=True)
     net.close()
     pick = None
     for spp, ms, inf, rel in r["denoised"]:
@@ -351,15 +410,45 @@ def time_to_relmse_leg(ctx):
                    denoised_relmse=pick[3])
     if r["plain_ms"] and r["denoised_ms"]:
         out["speedup"] = r["plain_ms"] / r["denoised_ms"]
+    if r["nlm"]:
+        out["nlm_16spp"] = {"input_relmse": r["nlm"]["input_relmse"], "best_space": r["nlm"]["best"][0],
+                            "best_h": r["nlm"]["best"][1], "best_relmse": r["nlm"]["best"][2]}
     return out
 
 
+def heldout_leg(ctx, count, gt_spp):
+    """The learned denoiser on `count` held-out scenes (seeds 3000..) from 4
+    spp, against NLM (search radius 7, patch radius 2, best strength) on the
+    same input and plain 256-spp rendering, all scored against gt_spp renders."""
+    import salty_amd
+    from salty_amd import denoise
+    net = salty_amd.Denoiser(ctx, seed=0)
+    if not denoise.load_weights(net):
+        net.close()
+        return {"error": "no trained weights"}
+    rows = denoise.evaluate_heldout(ctx, net, range(3000, 3000 + count), gt_spp=gt_spp,
+                                    log=lambda r: print(json.dumps(r), file=sys.stderr, flush=True))
+    net.close()
+    keys = ("input_relmse", "learned_relmse", "nlm_relmse", "plain_relmse", "input_ms", "guides_ms",
+            "inference_ms", "nlm_ms", "plain_ms")
+    mean = {k: float(np.mean([r[k] for r in rows])) for k in keys}
+    return {"scenes": count, "seeds": [3000, 3000 + count - 1], "resolution": [256, 256], "input_spp": 4,
+            "plain_spp": 256, "ground_truth_spp": gt_spp, "nlm_radii": [7, 2], "mean": mean,
+            "learned_at_or_below_2e-3": sum(r["learned_relmse"] <= 2e-3 for r in rows),
+            "learned_beats_nlm": sum(r["learned_relmse"] < r["nlm_relmse"] for r in rows),
+            "learned_beats_plain_256spp": sum(r["learned_relmse"] < r["plain_relmse"] for r in rows),
+            "per_scene": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows]}
+
+
 def denoise_main(args):
-    """Only the time-to-relMSE comparison: speedup of path tracing + guides +
-    U-Net over plain path tracing in reaching relMSE 2e-3 (device time)."""
+    """The time-to-relMSE comparison (speedup of path tracing + guides +
+    U-Net over plain path tracing in reaching relMSE 2e-3, device time) and
+    the held-out evaluation against NLM and plain 256-spp rendering."""
     import salty_amd
     ctx = salty_amd.Context(0)
     t = time_to_relmse_leg(ctx)
+    if args.heldout:
+        t["heldout"] = heldout_leg(ctx, args.heldout, args.gt_spp)
     line = {
         "metric": "time to relMSE 2e-3, learned denoiser vs plain path tracing (speedup), 1 GPU",
         "value": t.get("speedup"),
@@ -380,6 +469,10 @@ def denoise_main(args):
 
 
 def batch_main(args):
+    print(json.dumps(batch_run(args, args.steps, args.warmup)), flush=True)
+
+
+def batch_run(args, steps, warmup):
     """64 seeded scenes (1k-500k triangles, log-uniform), each at a resolution
     from {320x240, 640x480, 1280x720, 1920x1080} and 4, 16 or 64 spp, depth 8,
     rendered as one batch: all frames' paths share every launch."""
@@ -402,32 +495,34 @@ def batch_main(args):
                                    flags=abi.RENDER_COUNT_WORK)
     ext_nodes = cst.extend_nodes / max(cst.extension_rays, 1)
     ext_tris = cst.extend_tris / max(cst.extension_rays, 1)
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         sset.render_batch_device(frames, depth)
     ctx.synchronize()
-    ext_ms = 0.0
-    launches = rays_ext = rays_sh = paths = 0
+    # timed steps: the graph-replayed bounce loops, as a caller runs them
+    rays_ext = rays_sh = paths = 0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = sset.render_batch_device(frames, depth, flags=abi.RENDER_TIME_KERNELS)
-        ext_ms += st.extend_ms
-        launches += st.extend_launches
+    for _ in range(steps):
+        st = sset.render_batch_device(frames, depth)
         rays_ext += st.extension_rays
         rays_sh += st.shadow_rays
         paths += st.paths
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
-    avg_ext_ms = ext_ms / max(launches, 1)
+    # kernel split from one more step with an event pair around every launch
+    kst = sset.render_batch_device(frames, depth, flags=abi.RENDER_TIME_KERNELS)
+    launches = kst.extend_launches
+    avg_ext_ms = kst.extend_ms / max(launches, 1)
     bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + EXT_IO_BYTES
-    achieved = bytes_per_ray * rays_ext / max(launches, 1) / (avg_ext_ms * 1e-3) / 1e9 if avg_ext_ms else 0.0
+    achieved = (bytes_per_ray * kst.extension_rays / max(launches, 1) / (avg_ext_ms * 1e-3) / 1e9
+                if avg_ext_ms else 0.0)
     line = {
         "metric": "Grays/s + paths/s, 64 uneven scenes rendered as one batch, 1 GPU",
         "value": (rays_ext + rays_sh) / elapsed / 1e9,
         "unit": "Grays/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed * 1e3 / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": elapsed * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -435,16 +530,20 @@ def batch_main(args):
         "data": "synthetic (seeded procedural scenes)",
         "config": {"workload": f"{len(cfg)} scenes, {tris} triangles in all, resolutions 320x240..1920x1080, "
                                f"4/16/64 spp, depth 8, one scene set",
-                   "paths_per_s": paths / elapsed, "paths_per_step": paths // max(args.steps, 1),
+                   "paths_per_s": paths / elapsed, "paths_per_step": paths // max(steps, 1),
                    "scene_build_s": build_s, "scene_generate_s": gen_s},
         "roofline": {"kernel": "k_extend (closest-hit traversal of the compressed BVH8)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_ms": avg_ext_ms, "launches": launches,
                      "bytes_per_ray": bytes_per_ray, "nodes_per_ray": ext_nodes, "tris_per_ray": ext_tris},
     }
-    print(json.dumps(line), flush=True)
     for d in bufs:
         ctx.free(d)
+    sset.close()
+    for sc in scs:
+        sc.close()
+    ctx.close()
+    return line
 
 
 def main():
@@ -514,27 +613,31 @@ def main():
         scene.render_device(cfg(args.spp, first_pass=0), d_accum)
     ctx.synchronize()
 
-    ext_ms = sh_ms = 0.0
-    launches = rays_ext = rays_sh = paths = 0
+    # timed steps: each a whole frame, its bounce loop replayed from the
+    # captured HIP graph as a caller runs it
+    rays_ext = rays_sh = paths = 0
+    device_ms = 0.0
     t0 = time.perf_counter()
     for s in range(args.steps):
-        st = scene.render_device(cfg(args.spp, abi.RENDER_TIME_KERNELS, first_pass=0), d_accum)
-        ext_ms += st.extend_ms
-        sh_ms += st.shadow_ms
-        launches += st.extend_launches
+        st = scene.render_device(cfg(args.spp, first_pass=0), d_accum)
+        device_ms += st.milliseconds
         rays_ext += st.extension_rays
         rays_sh += st.shadow_rays
         paths += st.paths
     ctx.synchronize()
     elapsed = time.perf_counter() - t0
+    # the kernel split: one more frame with a HIP event pair around every
+    # launch (on the context's stream, where the kernels run)
+    kst = scene.render_device(cfg(args.spp, abi.RENDER_TIME_KERNELS, first_pass=0), d_accum)
+    launches = kst.extend_launches
 
     rays = rays_ext + rays_sh
     grays = rays / elapsed / 1e9
     default_workload = (W, H, args.spp, args.depth, args.seed) == (1920, 1080, 64, 12, 0)
     traffic = extend_traffic_bytes() if default_workload else None
-    avg_ext_ms = ext_ms / max(launches, 1)
+    avg_ext_ms = kst.extend_ms / max(launches, 1)
     bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + EXT_IO_BYTES
-    bytes_per_launch = bytes_per_ray * rays_ext / max(launches, 1)
+    bytes_per_launch = bytes_per_ray * kst.extension_rays / max(launches, 1)
     achieved = bytes_per_launch / (avg_ext_ms * 1e-3) / 1e9 if avg_ext_ms > 0 else 0.0
 
     line = {
@@ -563,6 +666,9 @@ def main():
             "scene_generate_s": gen_s,
             "device_scene_bytes": sstats.device_bytes,
             "tuning": tuning,
+            "timed_path": "HIP-graph bounce loop (default render flags)",
+            "device_ms_per_step": device_ms / args.steps,
+            "instrumented_frame_ms": kst.milliseconds,
         },
         "roofline": {
             "kernel": "k_extend (closest-hit traversal of the compressed BVH8)",
@@ -578,15 +684,31 @@ def main():
             "bytes_per_ray": bytes_per_ray,
             "nodes_per_ray": ext_nodes,
             "tris_per_ray": ext_tris,
-            "shadow_kernel_ms_per_launch": sh_ms / max(launches, 1),
+            "shadow_kernel_ms_per_launch": kst.shadow_ms / max(launches, 1),
+            "timing": "HIP events around each k_extend launch of one instrumented frame",
         },
     }
     if not args.no_denoise:
         line["config"]["time_to_relmse"] = time_to_relmse_leg(ctx)
+    ctx.free(d_accum)
+    scene.close()
+    ctx.close()
+    if not args.no_legs:
+        # short runs of the other configurations, reported beside the headline
+        cpu = None if args.no_cpu_baseline else 5.0
+        legs = {}
+        for name, run in (("soup", lambda: soup_run(args, 2, 1, cpu)),
+                          ("batch", lambda: batch_run(args, 2, 1)),
+                          ("unet", lambda: unet_run(args, 5, 2, cpu))):
+            t = time.perf_counter()
+            leg = run()
+            legs[name] = {k: leg[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "config",
+                                              "roofline", "cpu_baseline") if k in leg}
+            legs[name]["wall_s"] = time.perf_counter() - t
+        line["config"]["legs"] = legs
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(desc, args)
     print(json.dumps(line), flush=True)
-    ctx.free(d_accum)
 
 
 if __name__ == "__main__":

@@ -265,15 +265,101 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65
     nl = None
     if nlm:
         img, ms = fr.mean(16)
-        rgba = np.concatenate([img, np.ones(img.shape[:2] + (1,), np.float32)], -1)
-        coeffs = []
-        for c in (0.05, 0.1, 0.2, 0.4):
-            t = time.perf_counter()
-            out = ctx.nlm(rgba, c, 7, 2)
-            coeffs.append((c, rel_mse(out[..., :3], gt), (time.perf_counter() - t) * 1e3))
-        nl = dict(spp=16, render_ms=ms, results=coeffs)
+        best, results = nlm_sweep(ctx, img, gt)
+        nl = dict(spp=16, render_ms=ms, input_relmse=rel_mse(img, gt), best=best, results=results)
     ctx.free(d_f)
     ctx.free(d_o)
     fr.close()
     return dict(gt_spp=gt_spp, gt_ms=gt_ms, plain=plain, plain_ms=plain_ms, denoised=den, denoised_ms=den_ms,
                 guide_ms=guide_ms, best_denoised_relmse=min(r for *_, r in den), nlm=nl)
+
+
+# NLM filter strengths tried on radiance (the reference's FilterNLM input; its
+# patch distance sums squared differences over 5 x 5 pixels and 4 channels, so
+# on path-traced radiance up to ~15 the useful h spans a wide range) and on
+# log(1 + radiance).
+NLM_H_LINEAR = (0.125, 0.25, 0.5, 1.0, 2.0, 4.0, 8.0, 16.0, 32.0)
+NLM_H_LOG = (0.0125, 0.025, 0.05, 0.1, 0.2, 0.4, 0.8)
+
+
+def nlm_sweep(ctx, img, gt, search_radius=7, patch_radius=2):
+    """NLM (s3dg_nlm, FilterNLM's weights) of a noisy mean image at every
+    strength of NLM_H_LINEAR on radiance and of NLM_H_LOG on log(1 +
+    radiance); returns (best (space, h, relMSE), [(space, h, relMSE)])."""
+    ones = np.ones(img.shape[:2] + (1,), np.float32)
+    results = []
+    rgba = np.concatenate([img, ones], -1)
+    for h in NLM_H_LINEAR:
+        out = ctx.nlm(rgba, h, search_radius, patch_radius)
+        results.append(("linear", h, rel_mse(out[..., :3], gt)))
+    lg = np.concatenate([np.log1p(np.maximum(img, 0)), ones], -1)
+    for h in NLM_H_LOG:
+        out = ctx.nlm(lg, h, search_radius, patch_radius)
+        results.append(("log", h, rel_mse(np.expm1(out[..., :3]), gt)))
+    best = min(results, key=lambda r: r[2])
+    return best, results
+
+
+def nlm_device_ms(ctx, width, height, search_radius=7, patch_radius=2, reps=3):
+    """Device time of one NLM pass at this size (host clock around the call
+    and a synchronisation, the best of reps after one untimed pass)."""
+    d_in = ctx.malloc(width * height * 16)
+    d_out = ctx.malloc(width * height * 16)
+    try:
+        ctx.copy_to_device(d_in, np.ones(width * height * 4, np.float32))
+        ctx.nlm_device(width, height, 1.0, search_radius, patch_radius, d_in, d_out)
+        ctx.synchronize()
+        best = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            ctx.nlm_device(width, height, 1.0, search_radius, patch_radius, d_in, d_out)
+            ctx.synchronize()
+            ms = (time.perf_counter() - t) * 1e3
+            best = ms if best is None else min(best, ms)
+        return best
+    finally:
+        ctx.free(d_in)
+        ctx.free(d_out)
+
+
+def evaluate_heldout(ctx, net, seeds, width=256, height=256, spp_in=4, gt_spp=65536, plain_spp=256, max_depth=8,
+                     log=None):
+    """The learned denoiser against NLM and plain rendering on held-out
+    Cornell-style scenes (seeds outside the training range): per scene the
+    relMSE against a gt_spp render of the spp_in-sample input, of the U-Net
+    on that input with the guides, of NLM on that input at its best strength
+    (nlm_sweep), and of a plain plain_spp render, with the device times of
+    each path."""
+    d_f = ctx.malloc(width * height * 10 * 4)
+    d_o = ctx.malloc(width * height * 3 * 4)
+    nlm_ms = nlm_device_ms(ctx, width, height)
+    rows = []
+    try:
+        for s in seeds:
+            fr = FrameRenderer(ctx, scenes.cornell_variant(int(s)), width, height, max_depth, seed=int(s))
+            gt, _ = fr.mean(gt_spp, first_pass=1 << 24)
+            a, n, d, guide_ms = fr.guides()
+            img, _ = fr.mean(spp_in)
+            in_ms = fr.render_ms(spp_in)
+            ctx.copy_to_device(d_f, features(img, a, n, d)[None])
+            net.infer_device(1, height, width, d_f, d_o)
+            ctx.synchronize()
+            t = time.perf_counter()
+            net.infer_device(1, height, width, d_f, d_o)
+            ctx.synchronize()
+            inf_ms = (time.perf_counter() - t) * 1e3
+            den = ctx.download(d_o, (height, width, 3), np.float32)
+            best, _ = nlm_sweep(ctx, img, gt)
+            plain, _ = fr.mean(plain_spp, first_pass=1 << 16)
+            plain_ms = fr.render_ms(plain_spp, first_pass=1 << 16)
+            fr.close()
+            row = dict(seed=int(s), input_relmse=rel_mse(img, gt), learned_relmse=rel_mse(den, gt),
+                       nlm_relmse=best[2], nlm_space=best[0], nlm_h=best[1], plain_relmse=rel_mse(plain, gt),
+                       input_ms=in_ms, guides_ms=guide_ms, inference_ms=inf_ms, nlm_ms=nlm_ms, plain_ms=plain_ms)
+            rows.append(row)
+            if log:
+                log(row)
+    finally:
+        ctx.free(d_f)
+        ctx.free(d_o)
+    return rows
