@@ -431,6 +431,18 @@ int s3dg_tonemap(s3dg_context *ctx, int32_t type, int32_t width, int32_t height,
 int s3dg_tonemap_device(s3dg_context *ctx, int32_t type, int32_t width, int32_t height,
                         const float *d_in_rgb, float *d_out_rgb, uint8_t *d_out_rgb8);
 
+/* s3dg_tonemap_device that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its reduction scratch (12,352 bytes: 1,024 double and 1,024 float partials
+ * plus the two results) is a file-scope device array, one per GPU for the
+ * life of the process, shared by every call on that GPU: calls that overlap
+ * on one GPU (other streams, other contexts) race.  Images up to 2^28 pixels;
+ * larger ones are rejected with S3DG_ERR_INVALID.  Results are identical to
+ * s3dg_tonemap_device's. */
+int s3dg_tonemap_device_noalloc(s3dg_context *ctx, int32_t type, int32_t width, int32_t height,
+                                const float *d_in_rgb, float *d_out_rgb, uint8_t *d_out_rgb8);
+
 /* Replaces FilterNLM(int w, int h, float coeff, const Color4*, Color4*)
  * (src/include/s3d_denoiser.h:16, src/src/s3d_denoiser.cpp:19-95).  The
  * reference fixes search radius 6 and patch radius 2; both are parameters

@@ -19,6 +19,8 @@ struct s3dg_context {
     // small reduction buffer (tonemap partial sums)
     void *small = nullptr;
     size_t small_bytes = 0;
+    // this device's copy of the file-scope tonemap scratch array (post.hip)
+    void *noalloc_scratch = nullptr;
     // traversal kernels hand new rays to a wave once this many of its lanes are idle
     uint32_t refill = 16;
     // a traversal step keeps testing triangles while this many lanes have some pending
@@ -146,6 +148,7 @@ int check_error_flag(s3dg_context *ctx, Workspace &ws);
 int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks);
 uint32_t batch_grid(uint32_t n);
 int hip_check(s3dg_context *ctx, hipError_t e, const char *what);
+int tonemap_scratch_init(s3dg_context *ctx);
 int ensure_buffer(s3dg_context *ctx, void **ptr, size_t *have, size_t need);
 
 constexpr int kBlock = 256;

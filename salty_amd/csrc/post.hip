@@ -4,6 +4,8 @@
 #include <float.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "internal.h"
 
@@ -12,6 +14,12 @@ using namespace s3dg;
 namespace {
 
 constexpr int kLumBlocks = 1024;
+// Largest image the tonemap entry points take (pixel indices times 3 stay
+// below 2^32).
+constexpr uint32_t kMaxTonemapPixels = 1u << 28;
+// Tonemap scratch: k_lum_partial's per-block log sums (double) and maxima
+// (float), then the log-average and maximum luminance k_lum_final writes.
+constexpr size_t kTonemapScratchBytes = kLumBlocks * (sizeof(double) + sizeof(float)) + 64;
 constexpr float kLumEps = 0.00001f;   // epsilon passed by every operator (s3d_tonemapper.cpp:248,291,331)
 
 // RGBToY with BT.601 weights (s3d_tonemapper.cpp:20,31-33)
@@ -220,19 +228,22 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
     dst[y * W + x] = make_float4(sum.x / sw, sum.y / sw, sum.z / sw, sum.w / sw);
 }
 
-}  // namespace
-
-extern "C" int s3dg_tonemap_device(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *d_in,
-                                   float *d_out, uint8_t *d_out8) {
+// Checks a tonemap call's arguments; returns the pixel count through n.
+int tonemap_args(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *d_in, uint32_t &n) {
     if (!ctx || !d_in || w <= 0 || h <= 0) return S3DG_ERR_INVALID;
     if (type < S3DG_TONEMAP_REINHARD || type > S3DG_TONEMAP_ACES)
         return set_error(ctx, S3DG_ERR_INVALID, "bad tonemap type");
-    S3DG_HIP(ctx, hipSetDevice(ctx->device));
-    const uint32_t n = (uint32_t)w * (uint32_t)h;
+    if ((uint64_t)w * (uint64_t)h > kMaxTonemapPixels)
+        return set_error(ctx, S3DG_ERR_INVALID, "tonemap image larger than 2^28 pixels");
+    n = (uint32_t)w * (uint32_t)h;
+    return S3DG_OK;
+}
+
+// The three tonemap launches over a scratch area of kTonemapScratchBytes.
+int launch_tonemap(s3dg_context *ctx, int32_t type, uint32_t n, const float *d_in, float *d_out, uint8_t *d_out8,
+                   void *scratch_area) {
     const int nb = (int)std::min<uint32_t>(kLumBlocks, (n + 255) / 256);
-    int rc = ensure_buffer(ctx, &ctx->small, &ctx->small_bytes, kLumBlocks * (sizeof(double) + sizeof(float)) + 64);
-    if (rc) return rc;
-    double *psum = (double *)ctx->small;
+    double *psum = (double *)scratch_area;
     float *pmax = (float *)(psum + nb);
     float *lum = pmax + nb;
     hipLaunchKernelGGL(k_lum_partial, dim3(nb), dim3(256), 0, ctx->stream, d_in, n, psum, pmax);
@@ -241,6 +252,61 @@ extern "C" int s3dg_tonemap_device(s3dg_context *ctx, int32_t type, int32_t w, i
     hipLaunchKernelGGL(k_tonemap, dim3(grid), dim3(256), 0, ctx->stream, type, d_in, n, lum, d_out, d_out8);
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
+}
+
+}  // namespace
+
+// Device scratch of s3dg_tonemap_device_noalloc, one copy per GPU for the
+// life of the process.  Host code never uses this name as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[kTonemapScratchBytes];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::vector<void *> g_scratch_addr;   // device address of `scratch`, by device
+}  // namespace
+
+namespace s3dg {
+
+// Looks up (once per device, with that device current) the address of this
+// device's copy of `scratch` and gives it to the context.
+int tonemap_scratch_init(s3dg_context *ctx) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_addr.size() <= (size_t)ctx->device) g_scratch_addr.resize((size_t)ctx->device + 1, nullptr);
+    if (!g_scratch_addr[ctx->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_check(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                                    "hipGetSymbolAddress(scratch)");
+        g_scratch_addr[ctx->device] = p;
+    }
+    ctx->noalloc_scratch = g_scratch_addr[ctx->device];
+    return S3DG_OK;
+}
+
+}  // namespace s3dg
+
+extern "C" int s3dg_tonemap_device(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *d_in,
+                                   float *d_out, uint8_t *d_out8) {
+    uint32_t n = 0;
+    int rc = tonemap_args(ctx, type, w, h, d_in, n);
+    if (rc) return rc;
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    rc = ensure_buffer(ctx, &ctx->small, &ctx->small_bytes, kTonemapScratchBytes);
+    if (rc) return rc;
+    return launch_tonemap(ctx, type, n, d_in, d_out, d_out8, ctx->small);
+}
+
+extern "C" int s3dg_tonemap_device_noalloc(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *d_in,
+                                           float *d_out, uint8_t *d_out8) {
+    uint32_t n = 0;
+    int rc = tonemap_args(ctx, type, w, h, d_in, n);
+    if (rc) return rc;
+    if (!ctx->noalloc_scratch) return set_error(ctx, S3DG_ERR_DEVICE, "device scratch address unknown");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    // every scratch word the kernels read is written first in the same call
+    // (partials of blocks 0..nb-1, then the two results), so nothing is cleared
+    return launch_tonemap(ctx, type, n, d_in, d_out, d_out8, ctx->noalloc_scratch);
 }
 
 extern "C" int s3dg_tonemap(s3dg_context *ctx, int32_t type, int32_t w, int32_t h, const float *in, float *out,

@@ -59,6 +59,11 @@ extern "C" int s3dg_create(s3dg_context **out, int device) {
         delete ctx;
         return S3DG_ERR_DEVICE;
     }
+    if (tonemap_scratch_init(ctx) != S3DG_OK) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return S3DG_ERR_DEVICE;
+    }
     *out = ctx;
     return S3DG_OK;
 }
