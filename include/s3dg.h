@@ -484,6 +484,67 @@ int s3dg_cnn_gradient_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, con
 int s3dg_cnn_train_step_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
                                const float *d_target_rgb, float learning_rate, double *loss);
 
+/* ---------------------------------------------------------------------------
+ * Per-object calls of the reference's host interfaces, evaluated on the
+ * device with the renderer's own device functions, for a C++ (or other) host
+ * layer that keeps those interfaces.  Host arrays in and out; one kernel per
+ * call; at most 2^24 queries per call.
+ * ------------------------------------------------------------------------- */
+
+/* One ShadingArg (src/include/s3d_material.h:21-29). */
+typedef struct s3dg_shade_arg {
+    float    input[3];      /* incoming direction                                 */
+    float    normal[3];     /* shading normal                                     */
+    float    texcoord[2];   /* where a TexturedMaterial samples its texture       */
+    uint64_t random;        /* PCG32 state (PCG::m_State), advanced by the draws  */
+    float    output[3];     /* out: sampled direction (ShadingArg::output)        */
+    float    weight[3];     /* out: Shade's return value / GetBaseColor           */
+    int32_t  dice;          /* out: ShadingArg::dice                              */
+    int32_t  pad;
+} s3dg_shade_arg;
+
+/* IMaterial::Shade (src/include/s3d_material.h:38; Lambert, Mirror, Glass,
+ * Phong, Plastic: src/src/s3d_{lambert,mirror,glass,phong,plastic}.cpp) for n
+ * shading events of one material; with a texture it is a TexturedMaterial
+ * (src/src/s3d_texturedmaterial.cpp:63-87).  material->texture is ignored. */
+int s3dg_shade(s3dg_context *ctx, const s3dg_material *material, const s3dg_texture_desc *texture,
+               uint32_t n, s3dg_shade_arg *args);
+/* IMaterial::GetBaseColor(texcoord) (src/include/s3d_material.h:40) into
+ * args[i].weight; only texcoord is read. */
+int s3dg_base_color(s3dg_context *ctx, const s3dg_material *material, const s3dg_texture_desc *texture,
+                    uint32_t n, s3dg_shade_arg *args);
+
+/* ICamera::GetRay(x, y) (src/include/s3d_camera.h:19-25, 95-100, 207-225):
+ * org_dir receives origin and direction (6 floats per ray) through the
+ * image-plane points xy (2 floats each, [-0.5, 0.5]).  A thin-lens camera
+ * draws its lens sample from random[i] (advanced); a pinhole camera takes
+ * random = NULL. */
+int s3dg_camera_rays(s3dg_context *ctx, const s3dg_camera_desc *camera, uint32_t n, const float *xy,
+                     uint64_t *random, float *org_dir);
+
+/* Sphere::Sample(PCG&, Vector3*, float*) (src/src/s3d_sphere.cpp:106-115). */
+int s3dg_sphere_sample(s3dg_context *ctx, const float center[3], float radius, uint32_t n, uint64_t *random,
+                       float *position, float *pdf);
+
+/* IShape::CalcParam (src/include/s3d_shape.h:66; Triangle
+ * src/src/s3d_triangle.cpp:107-121, Sphere src/src/s3d_sphere.cpp:94-104) of a
+ * hit on a shape of a scene: triangle = index in its mesh's input order (-2:
+ * the shape is a sphere, whose parameters follow from position).  texcoord is
+ * the blend of the vertex coordinates when the scene keeps them (any material
+ * textured), else 0, 0. */
+typedef struct s3dg_param_query {
+    int32_t shape;
+    int32_t triangle;
+    float   position[3];
+    float   barycentric[2];
+} s3dg_param_query;
+int s3dg_scene_calc_param(s3dg_scene *scene, uint32_t n, const s3dg_param_query *queries, float *normal,
+                          float *texcoord);
+
+/* Scene::SampleIBL(dir) (src/include/s3d_scene.h:80-82): the IBL texture's
+ * lat-long lookup times 10, or the constant environment. */
+int s3dg_scene_environment(s3dg_scene *scene, uint32_t n, const float *dirs, float *rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,12 @@ _SIGNATURES = {
     "s3dg_nlm": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
     "s3dg_nlm_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, C.c_int32, C.c_void_p,
                         C.c_void_p],
+    "s3dg_shade": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "s3dg_base_color": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p],
+    "s3dg_camera_rays": [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_sphere_sample": [C.c_void_p, C.c_void_p, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_scene_calc_param": [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "s3dg_scene_environment": [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p],
     "s3dg_cnn_create": [C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)],
     "s3dg_cnn_destroy": [C.c_void_p],
     "s3dg_cnn_param_count": [C.c_void_p, C.POINTER(C.c_uint64)],

@@ -21,7 +21,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libs3dg.so")
 
-SOURCES = ["bvh.cpp", "scene.cpp", "io.cpp", "trace.hip", "render.hip", "sceneset.hip", "post.hip", "cnn.hip"]
+SOURCES = ["bvh.cpp", "scene.cpp", "io.cpp", "trace.hip", "render.hip", "sceneset.hip", "post.hip", "cnn.hip",
+           "dropin.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -72,10 +73,11 @@ def build(jobs: int = 8) -> str:
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
-    # C++ drop-in example: plain g++ against the C ABI, no HIP headers.
+    # C++ drop-in programs: plain g++ against the C ABI, no HIP headers.
     host = os.path.join(HERE, "host")
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", host, os.path.join(host, "cornell_main.cpp"),
-          "-L", HERE, "-ls3dg", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(host, "s3d_cornell")])
+    for src, exe in (("cornell_main.cpp", "s3d_cornell"), ("iface_demo.cpp", "s3d_iface_demo")):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", host, os.path.join(host, src),
+              "-L", HERE, "-ls3dg", "-Wl,-rpath,$ORIGIN/..", "-o", os.path.join(host, exe)])
     return LIB
 
 

@@ -149,6 +149,9 @@ int resident_grid(s3dg_context *ctx, const void *kernel, uint32_t *blocks);
 uint32_t batch_grid(uint32_t n);
 int hip_check(s3dg_context *ctx, hipError_t e, const char *what);
 int tonemap_scratch_init(s3dg_context *ctx);
+// Device records built from the ABI descriptions (scene.cpp).
+GCamera make_camera(const s3dg_camera_desc &c);
+void make_material(const s3dg_material &m, GMat &g);
 int ensure_buffer(s3dg_context *ctx, void **ptr, size_t *have, size_t need);
 
 constexpr int kBlock = 256;

@@ -355,11 +355,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, 
             const GLight &lt = S.lights[li];
             V3 lpos;
             if (lt.type == LIGHT_SPHERE) {
-                // Sphere::Sample (s3d_sphere.cpp:106-115)
-                const float r1 = k2Pi * rng.f32();
-                const float r2 = 1.0f - 2.0f * rng.f32();
-                const float r3 = sqrtf(1.0f - r2 * r2);
-                lpos = ld3(lt.corner) + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (lt.e1[0] + 1e-1f);
+                lpos = sphere_sample_point(ld3(lt.corner), lt.e1[0], rng);
             } else {
                 const float u = rng.f32();
                 const float v = rng.f32();

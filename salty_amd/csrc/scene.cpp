@@ -154,6 +154,29 @@ bool inverse_double(const float mf[4][4], double inv[4][4]) {
     return true;
 }
 
+}  // namespace
+
+// PinholeCamera::Update / ThinLensCamera::Update (src/include/s3d_camera.h:63-93,
+// 171-205): the screen vectors in the reference's operation order.
+GCamera make_camera(const s3dg_camera_desc &c) {
+    GCamera g;
+    memset(&g, 0, sizeof(g));
+    const F3 pos = f3(c.position), target = f3(c.target), up = f3(c.upward);
+    const F3 dir = unit(sub(target, pos));
+    const F3 cx = scale(scale(unit(cross(dir, up)), c.fov), c.aspect);
+    const F3 cy = scale(unit(cross(dir, cx)), c.fov);
+    const F3 cz = add(pos, scale(dir, c.near_clip));
+    g.pos[0] = pos.x; g.pos[1] = pos.y; g.pos[2] = pos.z;
+    g.cx[0] = cx.x; g.cx[1] = cx.y; g.cx[2] = cx.z;
+    g.cy[0] = cy.x; g.cy[1] = cy.y; g.cy[2] = cy.z;
+    g.cz[0] = cz.x; g.cz[1] = cz.y; g.cz[2] = cz.z;
+    g.type = c.type;
+    g.lens_radius = c.lens_radius;
+    const F3 tp = sub(target, pos);
+    g.focal_distance = sqrtf((tp.x * tp.x) + (tp.y * tp.y) + (tp.z * tp.z));
+    return g;
+}
+
 void make_material(const s3dg_material &m, GMat &g) {
     memset(&g, 0, sizeof(g));
     for (int a = 0; a < 3; ++a) {
@@ -177,6 +200,8 @@ void make_material(const s3dg_material &m, GMat &g) {
     g.thr0 = t;
     g.thr1 = t;
 }
+
+namespace {
 
 float scale_margin(const Aabb &b) {
     float m = 0.0f;
@@ -541,24 +566,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
     }
 
     // ---- camera: PinholeCamera::Update / ThinLensCamera::Update ----------
-    {
-        const s3dg_camera_desc &c = d->camera;
-        GCamera &g = sc->rec.cam;
-        memset(&g, 0, sizeof(g));
-        const F3 pos = f3(c.position), target = f3(c.target), up = f3(c.upward);
-        const F3 dir = unit(sub(target, pos));
-        const F3 cx = scale(scale(unit(cross(dir, up)), c.fov), c.aspect);
-        const F3 cy = scale(unit(cross(dir, cx)), c.fov);
-        const F3 cz = add(pos, scale(dir, c.near_clip));
-        g.pos[0] = pos.x; g.pos[1] = pos.y; g.pos[2] = pos.z;
-        g.cx[0] = cx.x; g.cx[1] = cx.y; g.cx[2] = cx.z;
-        g.cy[0] = cy.x; g.cy[1] = cy.y; g.cy[2] = cy.z;
-        g.cz[0] = cz.x; g.cz[1] = cz.y; g.cz[2] = cz.z;
-        g.type = c.type;
-        g.lens_radius = c.lens_radius;
-        const F3 tp = sub(target, pos);
-        g.focal_distance = sqrtf((tp.x * tp.x) + (tp.y * tp.y) + (tp.z * tp.z));
-    }
+    sc->rec.cam = make_camera(d->camera);
     for (int a = 0; a < 3; ++a) sc->rec.env[a] = d->environment[a];
     sc->rec.env_tex = d->environment_texture - 1;
     sc->rec.light_count = (int32_t)d->light_count;

@@ -295,6 +295,16 @@ __device__ __forceinline__ void camera_ray(const GCamera &c, float x, float y, P
     }
 }
 
+// Sphere::Sample (src/src/s3d_sphere.cpp:106-115): a point uniform on the
+// sphere of radius + 0.1 about the centre, and the pdf 1 / (4 pi r^2).
+__device__ __forceinline__ V3 sphere_sample_point(V3 center, float radius, Pcg &rng) {
+    const float r1 = k2Pi * rng.f32();
+    const float r2 = 1.0f - 2.0f * rng.f32();
+    const float r3 = sqrtf(1.0f - r2 * r2);
+    return center + safe_unit3(v3(r3 * cosf(r1), r3 * sinf(r1), r2)) * (radius + 1e-1f);
+}
+__device__ __forceinline__ float sphere_sample_pdf(float radius) { return 1.0f / (4.0f * kPi * radius * radius); }
+
 // Triangle::CalcParam normal (src/src/s3d_triangle.cpp:107-121) + SafeNormalize
 __device__ __forceinline__ V3 tri_normal(const GTriShade &s, float beta, float gamma) {
     const float alpha = 1.0f - beta - gamma;
