@@ -603,6 +603,7 @@ def main():
     cst = scene.render_device(cfg(2, abi.RENDER_COUNT_WORK, first_pass=1000), d_accum)
     ext_nodes = cst.extend_nodes / max(cst.extension_rays, 1)
     ext_tris = cst.extend_tris / max(cst.extension_rays, 1)
+    ext_spills = cst.extend_stack_spills / max(cst.extension_rays, 1)
 
     tuning = None
     if not args.no_autotune:
@@ -684,6 +685,7 @@ def main():
             "bytes_per_ray": bytes_per_ray,
             "nodes_per_ray": ext_nodes,
             "tris_per_ray": ext_tris,
+            "global_stack_pushes_per_ray": ext_spills,
             "shadow_kernel_ms_per_launch": kst.shadow_ms / max(launches, 1),
             "timing": "HIP events around each k_extend launch of one instrumented frame",
         },

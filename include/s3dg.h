@@ -231,6 +231,8 @@ typedef struct s3dg_render_stats {
     uint64_t extend_tris;      /* COUNT_WORK: triangles tested                 */
     uint64_t shadow_nodes;
     uint64_t shadow_tris;
+    uint64_t extend_stack_spills;   /* COUNT_WORK: traversal stack pushes beyond the */
+    uint64_t shadow_stack_spills;   /* entries kept in LDS (to global memory)        */
 } s3dg_render_stats;
 
 typedef struct s3dg_scene_stats {

@@ -110,7 +110,8 @@ class RenderStats(C.Structure):
                 ("milliseconds", C.c_double), ("extend_ms", C.c_double), ("shadow_ms", C.c_double),
                 ("extend_launches", C.c_uint64), ("shadow_launches", C.c_uint64),
                 ("extend_nodes", C.c_uint64), ("extend_tris", C.c_uint64),
-                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64)]
+                ("shadow_nodes", C.c_uint64), ("shadow_tris", C.c_uint64),
+                ("extend_stack_spills", C.c_uint64), ("shadow_stack_spills", C.c_uint64)]
 
 
 class SceneStats(C.Structure):

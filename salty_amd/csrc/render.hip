@@ -31,28 +31,37 @@ using namespace s3dg;
 
 namespace {
 
-// Path state, structure of arrays over N = width*height paths.
+// Path state over N path slots.  What lives from bounce to bounce (ray,
+// throughput, radiance, RNG, scene) is a structure of arrays by path slot;
+// what one bounce hands to the next kernel is kept in the order that kernel
+// reads it: the hit of queue entry i of the extension launch at hit[i], the
+// shadow ray of the path the shade launch visited i-th at shadow[3 i ...],
+// so that those records are written and read in whole lines.
 struct PathBuf {
     float *ox, *oy, *oz, *dx, *dy, *dz;     // current ray
     float *wr, *wg, *wb;                    // throughput W
     float *lr, *lg, *lb;                    // radiance L
     uint64_t *rng;                          // PCG state
-    float *ht, *hu, *hv;                    // extension hit
-    int32_t *hprim, *hshape;
-    float *sx, *sy, *sz;                    // shadow ray direction
-    float *stl;                             // distance to the light along it (< 0: light missed)
-    float *scr, *scg, *scb;                 // W * NEE, added to L when the light is visible
+    float4 *hit;                            // extension hit by queue position: t, u, v, prim (bits)
+    int32_t *hit_shape;                     // top-level shape of that hit
+    float4 *shadow;                         // shadow rays by shade position, three float4 each:
+                                            // origin + distance to the light (< 0: light missed),
+                                            // direction + path slot (bits), W * NEE (added to L
+                                            // when the light is visible)
     uint32_t *scene;                        // scene of the path (index into SceneDev::recs)
     int32_t single_scene;                   // >= 0: every path is in this scene (scene[] unused)
     uint32_t *q[2];                         // extension queues (ping-pong)
-    uint32_t *qs;                           // shadow queue
+    uint32_t *qs;                           // shadow queue (shade positions)
     uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count,
                                             // [4],[5] extend / shadow work-fetch cursors
-    unsigned long long *tot;                // [0] extension rays, [1] shadow rays
+    unsigned long long *tot;                // [0] extension rays, [1] shadow rays, [2..5] node visits
+                                            // and triangle tests (extend, shadow), [6], [7]
+                                            // stack pushes to global memory (extend, shadow)
 };
 
-// 25 four-byte fields + 1 eight-byte field per path and three queues
-constexpr size_t kPathBytes = 25 * 4 + 8 + 3 * 4;
+// bytes per path slot: 12 four-byte ray / W / L fields, the RNG, a 16-byte
+// hit and its shape, a 48-byte shadow record, the scene and three queues
+constexpr size_t kPathBytes = 12 * 4 + 8 + 16 + 4 + 48 + 4 + 3 * 4;
 
 size_t path_bytes(uint32_t n) { return (size_t)n * kPathBytes + 4096; }
 
@@ -67,11 +76,11 @@ PathBuf carve(void *base, uint32_t n) {
     b.cnt = (uint32_t *)take(64);
     b.tot = (unsigned long long *)take(64);
     b.rng = (uint64_t *)take((size_t)n * 8);
-    float **f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.wr, &b.wg, &b.wb, &b.lr, &b.lg, &b.lb,
-                   &b.ht, &b.hu, &b.hv, &b.sx, &b.sy, &b.sz, &b.stl, &b.scr, &b.scg, &b.scb};
+    float **f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.wr, &b.wg, &b.wb, &b.lr, &b.lg, &b.lb};
     for (float **x : f) *x = (float *)take((size_t)n * 4);
-    b.hprim = (int32_t *)take((size_t)n * 4);
-    b.hshape = (int32_t *)take((size_t)n * 4);
+    b.hit = (float4 *)take((size_t)n * 16);
+    b.hit_shape = (int32_t *)take((size_t)n * 4);
+    b.shadow = (float4 *)take((size_t)n * 48);
     b.scene = (uint32_t *)take((size_t)n * 4);
     b.q[0] = (uint32_t *)take((size_t)n * 4);
     b.q[1] = (uint32_t *)take((size_t)n * 4);
@@ -96,6 +105,13 @@ struct EventSet {
     ~EventSet() {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
+};
+
+// A shadow ray as k_shade hands it to k_shadow.
+struct ShadowRay {
+    V3 o, d;
+    float tl;    // distance to the light along d (< 0: the light was missed)
+    V3 c;        // W * NEE term, added to L when nothing is hit before the light
 };
 
 __device__ __forceinline__ uint32_t scene_of(const PathBuf &P, uint32_t p) {
@@ -193,9 +209,9 @@ struct StagedAppend2 {
         }
         __syncthreads();
     }
-    // Stages v in queue 0 if w0 and in queue 1 if w1 (at most kBlock entries
-    // per queue between two flush_if_full calls).
-    __device__ __forceinline__ void add(bool w0, bool w1, uint32_t v) {
+    // Stages v0 in queue 0 if w0 and v1 in queue 1 if w1 (at most kBlock
+    // entries per queue between two flush_if_full calls).
+    __device__ __forceinline__ void add(bool w0, uint32_t v0, bool w1, uint32_t v1) {
         const uint32_t lane = __lane_id();
         const uint64_t m0 = __ballot(w0), m1 = __ballot(w1);
         const uint64_t below = (1ull << lane) - 1ull;
@@ -206,8 +222,8 @@ struct StagedAppend2 {
         }
         s0 = __shfl(s0, 0) + (uint32_t)__popcll(m0 & below);
         s1 = __shfl(s1, 0) + (uint32_t)__popcll(m1 & below);
-        if (w0) buf[0][s0] = v;
-        if (w1) buf[1][s1] = v;
+        if (w0) buf[0][s0] = v0;
+        if (w1) buf[1][s1] = v1;
     }
 };
 
@@ -284,11 +300,12 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
     const uint32_t *q = P.q[cur];
     Tracer<false, COUNT> tr;
     tr.reset_counts();
-    uint32_t p = 0;
+    uint32_t qi = 0;
     refill_loop(
         &P.cnt[4], n, fetch,
         [&](uint32_t i) {
-            p = q[i];
+            qi = i;
+            const uint32_t p = q[i];
             DRay r;
             r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
             r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
@@ -298,24 +315,25 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
         },
         [&]() { return tr.step(S, st); },
         [&]() {
-            P.ht[p] = tr.best.t;
-            P.hu[p] = tr.best.u;
-            P.hv[p] = tr.best.v;
-            P.hprim[p] = tr.best.prim;
-            P.hshape[p] = tr.best.shape;
+            // by queue position: the lanes of a wave hold nearby entries
+            P.hit[qi] = make_float4(tr.best.t, tr.best.u, tr.best.v, __int_as_float(tr.best.prim));
+            P.hit_shape[qi] = tr.best.shape;
         });
     tr.flush(P.tot + 2);
+    if (COUNT) st.flush_spills(P.tot + 6);
 }
 
 // One path vertex (Radiance, s3d_pt.cpp:182-246, with the sampling half of
-// NextEventEstimation): emission or environment, the light sample and its NEE
-// term (kept for the shadow ray), the material sample and Russian roulette.
-// shadow: a shadow ray is queued; cont: the path goes on.
-__device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, uint32_t p, int32_t depth,
-                                           int32_t rr_start, bool &shadow, bool &cont) {
+// NextEventEstimation) at the closest hit h of the path's ray: emission or
+// environment, the light sample and its NEE term (returned as the shadow ray
+// sr), the material sample and Russian roulette.
+// shadow: sr is to be traced; cont: the path goes on.
+__device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, uint32_t p, const DHit &h,
+                                           int32_t depth, int32_t rr_start, bool &shadow, ShadowRay &sr,
+                                           bool &cont) {
     V3 W = v3(P.wr[p], P.wg[p], P.wb[p]);
     V3 Ladd;   // what this vertex adds to the path's radiance L
-    const int32_t prim = P.hprim[p];
+    const int32_t prim = h.prim;
     if (prim == -1) {
         // miss: L += W * SampleIBL(dir) (s3d_pt.cpp:199-203), or the
         // constant environment when the scene has no IBL texture
@@ -323,16 +341,16 @@ __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, 
     } else {
         const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
         const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
-        const float t = P.ht[p];
+        const float t = h.t;
         const V3 pos = o + d * t;
         V3 normal;
-        const GShapeRef sr = S.shape_ref[P.hshape[p]];
-        int32_t mat = sr.material;
+        const GShapeRef sref = S.shape_ref[h.shape];
+        int32_t mat = sref.material;
         if (prim == -2) {
-            normal = unit3(pos - ld3(S.insts[sr.inst].center));
+            normal = unit3(pos - ld3(S.insts[sref.inst].center));
         } else {
             const GTriShade &sh = S.shade[prim];
-            normal = tri_normal(sh, P.hu[p], P.hv[p]);
+            normal = tri_normal(sh, h.u, h.v);
             if (mat < 0) mat = sh.material;
         }
         const GMat &m = S.mats[mat];
@@ -343,7 +361,7 @@ __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, 
         if (m.texture >= 0) {
             float tu, tv;
             if (prim == -2) sphere_texcoord(normal, tu, tv);
-            else tri_texcoord(S.tri_uv[prim], P.hu[p], P.hv[p], tu, tv);
+            else tri_texcoord(S.tri_uv[prim], h.u, h.v, tu, tv);
             tcol = tex_sample(S.texels, S.textures[m.texture], tu, tv);
         }
         Pcg rng;
@@ -405,23 +423,25 @@ __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, 
                 const V3 nee = le * fs * mis * (G / lt.pdf);
                 c = W * nee;
             }
-            P.sx[p] = ldir.x; P.sy[p] = ldir.y; P.sz[p] = ldir.z;
-            P.stl[p] = tl;
-            P.scr[p] = c.x; P.scg[p] = c.y; P.scb[p] = c.z;
+            sr.o = pos;
+            sr.d = ldir;
+            sr.tl = tl;
+            sr.c = c;
             shadow = true;
         }
         V3 out;
         bool dice = false;
         W = W * (shade(m, d, normal, rng, out, dice) * tcol);
-        P.rng[p] = rng.state;
-        P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
         const bool stop = (dice && depth >= rr_start) ||
                           (W.x < FLT_EPSILON && W.y < FLT_EPSILON && W.z < FLT_EPSILON);
         if (!stop) {
+            // the state of a finished path is never read again
+            P.rng[p] = rng.state;
+            P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
             P.dx[p] = out.x; P.dy[p] = out.y; P.dz[p] = out.z;
+            P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
             cont = true;
         }
-        P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
     }
     // L += Ladd, skipped when Ladd is zero in every channel: L starts
     // at +0 and a sum is -0 only when both terms are, so adding a zero
@@ -450,10 +470,25 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t
         uint32_t p = 0;
         if (active) {
             p = q[i];
-            shade_path(S, P, p, depth, rr_start, shadow, cont);
+            const float4 hv = P.hit[i];
+            DHit h;
+            h.t = hv.x;
+            h.u = hv.y;
+            h.v = hv.z;
+            h.prim = __float_as_int(hv.w);
+            h.shape = P.hit_shape[i];
+            ShadowRay sr;
+            shade_path(S, P, p, h, depth, rr_start, shadow, sr, cont);
+            if (shadow) {
+                float4 *rec = P.shadow + 3 * (size_t)i;
+                rec[0] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tl);
+                rec[1] = make_float4(sr.d.x, sr.d.y, sr.d.z, __uint_as_float(p));
+                rec[2] = make_float4(sr.c.x, sr.c.y, sr.c.z, 0.0f);
+            }
         }
         stage.flush_if_full(&P.cnt[2], P.qs, &P.cnt[cur ^ 1], P.q[cur ^ 1], false);
-        stage.add(shadow, cont, p);
+        // the shadow queue takes the record's position, the extension queue the path
+        stage.add(shadow, i, cont, p);
     }
     stage.flush_if_full(&P.cnt[2], P.qs, &P.cnt[cur ^ 1], P.q[cur ^ 1], true);
 }
@@ -470,17 +505,20 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, Fetch
     st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
     Tracer<true, COUNT> tr;
     tr.reset_counts();
-    uint32_t p = 0;
+    uint32_t p = 0, j = 0;
     refill_loop(
         &P.cnt[5], n, fetch,
         [&](uint32_t i) {
-            p = P.qs[i];
+            j = P.qs[i];
+            const float4 a = P.shadow[3 * (size_t)j];
             // the light was missed (no NEE term): nothing to trace
-            const float tl = P.stl[p];
+            const float tl = a.w;
             if (!(tl >= 0.0f)) return false;
+            const float4 b = P.shadow[3 * (size_t)j + 1];
+            p = __float_as_uint(b.w);
             DRay r;
-            r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
-            r.dx = P.sx[p]; r.dy = P.sy[p]; r.dz = P.sz[p];
+            r.ox = a.x; r.oy = a.y; r.oz = a.z;
+            r.dx = b.x; r.dy = b.y; r.dz = b.z;
             // "closest hit is the light" == no other hit strictly before it
             st.sp = 0;
             tr.start(S.recs[scene_of(P, p)], r, kRefHitMin, nextafterf(tl, 0.0f));
@@ -489,11 +527,13 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, Fetch
         [&]() { return tr.step(S, st); },
         [&]() {
             if (tr.best.prim != -1) return;
-            P.lr[p] = P.lr[p] + P.scr[p];
-            P.lg[p] = P.lg[p] + P.scg[p];
-            P.lb[p] = P.lb[p] + P.scb[p];
+            const float4 c = P.shadow[3 * (size_t)j + 2];
+            P.lr[p] = P.lr[p] + c.x;
+            P.lg[p] = P.lg[p] + c.y;
+            P.lb[p] = P.lb[p] + c.z;
         });
     tr.flush(P.tot + 4);
+    if (COUNT) st.flush_spills(P.tot + 7);
 }
 
 // Renders of few paths: each thread carries its paths (grid-stride) through
@@ -521,27 +561,22 @@ __global__ __launch_bounds__(kBlock) void k_paths(SceneDev S, PathBuf P, const B
             r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
             r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
             const DHit h = traverse<false, false>(S, rec, r, kRefHitMin, kRefHitMax, st, nullptr);
-            P.ht[p] = h.t;
-            P.hu[p] = h.u;
-            P.hv[p] = h.v;
-            P.hprim[p] = h.prim;
-            P.hshape[p] = h.shape;
             ++ext;
             bool shadow = false, cont = false;
-            shade_path(S, P, p, depth, rr_start, shadow, cont);
+            ShadowRay sh;
+            shade_path(S, P, p, h, depth, rr_start, shadow, sh, cont);
             if (shadow) {
                 ++shadows;
-                const float tl = P.stl[p];
-                if (tl >= 0.0f) {
+                if (sh.tl >= 0.0f) {
                     DRay sr;
-                    sr.ox = P.ox[p]; sr.oy = P.oy[p]; sr.oz = P.oz[p];
-                    sr.dx = P.sx[p]; sr.dy = P.sy[p]; sr.dz = P.sz[p];
+                    sr.ox = sh.o.x; sr.oy = sh.o.y; sr.oz = sh.o.z;
+                    sr.dx = sh.d.x; sr.dy = sh.d.y; sr.dz = sh.d.z;
                     // "closest hit is the light" == no other hit strictly before it
-                    const DHit hs = traverse<true, false>(S, rec, sr, kRefHitMin, nextafterf(tl, 0.0f), st, nullptr);
+                    const DHit hs = traverse<true, false>(S, rec, sr, kRefHitMin, nextafterf(sh.tl, 0.0f), st, nullptr);
                     if (hs.prim == -1) {
-                        P.lr[p] = P.lr[p] + P.scr[p];
-                        P.lg[p] = P.lg[p] + P.scg[p];
-                        P.lb[p] = P.lb[p] + P.scb[p];
+                        P.lr[p] = P.lr[p] + sh.c.x;
+                        P.lg[p] = P.lg[p] + sh.c.y;
+                        P.lb[p] = P.lb[p] + sh.c.z;
                     }
                 }
             }
@@ -582,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, const BatchDes
 }
 
 __global__ void k_zero_tot(PathBuf P) {
-    if (threadIdx.x < 6 && blockIdx.x == 0) P.tot[threadIdx.x] = 0;
+    if (threadIdx.x < 8 && blockIdx.x == 0) P.tot[threadIdx.x] = 0;
 }
 
 // First-hit guide buffers for the denoisers.
@@ -883,7 +918,7 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     }
     S3DG_HIP(ctx, hipGetLastError());
     S3DG_HIP(ctx, hipEventRecord(e1, ctx->stream));
-    unsigned long long tot[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     S3DG_HIP(ctx, hipMemcpyAsync(tot, P.tot, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
     S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
@@ -912,6 +947,8 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
         stats->extend_tris = tot[3];
         stats->shadow_nodes = tot[4];
         stats->shadow_tris = tot[5];
+        stats->extend_stack_spills = tot[6];
+        stats->shadow_stack_spills = tot[7];
     }
     return check_error_flag(ctx, ws);
 }

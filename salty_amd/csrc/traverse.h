@@ -178,12 +178,14 @@ struct Stack {
     int32_t gcap;
     int32_t *err;
     int32_t sp;
+    uint32_t spills;   // pushes that went to the global area
 
     __device__ __forceinline__ void push(uint2 e) {
         if (sp < LDS) {
             lds[sp * kBlock] = pack_entry(e);
         } else if (sp - LDS < gcap) {
             gst[(size_t)(sp - LDS) * gstride] = pack_entry(e);
+            ++spills;
         } else {
             atomicOr(err, 1);
             return;
@@ -212,6 +214,14 @@ struct Stack {
         gcap = global_cap;
         err = error;
         sp = 0;
+        spills = 0;
+    }
+    // Adds the wave's global-area pushes to *counter (one atomic per wave);
+    // every lane of the wave must call it.
+    __device__ __forceinline__ void flush_spills(unsigned long long *counter) const {
+        unsigned long long v = spills;
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (__lane_id() == 0 && v) atomicAdd(counter, v);
     }
 };
 
