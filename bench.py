@@ -223,7 +223,7 @@ def soup_run(args, steps, warmup, cpu_seconds):
             best = None
             for refill in (16, 24, 32, 48):
                 ctx.set_tuning(abi.Tuning(base.traversal_variant, refill, base.tri_batch, base.passes_in_flight,
-                                          base.fetch_chunk, base.leaf_size))
+                                          base.fetch_chunk, base.leaf_size, base.tail_paths))
                 scene.trace_device(d_rays, n, mode, d_hits)
                 ctx.synchronize()
                 t = time.perf_counter()
@@ -233,7 +233,7 @@ def soup_run(args, steps, warmup, cpu_seconds):
                 if best is None or dt < best[0]:
                     best = (dt, refill)
             ctx.set_tuning(abi.Tuning(base.traversal_variant, best[1], base.tri_batch, base.passes_in_flight,
-                                      base.fetch_chunk, base.leaf_size))
+                                      base.fetch_chunk, base.leaf_size, base.tail_paths))
             picked[name] = best[1]
         for _ in range(warmup):
             scene.trace_device(d_rays, n, mode, d_hits)
@@ -609,7 +609,7 @@ def main():
     if not args.no_autotune:
         t, _ = scene.autotune(cfg(args.spp))
         tuning = {"traversal_variant": t.traversal_variant, "refill": t.refill, "tri_batch": t.tri_batch,
-                  "fetch_chunk": t.fetch_chunk}
+                  "fetch_chunk": t.fetch_chunk, "tail_paths": t.tail_paths}
     for w in range(args.warmup):
         scene.render_device(cfg(args.spp, first_pass=0), d_accum)
     ctx.synchronize()

@@ -387,6 +387,9 @@ typedef struct s3dg_tuning {
     uint32_t leaf_size;          /* most triangles per leaf of the traversal trees of scenes
                                     created afterwards (1..8); 0: traverse the reference-shaped
                                     tree.  Tied hits are ranked by the reference tree either way */
+    uint32_t tail_paths;         /* the wavefront's remaining bounces run thread per path in one
+                                    launch once its queue holds at most this many paths (capped
+                                    at the traversal grid's threads; 0: never)               */
 } s3dg_tuning;
 
 int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out);

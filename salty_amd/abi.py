@@ -90,7 +90,8 @@ class RenderConfig(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("traversal_variant", C.c_uint32), ("refill", C.c_uint32), ("tri_batch", C.c_uint32),
-                ("passes_in_flight", C.c_uint32), ("fetch_chunk", C.c_uint32), ("leaf_size", C.c_uint32)]
+                ("passes_in_flight", C.c_uint32), ("fetch_chunk", C.c_uint32), ("leaf_size", C.c_uint32),
+                ("tail_paths", C.c_uint32)]
 
 
 class FrameJob(C.Structure):

@@ -34,6 +34,10 @@ struct s3dg_context {
     // most triangles per leaf of the traversal trees of scenes created from
     // now on (1..8); 0 traverses the reference-shaped tree instead
     uint32_t leaf_size = 2;
+    // the wavefront's remaining bounces run thread per path once its
+    // extension queue holds at most this many paths (capped at one per
+    // thread of the traversal grid; 0: never)
+    uint32_t tail_paths = 0xffffffffu;
 };
 
 namespace s3dg {
@@ -70,7 +74,7 @@ struct SceneDev {
 };
 
 // Words of Workspace::graph_key.
-constexpr int kGraphKeyWords = 12;
+constexpr int kGraphKeyWords = 13;
 
 // Device buffers of the traversal and render kernels, owned by a scene or a
 // scene set and sized for the largest launch so far.

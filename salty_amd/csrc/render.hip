@@ -31,37 +31,39 @@ using namespace s3dg;
 
 namespace {
 
-// Path state over N path slots.  What lives from bounce to bounce (ray,
-// throughput, radiance, RNG, scene) is a structure of arrays by path slot;
-// what one bounce hands to the next kernel is kept in the order that kernel
+// Path state over N path slots.  What lives from bounce to bounce is one
+// 64-byte record per path slot (a gather after compaction then touches one
+// line per path, not one per field):
+//   path[4p]     ray origin x y z, direction x
+//   path[4p + 1] direction y z, throughput W r g
+//   path[4p + 2] W b, radiance L r g b
+//   path[4p + 3] PCG state (low, high word), scene, unused
+// What one bounce hands to the next kernel is kept in the order that kernel
 // reads it: the hit of queue entry i of the extension launch at hit[i], the
-// shadow ray of the path the shade launch visited i-th at shadow[3 i ...],
-// so that those records are written and read in whole lines.
+// shadow ray of the path the shade launch visited i-th at shadow[3 i ...].
 struct PathBuf {
-    float *ox, *oy, *oz, *dx, *dy, *dz;     // current ray
-    float *wr, *wg, *wb;                    // throughput W
-    float *lr, *lg, *lb;                    // radiance L
-    uint64_t *rng;                          // PCG state
+    float4 *path;                           // path records
     float4 *hit;                            // extension hit by queue position: t, u, v, prim (bits)
     int32_t *hit_shape;                     // top-level shape of that hit
     float4 *shadow;                         // shadow rays by shade position, three float4 each:
                                             // origin + distance to the light (< 0: light missed),
                                             // direction + path slot (bits), W * NEE (added to L
                                             // when the light is visible)
-    uint32_t *scene;                        // scene of the path (index into SceneDev::recs)
-    int32_t single_scene;                   // >= 0: every path is in this scene (scene[] unused)
+    int32_t single_scene;                   // >= 0: every path is in this scene (record scene unused)
     uint32_t *q[2];                         // extension queues (ping-pong)
     uint32_t *qs;                           // shadow queue (shade positions)
     uint32_t *cnt;                          // [0],[1] extension counts, [2] shadow count,
-                                            // [4],[5] extend / shadow work-fetch cursors
+                                            // [4],[5] extend / shadow work-fetch cursors,
+                                            // [8] a tail launch has finished the paths,
+                                            // [9] the launches after it have seen that
     unsigned long long *tot;                // [0] extension rays, [1] shadow rays, [2..5] node visits
                                             // and triangle tests (extend, shadow), [6], [7]
                                             // stack pushes to global memory (extend, shadow)
 };
 
-// bytes per path slot: 12 four-byte ray / W / L fields, the RNG, a 16-byte
-// hit and its shape, a 48-byte shadow record, the scene and three queues
-constexpr size_t kPathBytes = 12 * 4 + 8 + 16 + 4 + 48 + 4 + 3 * 4;
+// bytes per path slot: the 64-byte record, a 16-byte hit and its shape, a
+// 48-byte shadow record and three queues
+constexpr size_t kPathBytes = 64 + 16 + 4 + 48 + 3 * 4;
 
 size_t path_bytes(uint32_t n) { return (size_t)n * kPathBytes + 4096; }
 
@@ -75,13 +77,10 @@ PathBuf carve(void *base, uint32_t n) {
     PathBuf b;
     b.cnt = (uint32_t *)take(64);
     b.tot = (unsigned long long *)take(64);
-    b.rng = (uint64_t *)take((size_t)n * 8);
-    float **f[] = {&b.ox, &b.oy, &b.oz, &b.dx, &b.dy, &b.dz, &b.wr, &b.wg, &b.wb, &b.lr, &b.lg, &b.lb};
-    for (float **x : f) *x = (float *)take((size_t)n * 4);
+    b.path = (float4 *)take((size_t)n * 64);
     b.hit = (float4 *)take((size_t)n * 16);
     b.hit_shape = (int32_t *)take((size_t)n * 4);
     b.shadow = (float4 *)take((size_t)n * 48);
-    b.scene = (uint32_t *)take((size_t)n * 4);
     b.q[0] = (uint32_t *)take((size_t)n * 4);
     b.q[1] = (uint32_t *)take((size_t)n * 4);
     b.qs = (uint32_t *)take((size_t)n * 4);
@@ -115,7 +114,23 @@ struct ShadowRay {
 };
 
 __device__ __forceinline__ uint32_t scene_of(const PathBuf &P, uint32_t p) {
-    return P.single_scene >= 0 ? (uint32_t)P.single_scene : P.scene[p];
+    return P.single_scene >= 0 ? (uint32_t)P.single_scene : __float_as_uint(P.path[4 * (size_t)p + 3].z);
+}
+
+// The ray of path p.
+__device__ __forceinline__ DRay path_ray(const PathBuf &P, uint32_t p) {
+    const float4 a = P.path[4 * (size_t)p], b = P.path[4 * (size_t)p + 1];
+    DRay r;
+    r.ox = a.x; r.oy = a.y; r.oz = a.z;
+    r.dx = a.w; r.dy = b.x; r.dz = b.y;
+    return r;
+}
+
+// L += c for path p (the W b word of the same float4 is kept).
+__device__ __forceinline__ void path_add_l(const PathBuf &P, uint32_t p, V3 c) {
+    float4 *w = P.path + 4 * (size_t)p + 2;
+    const float4 v = *w;
+    *w = make_float4(v.x, v.y + c.x, v.z + c.y, v.w + c.z);
 }
 size_t carve_bytes(uint32_t n) { return path_bytes(n) + 40 * 256; }
 
@@ -257,12 +272,12 @@ __device__ __forceinline__ void raygen_path(const SceneDev &S, const PathBuf &P,
     const float fy = (0.5f + (float)y) / (float)g.height - 0.5f;
     V3 o, d;
     camera_ray(S.recs[g.scene].cam, fx, fy, rng, o, d);
-    if (P.single_scene < 0) P.scene[p] = g.scene;
-    P.ox[p] = o.x; P.oy[p] = o.y; P.oz[p] = o.z;
-    P.dx[p] = d.x; P.dy[p] = d.y; P.dz[p] = d.z;
-    P.wr[p] = 1.0f; P.wg[p] = 1.0f; P.wb[p] = 1.0f;
-    P.lr[p] = 0.0f; P.lg[p] = 0.0f; P.lb[p] = 0.0f;
-    P.rng[p] = rng.state;
+    float4 *rec = P.path + 4 * (size_t)p;
+    rec[0] = make_float4(o.x, o.y, o.z, d.x);
+    rec[1] = make_float4(d.y, d.z, 1.0f, 1.0f);
+    rec[2] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+    rec[3] = make_float4(__uint_as_float((uint32_t)rng.state), __uint_as_float((uint32_t)(rng.state >> 32)),
+                         __uint_as_float(g.scene), 0.0f);
 }
 
 __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const BatchDesc *__restrict__ bd,
@@ -275,6 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(SceneDev S, PathBuf P, const 
         P.cnt[2] = 0;
         P.cnt[4] = 0;
         P.cnt[5] = 0;
+        P.cnt[8] = 0;
+        P.cnt[9] = 0;
     }
     for (uint32_t p = gtid; p < npaths; p += gridDim.x * blockDim.x) {
         raygen_path(S, P, p, nseg, segs);
@@ -286,9 +303,14 @@ template <bool COUNT, int LDS, int WAVES>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
 k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_t *err, FetchParams fetch) {
     __shared__ uint2 lds[LDS * kBlock];
-    const uint32_t n = P.cnt[cur];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (P.cnt[8]) {
+        // a tail launch has finished every path: tell the later tails
+        if (gtid == 0) P.cnt[9] = 1;
+        return;
+    }
+    const uint32_t n = P.cnt[cur];
     if (gtid == 0) {
         P.cnt[cur ^ 1] = 0;
         P.cnt[2] = 0;
@@ -306,9 +328,7 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
         [&](uint32_t i) {
             qi = i;
             const uint32_t p = q[i];
-            DRay r;
-            r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
-            r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
+            const DRay r = path_ray(P, p);
             st.sp = 0;
             tr.start(S.recs[scene_of(P, p)], r, kRefHitMin, kRefHitMax);
             return true;
@@ -331,16 +351,19 @@ k_extend(SceneDev S, PathBuf P, int32_t cur, uint2 *gstack, int32_t gcap, int32_
 __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, uint32_t p, const DHit &h,
                                            int32_t depth, int32_t rr_start, bool &shadow, ShadowRay &sr,
                                            bool &cont) {
-    V3 W = v3(P.wr[p], P.wg[p], P.wb[p]);
+    float4 *rp = P.path + 4 * (size_t)p;
+    const float4 ra = rp[0], rb = rp[1], rc = rp[2], rd = rp[3];
+    V3 W = v3(rb.z, rb.w, rc.x);
+    const V3 o = v3(ra.x, ra.y, ra.z);
+    const V3 d = v3(ra.w, rb.x, rb.y);
+    const uint32_t scene = P.single_scene >= 0 ? (uint32_t)P.single_scene : __float_as_uint(rd.z);
     V3 Ladd;   // what this vertex adds to the path's radiance L
     const int32_t prim = h.prim;
     if (prim == -1) {
         // miss: L += W * SampleIBL(dir) (s3d_pt.cpp:199-203), or the
         // constant environment when the scene has no IBL texture
-        Ladd = W * environment(S, S.recs[scene_of(P, p)], v3(P.dx[p], P.dy[p], P.dz[p]));
+        Ladd = W * environment(S, S.recs[scene], d);
     } else {
-        const V3 o = v3(P.ox[p], P.oy[p], P.oz[p]);
-        const V3 d = v3(P.dx[p], P.dy[p], P.dz[p]);
         const float t = h.t;
         const V3 pos = o + d * t;
         V3 normal;
@@ -365,8 +388,8 @@ __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, 
             tcol = tex_sample(S.texels, S.textures[m.texture], tu, tv);
         }
         Pcg rng;
-        rng.state = P.rng[p];
-        const SceneRec &rec = S.recs[scene_of(P, p)];
+        rng.state = (uint64_t)__float_as_uint(rd.x) | ((uint64_t)__float_as_uint(rd.y) << 32);
+        const SceneRec &rec = S.recs[scene];
         if (!m.delta && rec.light_count > 0) {
             // NextEventEstimation sampling half (s3d_pt.cpp:258-268)
             const uint32_t li = (uint32_t)rec.light_base + rng.u32() % (uint32_t)rec.light_count;
@@ -435,26 +458,29 @@ __device__ __forceinline__ void shade_path(const SceneDev &S, const PathBuf &P, 
         const bool stop = (dice && depth >= rr_start) ||
                           (W.x < FLT_EPSILON && W.y < FLT_EPSILON && W.z < FLT_EPSILON);
         if (!stop) {
-            // the state of a finished path is never read again
-            P.rng[p] = rng.state;
-            P.ox[p] = pos.x; P.oy[p] = pos.y; P.oz[p] = pos.z;
-            P.dx[p] = out.x; P.dy[p] = out.y; P.dz[p] = out.z;
-            P.wr[p] = W.x; P.wg[p] = W.y; P.wb[p] = W.z;
+            // the next ray, W and the RNG (the state of a finished path is
+            // never read again); L follows below
+            rp[0] = make_float4(pos.x, pos.y, pos.z, out.x);
+            rp[1] = make_float4(out.y, out.z, W.x, W.y);
+            rp[3] = make_float4(__uint_as_float((uint32_t)rng.state), __uint_as_float((uint32_t)(rng.state >> 32)),
+                                rd.z, rd.w);
             cont = true;
         }
     }
     // L += Ladd, skipped when Ladd is zero in every channel: L starts
     // at +0 and a sum is -0 only when both terms are, so adding a zero
-    // leaves L's bits as they are (a NaN compares unequal and is added)
-    if (Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f) {
-        P.lr[p] = P.lr[p] + Ladd.x;
-        P.lg[p] = P.lg[p] + Ladd.y;
-        P.lb[p] = P.lb[p] + Ladd.z;
+    // leaves L's bits as they are (a NaN compares unequal and is added).
+    // The float4 holding L also holds W b, written when the path goes on.
+    const bool add = Ladd.x != 0.0f || Ladd.y != 0.0f || Ladd.z != 0.0f;
+    if (add || cont) {
+        const V3 L = add ? v3(rc.y + Ladd.x, rc.z + Ladd.y, rc.w + Ladd.z) : v3(rc.y, rc.z, rc.w);
+        rp[2] = make_float4(cont ? W.z : rc.x, L.x, L.y, L.z);
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade(SceneDev S, PathBuf P, int32_t cur, int32_t depth,
                                                    int32_t rr_start) {
+    if (P.cnt[8]) return;   // a tail launch has finished every path
     const uint32_t n = P.cnt[cur];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -497,6 +523,7 @@ template <bool COUNT, int LDS, int WAVES>
 __global__ __attribute__((amdgpu_flat_work_group_size(kBlock, kBlock), amdgpu_waves_per_eu(WAVES, WAVES))) void
 k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, FetchParams fetch) {
     __shared__ uint2 lds[LDS * kBlock];
+    if (P.cnt[8]) return;   // a tail launch has finished every path
     const uint32_t n = P.cnt[2];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,9 +555,7 @@ k_shadow(SceneDev S, PathBuf P, uint2 *gstack, int32_t gcap, int32_t *err, Fetch
         [&]() {
             if (tr.best.prim != -1) return;
             const float4 c = P.shadow[3 * (size_t)j + 2];
-            P.lr[p] = P.lr[p] + c.x;
-            P.lg[p] = P.lg[p] + c.y;
-            P.lb[p] = P.lb[p] + c.z;
+            path_add_l(P, p, v3(c.x, c.y, c.z));
         });
     tr.flush(P.tot + 4);
     if (COUNT) st.flush_spills(P.tot + 7);
@@ -557,9 +582,7 @@ __global__ __launch_bounds__(kBlock) void k_paths(SceneDev S, PathBuf P, const B
         raygen_path(S, P, p, bd->nseg, segs);
         const SceneRec &rec = S.recs[scene_of(P, p)];
         for (int32_t depth = 0; depth < max_bounce; ++depth) {
-            DRay r;
-            r.ox = P.ox[p]; r.oy = P.oy[p]; r.oz = P.oz[p];
-            r.dx = P.dx[p]; r.dy = P.dy[p]; r.dz = P.dz[p];
+            const DRay r = path_ray(P, p);
             const DHit h = traverse<false, false>(S, rec, r, kRefHitMin, kRefHitMax, st, nullptr);
             ++ext;
             bool shadow = false, cont = false;
@@ -573,17 +596,71 @@ __global__ __launch_bounds__(kBlock) void k_paths(SceneDev S, PathBuf P, const B
                     sr.dx = sh.d.x; sr.dy = sh.d.y; sr.dz = sh.d.z;
                     // "closest hit is the light" == no other hit strictly before it
                     const DHit hs = traverse<true, false>(S, rec, sr, kRefHitMin, nextafterf(sh.tl, 0.0f), st, nullptr);
-                    if (hs.prim == -1) {
-                        P.lr[p] = P.lr[p] + sh.c.x;
-                        P.lg[p] = P.lg[p] + sh.c.y;
-                        P.lb[p] = P.lb[p] + sh.c.z;
-                    }
+                    if (hs.prim == -1) path_add_l(P, p, sh.c);
                 }
             }
             if (!cont) break;
         }
     }
     // ray counts of the render statistics: one atomic per wave
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ext += __shfl_xor(ext, off);
+        shadows += __shfl_xor(shadows, off);
+    }
+    if (__lane_id() == 0) {
+        if (ext) atomicAdd(&P.tot[0], ext);
+        if (shadows) atomicAdd(&P.tot[1], shadows);
+    }
+}
+
+// The last bounces of the wavefront: once the extension queue holds at most
+// `limit` paths (at most one per thread of the grid), one thread carries each
+// of them through every remaining bounce, depth0 .. max_bounce - 1, with the
+// per-path code of k_paths, instead of three launches per bounce that each
+// wait on the slowest of a few rays.  Same arithmetic per path in the same
+// order, so the sums and ray counts are those of the pipeline.  Launched after
+// every bounce's shadow launch; the first one whose queue is short enough
+// runs, sets cnt[8] so that the later bounce launches return at once, and the
+// later tails see cnt[9], which the first skipped extend launch sets.
+template <int LDS>
+__global__ __launch_bounds__(kBlock) void k_tail(SceneDev S, PathBuf P, int32_t cur, int32_t depth0,
+                                                 int32_t max_bounce, int32_t rr_start, uint32_t limit,
+                                                 uint2 *gstack, int32_t gcap, int32_t *err) {
+    __shared__ uint2 lds[LDS * kBlock];
+    if (P.cnt[9]) return;
+    const uint32_t n = P.cnt[cur];
+    if (n > limit) return;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if (gtid == 0) P.cnt[8] = 1;
+    Stack<LDS> st;
+    st.init(lds, threadIdx.x, gstack + gtid, stride, gcap, err);
+    unsigned long long ext = 0, shadows = 0;
+    for (uint32_t i = gtid; i < n; i += stride) {
+        const uint32_t p = P.q[cur][i];
+        const SceneRec &rec = S.recs[scene_of(P, p)];
+        for (int32_t depth = depth0; depth < max_bounce; ++depth) {
+            const DRay r = path_ray(P, p);
+            const DHit h = traverse<false, false>(S, rec, r, kRefHitMin, kRefHitMax, st, nullptr);
+            ++ext;
+            bool shadow = false, cont = false;
+            ShadowRay sh;
+            shade_path(S, P, p, h, depth, rr_start, shadow, sh, cont);
+            if (shadow) {
+                ++shadows;
+                if (sh.tl >= 0.0f) {
+                    DRay sr;
+                    sr.ox = sh.o.x; sr.oy = sh.o.y; sr.oz = sh.o.z;
+                    sr.dx = sh.d.x; sr.dy = sh.d.y; sr.dz = sh.d.z;
+                    // "closest hit is the light" == no other hit strictly before it
+                    const DHit hs = traverse<true, false>(S, rec, sr, kRefHitMin, nextafterf(sh.tl, 0.0f), st, nullptr);
+                    if (hs.prim == -1) path_add_l(P, p, sh.c);
+                }
+            }
+            if (!cont) break;
+        }
+    }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         ext += __shfl_xor(ext, off);
@@ -606,9 +683,10 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(PathBuf P, const BatchDes
         float r = acc[0], gg = acc[1], b = acc[2];
         for (uint32_t s = 0; s < g.passes; ++s) {
             const size_t q = (size_t)g.path0 + (size_t)s * g.npix + j;
-            r = r + P.lr[q];
-            gg = gg + P.lg[q];
-            b = b + P.lb[q];
+            const float4 l = P.path[4 * q + 2];
+            r = r + l.y;
+            gg = gg + l.z;
+            b = b + l.w;
         }
         acc[0] = r;
         acc[1] = gg;
@@ -675,6 +753,7 @@ struct TraceKernels {
     const void *ext[2];
     const void *sh[2];
     const void *paths;
+    const void *tail;
     int lds;
 };
 
@@ -685,7 +764,8 @@ TraceKernels trace_kernels_of() {
                          reinterpret_cast<const void *>(&k_extend<true, L, W>)},
                         {reinterpret_cast<const void *>(&k_shadow<false, L, W>),
                          reinterpret_cast<const void *>(&k_shadow<true, L, W>)},
-                        reinterpret_cast<const void *>(&k_paths<L>), L};
+                        reinterpret_cast<const void *>(&k_paths<L>), reinterpret_cast<const void *>(&k_tail<L>),
+                        L};
 }
 
 TraceKernels trace_kernels(uint32_t v) {
@@ -850,6 +930,10 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     // its grid: one thread per path slot, at most as many threads as the
     // traversal stack area was sized for
     const uint32_t pgrid = std::min<uint32_t>((slots + kBlock - 1) / kBlock, tgrid);
+    // the pipeline's tail launches take over queues of at most this many
+    // paths (one per thread of the traversal grid); not with instrumentation,
+    // whose per-launch timings and work counts cover the pipeline kernels
+    uint32_t tail_limit = (timed || count) ? 0u : std::min<uint32_t>(ctx->tail_paths, tgrid * kBlock);
     auto launch_batch = [&](size_t li) {
         if (per_path) {
             void *args[] = {&Sx, &P, &d_desc, &d_segs, &max_bounce, &rr_start, &gs, &gcap, &d_err};
@@ -870,6 +954,11 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
             (void)hipLaunchKernel(K.sh[count ? 1 : 0], dim3(tgrid), dim3(kBlock), sh_args, 0, ctx->stream);
             if (timed) (void)hipEventRecord(ev[4 * li + 3], ctx->stream);
             cur ^= 1;
+            if (tail_limit && depth + 1 < max_bounce) {
+                int32_t depth0 = depth + 1;
+                void *tail_args[] = {&Sx, &P, &cur, &depth0, &max_bounce, &rr_start, &tail_limit, &gs, &gcap, &d_err};
+                (void)hipLaunchKernel(K.tail, dim3(tgrid), dim3(kBlock), tail_args, 0, ctx->stream);
+            }
         }
         hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, ctx->stream, P, d_desc, d_segs);
     };
@@ -891,7 +980,8 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
                                               (uint64_t)(uintptr_t)S.nodes, (uint64_t)(uintptr_t)S.recs,
                                               ((uint64_t)fetch.refill << 48) | ((uint64_t)fetch.chunk << 16) |
                                                   Sx.tri_batch,
-                                              (uint64_t)(uintptr_t)ws.d_batch, (uint64_t)(uintptr_t)ws.d_error};
+                                              (uint64_t)(uintptr_t)ws.d_batch,
+                                              (uint64_t)(uintptr_t)ws.d_error, tail_limit};
         if (!ws.graph || memcmp(key, ws.graph_key, sizeof(key)) != 0) {
             if (ws.graph) {
                 (void)hipGraphExecDestroy(ws.graph);
@@ -983,6 +1073,7 @@ extern "C" int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out) {
     out->passes_in_flight = ctx->passes_in_flight;
     out->fetch_chunk = ctx->fetch_chunk;
     out->leaf_size = ctx->leaf_size;
+    out->tail_paths = ctx->tail_paths;
     return S3DG_OK;
 }
 
@@ -997,11 +1088,13 @@ extern "C" int s3dg_set_tuning(s3dg_context *ctx, const s3dg_tuning *t) {
     ctx->passes_in_flight = t->passes_in_flight;
     ctx->fetch_chunk = t->fetch_chunk;
     ctx->leaf_size = t->leaf_size;
+    ctx->tail_paths = t->tail_paths;
     return S3DG_OK;
 }
 
 // Coordinate descent over the traversal variant, the refill threshold, the
-// triangle batching threshold and the fetch chunk; each candidate is timed (device time)
+// triangle batching threshold, the fetch chunk and the tail queue length;
+// each candidate is timed (device time)
 // on its second trial render, after a first one that captures its graph.
 extern "C" int s3dg_autotune(s3dg_scene *sc, const s3dg_render_config *cfg, s3dg_tuning *best, double *best_ms) {
     if (!sc || !cfg || cfg->width <= 0 || cfg->height <= 0) return S3DG_ERR_INVALID;
@@ -1033,7 +1126,8 @@ extern "C" int s3dg_autotune(s3dg_scene *sc, const s3dg_render_config *cfg, s3dg
     const uint32_t refills[] = {16, 24, 32, 48};
     const uint32_t batches[] = {8, 16, 24};
     const uint32_t chunks[] = {64, 128, 256};
-    for (int axis = 0; axis < 4; ++axis) {
+    const uint32_t tails[] = {0u, 1u << 16, 0xffffffffu};
+    for (int axis = 0; axis < 5; ++axis) {
         const int n = axis == 0 ? kVariantCount : axis == 1 ? 4 : 3;
         s3dg_tuning axis_best = cur;
         for (int i = 0; i < n; ++i) {
@@ -1041,7 +1135,8 @@ extern "C" int s3dg_autotune(s3dg_scene *sc, const s3dg_render_config *cfg, s3dg
             if (axis == 0) t.traversal_variant = (uint32_t)i;
             else if (axis == 1) t.refill = refills[i];
             else if (axis == 2) t.tri_batch = batches[i];
-            else t.fetch_chunk = chunks[i];
+            else if (axis == 3) t.fetch_chunk = chunks[i];
+            else t.tail_paths = tails[i];
             if (memcmp(&t, &cur, sizeof(t)) == 0) continue;
             double ms = 0.0;
             rc = time_one(t, ms);

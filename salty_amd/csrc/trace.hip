@@ -45,6 +45,12 @@ extern "C" int s3dg_create(s3dg_context **out, int device) {
         const long v = std::strtol(env, nullptr, 10);
         if (v >= 0 && v < kVariantCount) ctx->variant = (uint32_t)v;
     }
+    // S3DG_TAIL_PATHS overrides the queue length below which the remaining
+    // bounces run thread per path (0: never)
+    if (const char *env = std::getenv("S3DG_TAIL_PATHS")) {
+        const long long v = std::strtoll(env, nullptr, 10);
+        if (v >= 0) ctx->tail_paths = v > 0xffffffffll ? 0xffffffffu : (uint32_t)v;
+    }
     // S3DG_LEAF_SIZE (0..8) overrides the leaf size of the traversal trees
     if (const char *env = std::getenv("S3DG_LEAF_SIZE")) {
         const long v = std::strtol(env, nullptr, 10);

@@ -52,6 +52,37 @@ def test_variants_render_identically(ctx):
         ctx.set_tuning(start)
 
 
+@pytest.mark.parametrize("rr_start", [0, 3])
+def test_tail_launch_changes_nothing(ctx, rr_start):
+    """The tail launch (remaining bounces thread per path once the
+    extension queue is short) gives the pipeline's sums and ray counts bit
+    for bit, wherever it takes over: never, after a few bounces, or early;
+    also for several batches and for a scene set."""
+    import salty_amd
+    start = ctx.get_tuning()
+    desc = scenes.mesh_scene(big_level=4, small_level=2, ground=(40, 20), instances=8, materials="all")
+    sc = salty_amd.Scene(ctx, desc)
+    try:
+        outs = []
+        for tail in (0, 50, 2000, 1 << 30):
+            t = ctx.get_tuning()
+            t.tail_paths = tail
+            ctx.set_tuning(t)
+            for flight in (0, 2):
+                img, st = sc.render(96, 64, spp=5, max_depth=10, seed=4, rr_start_depth=rr_start,
+                                    flags=abi.RENDER_WAVEFRONT, passes_in_flight=flight)
+                outs.append((img, st.extension_rays, st.shadow_rays))
+        for img, ne, ns in outs[1:]:
+            assert np.array_equal(img, outs[0][0])
+            assert (ne, ns) == outs[0][1:]
+        # instrumented renders (no tail launch) agree too
+        img, st = sc.render(96, 64, spp=5, max_depth=10, seed=4, rr_start_depth=rr_start,
+                            flags=abi.RENDER_TIME_KERNELS)
+        assert np.array_equal(img, outs[0][0]) and (st.extension_rays, st.shadow_rays) == outs[0][1:]
+    finally:
+        ctx.set_tuning(start)
+
+
 def test_autotune_keeps_a_valid_setting(ctx):
     import salty_amd
     start = ctx.get_tuning()
