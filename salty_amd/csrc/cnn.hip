@@ -92,7 +92,8 @@ constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 paddi
 template <int COT, bool PF>
 __global__ __launch_bounds__(256, PF ? 1 : (COT == 32 ? 3 : 2)) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                      const bf16 *__restrict__ wt, const float *__restrict__ bias,
-                                                     int32_t Cout, int32_t relu, bf16 *__restrict__ out) {
+                                                     int32_t Cout, int32_t relu, const bf16 *__restrict__ mask,
+                                                     bf16 *__restrict__ out) {
     __shared__ __attribute__((aligned(16))) bf16 s_in[kHalo * kHalo * kRow];
     __shared__ __attribute__((aligned(16))) bf16 s_w[9 * COT * kRow];
     constexpr int NT = COT / 32;
@@ -223,8 +224,16 @@ __global__ __launch_bounds__(256, PF ? 1 : (COT == 32 ? 3 : 2)) void k_conv3x3(C
         const int pix = e / (COT / 8), q = e % (COT / 8);
         const int y = y0 + pix / kTile, x = x0 + pix % kTile, co = co0 + 8 * q;
         if (y >= H || x >= W || co >= Cout) continue;
-        *reinterpret_cast<bf16x8 *>(out + (((size_t)n * H + y) * W + x) * Cout + co) =
-            *reinterpret_cast<const bf16x8 *>(&s_w[pix * kOutRow + 8 * q]);
+        const size_t off = (((size_t)n * H + y) * W + x) * Cout + co;
+        bf16x8 v = *reinterpret_cast<const bf16x8 *>(&s_w[pix * kOutRow + 8 * q]);
+        if (mask) {
+            // gradient through the ReLU whose output is `mask` (same shape)
+            const bf16x8 m = *reinterpret_cast<const bf16x8 *>(mask + off);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (!(b2f(m[j]) > 0.0f)) v[j] = f2b(0.0f);
+        }
+        *reinterpret_cast<bf16x8 *>(out + off) = v;
     }
 }
 
@@ -247,7 +256,8 @@ constexpr int kDwCi = 200;                       // one channel of a halo copy: 
 
 template <int COT>
 __global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H, int32_t W,
-                                                  const bf16 *__restrict__ dpre, int32_t Cout, float *__restrict__ dw) {
+                                                  const bf16 *__restrict__ dpre, int32_t Cout, float *__restrict__ dw,
+                                                  float *__restrict__ db) {
     constexpr int CT = COT == 64 ? 2 : 1;        // input-channel tiles of 16 per wave
     __shared__ __attribute__((aligned(16))) bf16 s_dy[COT * kDwPitch];
     __shared__ __attribute__((aligned(16))) bf16 s_x[3 * 32 * kDwCi];
@@ -268,6 +278,12 @@ __global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H,
 #pragma unroll
             for (int i = 0; i < 4; ++i) acc[t][c][i] = 0.0f;
     const bf16 z = f2b(0.0f);
+    // bias gradient (sum of dPre over pixels), by the workgroups of the first
+    // input-channel chunk: thread t sums segment t % SEGS of channel t / SEGS
+    // of every staged tile
+    constexpr int SEGS = 256 / COT, SEG_PIX = kDwPix / SEGS;
+    const bool bias_wg = db && ci0 == 0;
+    float bsum = 0.0f;
     for (long long tile = blockIdx.y; tile < ntiles; tile += gridDim.y) {
         const int n = (int)(tile / ((long long)tiles_x * tiles_y));
         const int rem = (int)(tile % ((long long)tiles_x * tiles_y));
@@ -301,6 +317,15 @@ __global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H,
             }
         }
         __syncthreads();
+        if (bias_wg) {
+            const bf16 *row = &s_dy[(threadIdx.x / SEGS) * kDwPitch + (threadIdx.x % SEGS) * SEG_PIX];
+#pragma unroll
+            for (int k = 0; k < SEG_PIX; k += 8) {
+                const bf16x8 v = *reinterpret_cast<const bf16x8 *>(row + k);
+#pragma unroll
+                for (int j = 0; j < 8; ++j) bsum += b2f(v[j]);
+            }
+        }
 #pragma unroll 1
         for (int ks = 0; ks < kDwPix / 32; ++ks) {
             const int g = lane >> 4;                    // 8-pixel group of this lane
@@ -333,29 +358,13 @@ __global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H,
                 const float v = acc[tap][ct][i];
                 if (co < Cout && v != 0.0f) atomicAdd(&dw[((size_t)tap * Cout + co) * Cin + ci], v);
             }
-}
-
-__global__ void k_bias_grad(const bf16 *__restrict__ dpre, long long npix, int32_t C, float *__restrict__ db) {
-    // one block per channel group of 64 channels x slice of pixels
-    const int c = blockIdx.y * 64 + (threadIdx.x & 63);
-    const int part = threadIdx.x >> 6;   // 4 pixel phases
-    float s = 0.0f;
-    if (c < C)
-        for (long long p = (long long)blockIdx.x * 4 + part; p < npix; p += (long long)gridDim.x * 4)
-            s += b2f(dpre[p * C + c]);
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x < 64 && c < C) {
-        const float t = red[threadIdx.x] + red[threadIdx.x + 64] + red[threadIdx.x + 128] + red[threadIdx.x + 192];
-        if (t != 0.0f) atomicAdd(&db[c], t);
+    if (bias_wg) {
+        // the SEGS segments of a channel are adjacent lanes
+#pragma unroll
+        for (int off = 1; off < SEGS; off <<= 1) bsum += __shfl_xor(bsum, off);
+        const int co = co0 + (int)threadIdx.x / SEGS;
+        if (threadIdx.x % SEGS == 0 && co < Cout && bsum != 0.0f) atomicAdd(&db[co], bsum);
     }
-}
-
-// dY *= (Y > 0): gradient through ReLU, in place.
-__global__ void k_relu_back(bf16 *__restrict__ d, const bf16 *__restrict__ y, long long n) {
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        if (!(b2f(y[i]) > 0.0f)) d[i] = f2b(0.0f);
 }
 
 // 2x2 average pooling, NHWC.
@@ -380,7 +389,7 @@ __global__ void k_pool2(const bf16 *__restrict__ x, int32_t N, int32_t H, int32_
 }
 // dX += upsample(dY) / 4
 __global__ void k_pool2_back(const bf16 *__restrict__ dy, int32_t N, int32_t H, int32_t W, int32_t C,
-                             bf16 *__restrict__ dx) {
+                             const bf16 *__restrict__ mask, bf16 *__restrict__ dx) {
     const long long n_in = (long long)N * H * W * C;
     const int Ho = H / 2, Wo = W / 2;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n_in; i += (long long)gridDim.x * 256) {
@@ -389,13 +398,15 @@ __global__ void k_pool2_back(const bf16 *__restrict__ dy, int32_t N, int32_t H, 
         const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
         float g = 0.0f;
         if (y / 2 < Ho && x / 2 < Wo) g = 0.25f * b2f(dy[(((size_t)n * Ho + y / 2) * Wo + x / 2) * C + c]);
-        dx[i] = f2b(b2f(dx[i]) + g);
+        // the last contribution to dX: then the gradient through the ReLU
+        // whose output is `mask`
+        dx[i] = (mask && !(b2f(mask[i]) > 0.0f)) ? f2b(0.0f) : f2b(b2f(dx[i]) + g);
     }
 }
 // Split the data gradient of a concat(upsample2(low), skip) input:
 // dLow = 2x2 sum of the first C1 channels, dSkip = the rest (both written).
 __global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H, int32_t W, int32_t C1, int32_t C2,
-                             bf16 *__restrict__ dlow, bf16 *__restrict__ dskip) {
+                             const bf16 *__restrict__ low_mask, bf16 *__restrict__ dlow, bf16 *__restrict__ dskip) {
     const int C = C1 + C2;
     const int Hl = (H + 1) / 2, Wl = (W + 1) / 2;
     const long long n_low = (long long)N * Hl * Wl * C1;
@@ -411,7 +422,8 @@ __global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H
                 const int y = 2 * yl + (k >> 1), x = 2 * xl + (k & 1);
                 if (y < H && x < W) s += b2f(dcat[(((size_t)n * H + y) * W + x) * C + c]);
             }
-            dlow[i] = f2b(s);
+            // dLow is the whole gradient of `low`, the output of a ReLU
+            dlow[i] = (low_mask && !(b2f(low_mask[i]) > 0.0f)) ? f2b(0.0f) : f2b(s);
         } else {
             const long long j = i - n_low;
             const int c = (int)(j % C2);
@@ -441,13 +453,15 @@ __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *
     }
 }
 
-// 1x1 head (32 -> 3, FP32).  The prediction is expm1 of the head output;
+// 1x1 head (32 -> 3, FP32), its input the last ReLU's output.  The
+// prediction is expm1 of the head output;
 // inference writes it clamped at 0.  Pixels whose camera ray missed the scene
 // (depth guide exactly 0) carry the exact environment radiance in every
 // sample, so their input radiance is passed through and they take no part in
 // the loss.  Training uses the relative squared error
 // (pred - target)^2 / (pred^2 + kRelEps) with the denominator held constant
-// (unbiased under noisy targets), writes dL/dX (bf16) and accumulates the head
+// (unbiased under noisy targets), writes dL/dX masked by X > 0 (bf16: the
+// gradient of the last convolution's pre-activation) and accumulates the head
 // gradients and the loss.
 constexpr float kRelEps = 1e-2f;
 
@@ -499,7 +513,9 @@ __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long l
                 s += sw[k * 32 + c] * g[k];
                 gw[k * 33 + c] += g[k] * xi[c];
             }
-            dx[p * 32 + c] = f2b(s);
+            // through the ReLU whose output x is: the gradient of the last
+            // convolution's pre-activation
+            dx[p * 32 + c] = (xi[c] > 0.0f) ? f2b(s) : f2b(0.0f);
         }
         for (int k = 0; k < kOut; ++k) gw[k * 33 + 32] += g[k];
     }
@@ -691,8 +707,10 @@ int conv_prefetch_limit(s3dg_cnn *c) {
     return cached_cus;
 }
 
+// mask (optional): out is the gradient of a ReLU's output, passed on only
+// where that output (mask, same shape as out) is positive.
 void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
-          bf16 *out) {
+          bf16 *out, const bf16 *mask = nullptr) {
     hipStream_t s = c->ctx->stream;
     const unsigned tiles = (unsigned)(((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile));
     const int r = relu ? 1 : 0;
@@ -700,19 +718,19 @@ void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, con
     if (Cout % 64 == 0) {
         dim3 grid(tiles, N * (Cout / 64));
         if ((long long)grid.x * grid.y <= limit)
-            hipLaunchKernelGGL((k_conv3x3<64, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
+            hipLaunchKernelGGL((k_conv3x3<64, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
         else
-            hipLaunchKernelGGL((k_conv3x3<64, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
+            hipLaunchKernelGGL((k_conv3x3<64, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
     } else {
         dim3 grid(tiles, N * ((Cout + 31) / 32));
         if ((long long)grid.x * grid.y <= limit)
-            hipLaunchKernelGGL((k_conv3x3<32, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
+            hipLaunchKernelGGL((k_conv3x3<32, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
         else
-            hipLaunchKernelGGL((k_conv3x3<32, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, out);
+            hipLaunchKernelGGL((k_conv3x3<32, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
     }
 }
 
-void conv_dw(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *dpre, int Cout, float *dw) {
+void conv_dw(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *dpre, int Cout, float *dw, float *db) {
     const int Cin = I.C1 + I.C2;
     const long long ntiles = (long long)N * ((W + kTile - 1) / kTile) * ((H + kDwRows - 1) / kDwRows);
     const int cot = (Cout % 64 == 0) ? 64 : 32;
@@ -721,14 +739,9 @@ void conv_dw(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *dpre
     const long long slices = std::max<long long>(1, std::min<long long>(ntiles, 1024 / blocks_xy + 1));
     dim3 grid((unsigned)blocks_xy, (unsigned)slices);
     if (cot == 64)
-        hipLaunchKernelGGL(k_conv_dw<64>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw);
+        hipLaunchKernelGGL(k_conv_dw<64>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw, db);
     else
-        hipLaunchKernelGGL(k_conv_dw<32>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw);
-}
-
-void bias_grad(s3dg_cnn *c, const bf16 *dpre, long long npix, int C, float *db) {
-    dim3 grid((unsigned)std::max<long long>(1, std::min<long long>(npix / 64 + 1, 1024)), (C + 63) / 64);
-    hipLaunchKernelGGL(k_bias_grad, grid, dim3(256), 0, c->ctx->stream, dpre, npix, C, db);
+        hipLaunchKernelGGL(k_conv_dw<32>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw, db);
 }
 
 ConvIn cin1(const bf16 *a, int C) { return ConvIn{a, nullptr, C, 0, 1}; }
@@ -770,55 +783,59 @@ void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
     conv(c, cin1(a.d0a, 32), N, H, W, L[L_D0B].wf, B(L_D0B), 32, true, a.d0b);
 }
 
-// Backward of one conv layer whose output gradient (w.r.t. its post-ReLU
-// output) is in dy: masks it by the output, accumulates dW and db, and
-// (if dx) writes the input gradient.
-void layer_back(s3dg_cnn *c, int l, const ConvIn &I, int N, int H, int W, bf16 *dy, const bf16 *y, bf16 *dx) {
-    hipStream_t s = c->ctx->stream;
+// Backward of one conv layer whose pre-activation gradient is in dpre (the
+// producers of dpre apply the ReLU mask): accumulates dW and db, and (if dx)
+// writes the input gradient, masked by dx_mask when the input is the output
+// of a ReLU whose whole gradient this is.
+void layer_back(s3dg_cnn *c, int l, const ConvIn &I, int N, int H, int W, const bf16 *dpre, bf16 *dx,
+                const bf16 *dx_mask) {
     ConvLayer &L = c->layer[l];
-    const long long n = (long long)N * H * W;
-    hipLaunchKernelGGL(k_relu_back, dim3(grid1(n * L.cout)), dim3(256), 0, s, dy, y, n * L.cout);
-    conv_dw(c, I, N, H, W, dy, L.cout, c->g + L.w_off);
-    bias_grad(c, dy, n, L.cout, c->g + L.b_off);
-    if (dx) conv(c, cin1(dy, L.cout), N, H, W, L.wb, nullptr, L.cin, false, dx);
+    conv_dw(c, I, N, H, W, dpre, L.cout, c->g + L.w_off, c->g + L.b_off);
+    if (dx) conv(c, cin1(dpre, L.cout), N, H, W, L.wb, nullptr, L.cin, false, dx, dx_mask);
 }
 
-void split(s3dg_cnn *c, const bf16 *dcat, int N, int H, int W, int C1, int C2, bf16 *dlow, bf16 *dskip) {
+// Splits the gradient of a concat(upsample2(low), skip) input; dLow is masked
+// by low (a ReLU output whose whole gradient it is).
+void split(s3dg_cnn *c, const bf16 *dcat, int N, int H, int W, int C1, int C2, const bf16 *low, bf16 *dlow,
+           bf16 *dskip) {
     const long long total = (long long)N * ((H + 1) / 2) * ((W + 1) / 2) * C1 + (long long)N * H * W * C2;
-    hipLaunchKernelGGL(k_split_grad, dim3(grid1(total)), dim3(256), 0, c->ctx->stream, dcat, N, H, W, C1, C2, dlow,
-                       dskip);
+    hipLaunchKernelGGL(k_split_grad, dim3(grid1(total)), dim3(256), 0, c->ctx->stream, dcat, N, H, W, C1, C2, low,
+                       dlow, dskip);
 }
 
-void pool_back(s3dg_cnn *c, const bf16 *dy, int N, int H, int W, int C, bf16 *dx) {
+// dX += upsample(dY) / 4, the last contribution to dX, then masked by the
+// ReLU output `mask`.
+void pool_back(s3dg_cnn *c, const bf16 *dy, int N, int H, int W, int C, const bf16 *mask, bf16 *dx) {
     hipLaunchKernelGGL(k_pool2_back, dim3(grid1((long long)N * H * W * C)), dim3(256), 0, c->ctx->stream, dy, N, H, W,
-                       C, dx);
+                       C, mask, dx);
 }
 
-// Backward pass.  a.g_a holds d(d0b) from the head on entry.  Skip gradients
-// get the decoder's contribution first (the skip buffers are zeroed before),
-// then the pooling path's.
+// Backward pass.  a.g_a holds the gradient of d0b's pre-activation from the
+// head on entry.  Skip gradients get the decoder's contribution first (the
+// skip buffers are zeroed before), then the pooling path's, which applies
+// their ReLU mask.
 void backward(s3dg_cnn *c, Acts &a, int N, int H, int W) {
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-    layer_back(c, L_D0B, cin1(a.d0a, 32), N, H, W, a.g_a, a.d0b, a.g_b);
-    layer_back(c, L_D0A, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, a.g_b, a.d0a, a.g_cat);
-    split(c, a.g_cat, N, H, W, 64, 32, a.g_low, a.g_e0b);
-    layer_back(c, L_D1B, cin1(a.d1a, 64), N, H1, W1, a.g_low, a.d1b, a.g_a);
-    layer_back(c, L_D1A, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, a.g_a, a.d1a, a.g_cat);
-    split(c, a.g_cat, N, H1, W1, 96, 64, a.g_low, a.g_e1b);
-    layer_back(c, L_D2B, cin1(a.d2a, 96), N, H2, W2, a.g_low, a.d2b, a.g_a);
-    layer_back(c, L_D2A, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, a.g_a, a.d2a, a.g_cat);
-    split(c, a.g_cat, N, H2, W2, 128, 96, a.g_low, a.g_e2b);
-    layer_back(c, L_B1, cin1(a.b0, 128), N, H3, W3, a.g_low, a.b1, a.g_a);
-    layer_back(c, L_B0, cin1(a.p2, 96), N, H3, W3, a.g_a, a.b0, a.g_b);
-    pool_back(c, a.g_b, N, H2, W2, 96, a.g_e2b);
-    layer_back(c, L_E2B, cin1(a.e2a, 96), N, H2, W2, a.g_e2b, a.e2b, a.g_a);
-    layer_back(c, L_E2A, cin1(a.p1, 64), N, H2, W2, a.g_a, a.e2a, a.g_b);
-    pool_back(c, a.g_b, N, H1, W1, 64, a.g_e1b);
-    layer_back(c, L_E1B, cin1(a.e1a, 64), N, H1, W1, a.g_e1b, a.e1b, a.g_a);
-    layer_back(c, L_E1A, cin1(a.p0, 32), N, H1, W1, a.g_a, a.e1a, a.g_b);
-    pool_back(c, a.g_b, N, H, W, 32, a.g_e0b);
-    layer_back(c, L_E0B, cin1(a.e0a, 32), N, H, W, a.g_e0b, a.e0b, a.g_a);
-    layer_back(c, L_E0A, cin1(a.x, kInPadded), N, H, W, a.g_a, a.e0a, nullptr);
+    layer_back(c, L_D0B, cin1(a.d0a, 32), N, H, W, a.g_a, a.g_b, a.d0a);
+    layer_back(c, L_D0A, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, a.g_b, a.g_cat, nullptr);
+    split(c, a.g_cat, N, H, W, 64, 32, a.d1b, a.g_low, a.g_e0b);
+    layer_back(c, L_D1B, cin1(a.d1a, 64), N, H1, W1, a.g_low, a.g_a, a.d1a);
+    layer_back(c, L_D1A, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, a.g_a, a.g_cat, nullptr);
+    split(c, a.g_cat, N, H1, W1, 96, 64, a.d2b, a.g_low, a.g_e1b);
+    layer_back(c, L_D2B, cin1(a.d2a, 96), N, H2, W2, a.g_low, a.g_a, a.d2a);
+    layer_back(c, L_D2A, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, a.g_a, a.g_cat, nullptr);
+    split(c, a.g_cat, N, H2, W2, 128, 96, a.b1, a.g_low, a.g_e2b);
+    layer_back(c, L_B1, cin1(a.b0, 128), N, H3, W3, a.g_low, a.g_a, a.b0);
+    layer_back(c, L_B0, cin1(a.p2, 96), N, H3, W3, a.g_a, a.g_b, nullptr);
+    pool_back(c, a.g_b, N, H2, W2, 96, a.e2b, a.g_e2b);
+    layer_back(c, L_E2B, cin1(a.e2a, 96), N, H2, W2, a.g_e2b, a.g_a, a.e2a);
+    layer_back(c, L_E2A, cin1(a.p1, 64), N, H2, W2, a.g_a, a.g_b, nullptr);
+    pool_back(c, a.g_b, N, H1, W1, 64, a.e1b, a.g_e1b);
+    layer_back(c, L_E1B, cin1(a.e1a, 64), N, H1, W1, a.g_e1b, a.g_a, a.e1a);
+    layer_back(c, L_E1A, cin1(a.p0, 32), N, H1, W1, a.g_a, a.g_b, nullptr);
+    pool_back(c, a.g_b, N, H, W, 32, a.e0b, a.g_e0b);
+    layer_back(c, L_E0B, cin1(a.e0a, 32), N, H, W, a.g_e0b, a.g_a, a.e0a);
+    layer_back(c, L_E0A, cin1(a.x, kInPadded), N, H, W, a.g_a, nullptr, nullptr);
 }
 
 int check_dims(s3dg_context *ctx, int n, int h, int w) {
