@@ -23,6 +23,7 @@ using namespace s3dg;
 
 typedef __bf16 bf16;
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
@@ -238,132 +239,224 @@ __global__ __launch_bounds__(256, PF ? 1 : (COT == 32 ? 3 : 2)) void k_conv3x3(C
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: dW[tap][co][ci] += sum_p dPre[p][co] * X[p + off(tap)][ci],
-// a GEMM per tap with M = co, N = ci, K = pixels, on
-// v_mfma_f32_16x16x32_bf16.  A workgroup owns COT output channels x 32 input
-// channels x all 9 taps and walks a strided set of 8 x 16 pixel tiles.  Per
-// tile it stages dPre transposed ([co][pixel]) and the 10 x 18 input halo
-// transposed in three copies shifted by 0, 1 and 2 columns ([shift][ci][row]
-// [16 columns]), so that every MFMA operand is one aligned 16-byte LDS read
-// (8 consecutive pixels of one channel) whatever the tap.  Wave w accumulates
-// 16 output channels x CT 16-channel input tiles x 9 taps in registers and
-// adds them to dW with FP32 atomics at the end.
+// Weight gradient: dW[tap][co][ci] += sum_p dPre[p][co] * X[p + off(tap)][ci]
+// and db[co] += sum_p dPre[p][co]: a GEMM per tap with M = co, N = ci,
+// K = pixels, on v_mfma_f32_32x32x16_bf16.  Both operands need K (pixels)
+// consecutive in a lane while NHWC keeps channels consecutive, so the tiles
+// are staged in LDS as they lie in memory (16-byte copies) and read with the
+// gfx950 transposing LDS load ds_read_b64_tr_b16: in each group of 16 lanes,
+// lane l receives element l % 4 of the 8-byte rows addressed by lanes
+// 4k + l / 4, k = 0..3.  A lane that addresses pixel kr + 4h, channels
+// cc .. cc+3 therefore delivers, after the exchange, 4 consecutive pixels of
+// one channel, and two such loads make one MFMA operand.
+//
+// A workgroup owns 32 output x 32 input channels x all 9 taps and walks a
+// strided set of 16 x 16 pixel tiles; wave w takes tile rows 4w .. 4w+3 as
+// the K steps (one 16-pixel row each).  Input rows r + dy serve taps dy = 0..2
+// of three consecutive output rows, so each K step loads one new halo row
+// (three column shifts) and the dPre row: 8 transposing loads per 9 MFMAs.
+// The bias gradient is one more MFMA per K step, against a ones operand, in
+// the workgroups of the first input-channel block.  The four waves' partial
+// sums meet in LDS; one atomic add per weight and workgroup.
 // ---------------------------------------------------------------------------
-constexpr int kDwRows = 8;                       // pixel tile: 8 rows x 16 columns
-constexpr int kDwPix = kDwRows * kTile;          // 128
-constexpr int kDwPitch = kDwPix + 8;             // dPre row (one channel), padded
-constexpr int kDwCi = 200;                       // one channel of a halo copy: 10 rows x 16 + padding
+constexpr int kDwPitch = 48;   // LDS pixel row: 32 channels + 16 bf16 padding (96 B, 4 pixels on distinct banks)
 
-template <int COT>
-__global__ __launch_bounds__(256) void k_conv_dw(ConvIn I, int32_t N, int32_t H, int32_t W,
-                                                  const bf16 *__restrict__ dpre, int32_t Cout, float *__restrict__ dw,
-                                                  float *__restrict__ db) {
-    constexpr int CT = COT == 64 ? 2 : 1;        // input-channel tiles of 16 per wave
-    __shared__ __attribute__((aligned(16))) bf16 s_dy[COT * kDwPitch];
-    __shared__ __attribute__((aligned(16))) bf16 s_x[3 * 32 * kDwCi];
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) v4s lds_v4s;
+
+// 8 bf16 of one MFMA operand: two transposing loads, the second `4 * step`
+// elements (4 pixels) after the first.  Assembled from whole dwords: built
+// element by element from the two short4 results, the operand came out with
+// its dwords duplicated.
+__device__ __forceinline__ bf16x8 tr_operand(const bf16 *p, int step) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    const u32x2 lo = __builtin_bit_cast(u32x2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(p)));
+    const u32x2 hi = __builtin_bit_cast(u32x2, __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s *)(p + 4 * step)));
+    const u32x4 w = {lo.x, lo.y, hi.x, hi.y};
+    return __builtin_bit_cast(bf16x8, w);
+}
+
+__global__ __launch_bounds__(256, 2) void k_conv_dw(ConvIn I, int32_t N, int32_t H, int32_t W,
+                                                     const bf16 *__restrict__ dpre, int32_t Cout, int32_t bias_grad,
+                                                     float *__restrict__ part, float *__restrict__ bpart) {
+    // one array: the cross-wave reduction at the end reuses both tiles
+    __shared__ __attribute__((aligned(16))) bf16 smem[(kTile * kTile + kHalo * kHalo) * kDwPitch];
+    bf16 *const s_d = smem;                              // dPre tile [pixel][32 co]
+    bf16 *const s_x = smem + kTile * kTile * kDwPitch;   // input halo [pixel][32 ci]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int Cin = I.C1 + I.C2;
-    const int nchunk = Cin / 32;
-    const int ci0 = (blockIdx.x % nchunk) * 32;
-    const int co0 = (blockIdx.x / nchunk) * COT;
-    const int cot = COT == 64 ? wave : (wave & 1);   // this wave's 16-channel output tile
-    const int ct0 = COT == 64 ? 0 : (wave >> 1);     // first input-channel tile
-    const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kDwRows - 1) / kDwRows;
+    const int nci = Cin / 32;
+    const int ci0 = (blockIdx.x % nci) * 32;
+    const int co0 = (blockIdx.x / nci) * 32;
+    const bool bias = bias_grad && ci0 == 0;
+    const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
     const long long ntiles = (long long)N * tiles_x * tiles_y;
-    f32x4 acc[9][CT];
+    // transposing-load addressing of this lane: pixel kr (+4 for the second
+    // load) of a 16-pixel K step, channels cc .. cc+3 of the 32-channel block
+    const int j = lane & 15, g = lane >> 4;
+    const int kr = 8 * (g >> 1) + (j >> 2);
+    const int cc = 16 * (g & 1) + 4 * (j & 3);
+    f32x16 acc[9], accb;
 #pragma unroll
     for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int c = 0; c < CT; ++c)
+        for (int i = 0; i < 16; ++i) acc[t][i] = 0.0f;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) acc[t][c][i] = 0.0f;
-    const bf16 z = f2b(0.0f);
-    // bias gradient (sum of dPre over pixels), by the workgroups of the first
-    // input-channel chunk: thread t sums segment t % SEGS of channel t / SEGS
-    // of every staged tile
-    constexpr int SEGS = 256 / COT, SEG_PIX = kDwPix / SEGS;
-    const bool bias_wg = db && ci0 == 0;
-    float bsum = 0.0f;
+    for (int i = 0; i < 16; ++i) accb[i] = 0.0f;
+    bf16x8 ones, zero;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        ones[e] = f2b(1.0f);
+        zero[e] = f2b(0.0f);
+    }
+    constexpr int kDCh = kTile * kTile * 4, kDPer = kDCh / 256;              // 16-byte chunks of the dPre tile
+    constexpr int kXCh = kHalo * kHalo * 4, kXPer = (kXCh + 255) / 256;      // ... of the halo
     for (long long tile = blockIdx.y; tile < ntiles; tile += gridDim.y) {
         const int n = (int)(tile / ((long long)tiles_x * tiles_y));
         const int rem = (int)(tile % ((long long)tiles_x * tiles_y));
-        const int y0 = (rem / tiles_x) * kDwRows, x0 = (rem % tiles_x) * kTile;
-        // dPre tile, transposed
-        for (int e = threadIdx.x; e < kDwPix * (COT / 8); e += 256) {
-            const int px = e / (COT / 8), q = e % (COT / 8);
+        const int y0 = (rem / tiles_x) * kTile, x0 = (rem % tiles_x) * kTile;
+        // every global load of the tile is issued before the first LDS store
+        bf16x8 dv[kDPer], xv[kXPer];
+#pragma unroll
+        for (int k = 0; k < kDPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            const int px = e >> 2, q = e & 3;
             const int y = y0 + px / kTile, x = x0 + px % kTile;
-            const int co = co0 + q * 8;
-            bf16x8 v;
-            if (y < H && x < W && co < Cout) {
-                v = *reinterpret_cast<const bf16x8 *>(dpre + (((size_t)n * H + y) * W + x) * Cout + co);
-            } else {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) v[j] = z;
-            }
-#pragma unroll
-            for (int j = 0; j < 8; ++j) s_dy[(q * 8 + j) * kDwPitch + px] = v[j];
+            dv[k] = zero;
+            if (y < H && x < W) dv[k] = *reinterpret_cast<const bf16x8 *>(dpre + (((size_t)n * H + y) * W + x) * Cout + co0 + 8 * q);
         }
-        // input halo (rows y0-1 .. y0+8, columns x0-1 .. x0+16), transposed, three shifted copies
-        for (int e = threadIdx.x; e < (kDwRows + 2) * (kTile + 2) * 4; e += 256) {
-            const int hp = e >> 2, q = e & 3;
-            const int hy = hp / (kTile + 2), hx = hp % (kTile + 2);
-            const bf16x8 v = load8(I, n, y0 + hy - 1, x0 + hx - 1, H, W, ci0 + q * 8);
 #pragma unroll
-            for (int sh = 0; sh < 3; ++sh) {
-                const int j = hx - sh;
-                if (j < 0 || j >= kTile) continue;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) s_x[(sh * 32 + q * 8 + k) * kDwCi + hy * kTile + j] = v[k];
+        for (int k = 0; k < kXPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kXCh) {
+                const int hp = e >> 2, q = e & 3;
+                xv[k] = load8(I, n, y0 + hp / kHalo - 1, x0 + hp % kHalo - 1, H, W, ci0 + 8 * q);
             }
+        }
+#pragma unroll
+        for (int k = 0; k < kDPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            *reinterpret_cast<bf16x8 *>(&s_d[(e >> 2) * kDwPitch + (e & 3) * 8]) = dv[k];
+        }
+#pragma unroll
+        for (int k = 0; k < kXPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kXCh) *reinterpret_cast<bf16x8 *>(&s_x[(e >> 2) * kDwPitch + (e & 3) * 8]) = xv[k];
         }
         __syncthreads();
-        if (bias_wg) {
-            const bf16 *row = &s_dy[(threadIdx.x / SEGS) * kDwPitch + (threadIdx.x % SEGS) * SEG_PIX];
+        // K steps: output rows r0 .. r0+3; xw[i][dx] holds halo row r0 + i
+        // (output row r0 + i - 1) shifted by dx
+        const int r0 = 4 * wave;
+        bf16x8 xw[6][3];
 #pragma unroll
-            for (int k = 0; k < SEG_PIX; k += 8) {
-                const bf16x8 v = *reinterpret_cast<const bf16x8 *>(row + k);
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int j = 0; j < 8; ++j) bsum += b2f(v[j]);
-            }
-        }
-#pragma unroll 1
-        for (int ks = 0; ks < kDwPix / 32; ++ks) {
-            const int g = lane >> 4;                    // 8-pixel group of this lane
-            const int px = 32 * ks + 8 * g;
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&s_dy[(16 * cot + (lane & 15)) * kDwPitch + px]);
-            const int r = 2 * ks + (g >> 1), c = 8 * (g & 1);
+            for (int dx = 0; dx < 3; ++dx)
+                xw[i][dx] = tr_operand(&s_x[((r0 + i) * kHalo + kr + dx) * kDwPitch + cc], kDwPitch);
 #pragma unroll
-            for (int tap = 0; tap < 9; ++tap) {
-                const int dy = tap / 3, sh = tap % 3;   // halo row offset, column shift
+        for (int i = 0; i < 4; ++i) {
 #pragma unroll
-                for (int ct = 0; ct < CT; ++ct) {
-                    const int ci = 16 * (ct0 + ct) + (lane & 15);
-                    const bf16x8 b =
-                        *reinterpret_cast<const bf16x8 *>(&s_x[(sh * 32 + ci) * kDwCi + (r + dy) * kTile + c]);
-                    acc[tap][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tap][ct], 0, 0, 0);
-                }
-            }
+            for (int dx = 0; dx < 3; ++dx)
+                xw[i + 2][dx] = tr_operand(&s_x[((r0 + i + 2) * kHalo + kr + dx) * kDwPitch + cc], kDwPitch);
+            const bf16x8 a = tr_operand(&s_d[((r0 + i) * kTile + kr) * kDwPitch + cc], kDwPitch);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx)
+                    acc[dy * 3 + dx] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, xw[i + dy][dx], acc[dy * 3 + dx], 0, 0, 0);
+            if (bias) accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, ones, accb, 0, 0, 0);
         }
         __syncthreads();
     }
-    // D layout: row (output channel) = 4*(lane/16) + i, column (input channel) = lane % 16
+    // the four waves' sums: waves 1..3 in turn through LDS (the tile buffers
+    // are free now) into wave 0
+    float *red = reinterpret_cast<float *>(smem);
+    static_assert(10 * 16 * 64 * sizeof(float) <= sizeof(smem), "reduction buffer");
+    for (int w = 1; w < 4; ++w) {
+        if (wave == w) {
 #pragma unroll
-    for (int tap = 0; tap < 9; ++tap)
+            for (int t = 0; t < 10; ++t)
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
+                for (int i = 0; i < 16; i += 4) {
+                    const f32x16 &v = t < 9 ? acc[t] : accb;
+                    *reinterpret_cast<f32x4 *>(&red[((t * 4 + i / 4) * 64 + lane) * 4]) =
+                        f32x4{v[i], v[i + 1], v[i + 2], v[i + 3]};
+                }
+        }
+        __syncthreads();
+        if (wave == 0) {
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int co = co0 + 16 * cot + 4 * (lane >> 4) + i;
-                const int ci = ci0 + 16 * (ct0 + ct) + (lane & 15);
-                const float v = acc[tap][ct][i];
-                if (co < Cout && v != 0.0f) atomicAdd(&dw[((size_t)tap * Cout + co) * Cin + ci], v);
-            }
-    if (bias_wg) {
-        // the SEGS segments of a channel are adjacent lanes
+            for (int t = 0; t < 10; ++t)
 #pragma unroll
-        for (int off = 1; off < SEGS; off <<= 1) bsum += __shfl_xor(bsum, off);
-        const int co = co0 + (int)threadIdx.x / SEGS;
-        if (threadIdx.x % SEGS == 0 && co < Cout && bsum != 0.0f) atomicAdd(&db[co], bsum);
+                for (int i = 0; i < 16; i += 4) {
+                    const f32x4 v = *reinterpret_cast<const f32x4 *>(&red[((t * 4 + i / 4) * 64 + lane) * 4]);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (t < 9) acc[t][i + k] += v[k];
+                        else accb[i + k] += v[k];
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    if (wave != 0) return;
+    // this slice's partial sums (plain stores; k_conv_dw_reduce adds the
+    // slices up: same-address atomics from hundreds of workgroups serialise).
+    // D layout: row (output channel) = 8*(i/4) + 4*(lane/32) + i%4, column
+    // (input channel) = lane % 32
+    float *pw = part + (size_t)blockIdx.y * 9 * Cout * Cin;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int co = co0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+            pw[((size_t)t * Cout + co) * Cin + ci0 + (lane & 31)] = acc[t][i];
+        }
+    // bias: every column of accb is the row sum
+    if (bias && (lane & 31) == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+            bpart[(size_t)blockIdx.y * Cout + co0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3)] = accb[i];
+    }
+}
+
+// dst[i] += sum over slices s of part[s * n + i], i < n (n a multiple of 4),
+// for two arrays at once (workgroups [0, blocks_a) take the first): 16 float4
+// columns per workgroup and 16 groups of slices (a slice every 16th), four
+// independent loads in flight per thread, the groups summed in a fixed order.
+struct SliceSum {
+    const float *part;
+    int32_t slices, n;
+    float *dst;
+};
+
+__global__ __launch_bounds__(256) void k_slice_sum(SliceSum a, SliceSum b, int32_t blocks_a) {
+    __shared__ f32x4 red[16][16];
+    const bool first = (int)blockIdx.x < blocks_a;
+    const SliceSum &S = first ? a : b;
+    const int col = (first ? (int)blockIdx.x : (int)blockIdx.x - blocks_a) * 16 + (threadIdx.x & 15);
+    const int grp = threadIdx.x >> 4;
+    f32x4 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    if (4 * col < S.n) {
+        const float *src = S.part + 4 * col;
+        int k = grp;
+        for (; k + 48 < S.slices; k += 64) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] += *reinterpret_cast<const f32x4 *>(src + (size_t)(k + 16 * u) * S.n);
+        }
+        for (; k < S.slices; k += 16) acc[0] += *reinterpret_cast<const f32x4 *>(src + (size_t)k * S.n);
+    }
+    red[grp][threadIdx.x & 15] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (grp == 0 && 4 * col < S.n) {
+        f32x4 t = red[0][threadIdx.x];
+#pragma unroll
+        for (int g = 1; g < 16; ++g) t += red[g][threadIdx.x];
+        f32x4 *d = reinterpret_cast<f32x4 *>(S.dst + 4 * col);
+        *d += t;
     }
 }
 
@@ -387,48 +480,72 @@ __global__ void k_pool2(const bf16 *__restrict__ x, int32_t N, int32_t H, int32_
         *reinterpret_cast<bf16x8 *>(y + (size_t)p * C + 8 * g) = o;
     }
 }
-// dX += upsample(dY) / 4
+// dX += upsample(dY) / 4, the last contribution to dX, then (with mask) the
+// gradient through the ReLU whose output is `mask`.  One thread per pixel and
+// group of 8 channels (16-byte accesses), 32-bit indexing (N*H*W <= 2^27,
+// C <= 224).
 __global__ void k_pool2_back(const bf16 *__restrict__ dy, int32_t N, int32_t H, int32_t W, int32_t C,
                              const bf16 *__restrict__ mask, bf16 *__restrict__ dx) {
-    const long long n_in = (long long)N * H * W * C;
-    const int Ho = H / 2, Wo = W / 2;
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n_in; i += (long long)gridDim.x * 256) {
-        const int c = (int)(i % C);
-        const long long p = i / C;
-        const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((long long)W * H));
-        float g = 0.0f;
-        if (y / 2 < Ho && x / 2 < Wo) g = 0.25f * b2f(dy[(((size_t)n * Ho + y / 2) * Wo + x / 2) * C + c]);
-        // the last contribution to dX: then the gradient through the ReLU
-        // whose output is `mask`
-        dx[i] = (mask && !(b2f(mask[i]) > 0.0f)) ? f2b(0.0f) : f2b(b2f(dx[i]) + g);
+    const uint32_t G = (uint32_t)C / 8, Ho = (uint32_t)H / 2, Wo = (uint32_t)W / 2;
+    const uint32_t n = (uint32_t)N * H * W * G;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint32_t g = i % G, p = i / G;
+        const uint32_t x = p % (uint32_t)W, y = (p / (uint32_t)W) % (uint32_t)H, b = p / ((uint32_t)W * H);
+        const size_t off = (size_t)p * C + 8 * g;
+        bf16x8 v = *reinterpret_cast<const bf16x8 *>(dx + off);
+        bf16x8 d;
+        if (y / 2 < Ho && x / 2 < Wo) {
+            d = *reinterpret_cast<const bf16x8 *>(dy + (((size_t)b * Ho + y / 2) * Wo + x / 2) * C + 8 * g);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) d[j] = f2b(0.0f);
+        }
+        bf16x8 m;
+        if (mask) m = *reinterpret_cast<const bf16x8 *>(mask + off);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            v[j] = (mask && !(b2f(m[j]) > 0.0f)) ? f2b(0.0f) : f2b(b2f(v[j]) + 0.25f * b2f(d[j]));
+        *reinterpret_cast<bf16x8 *>(dx + off) = v;
     }
 }
 // Split the data gradient of a concat(upsample2(low), skip) input:
-// dLow = 2x2 sum of the first C1 channels, dSkip = the rest (both written).
+// dLow = 2x2 sum of the first C1 channels (with low_mask: zero where that
+// ReLU output is not positive, dLow being its whole gradient), dSkip = the
+// rest.  One thread per (low pixel or skip pixel, group of 8 channels),
+// 16-byte accesses, 32-bit indexing (N*H*W <= 2^27, C1 <= 128, C2 <= 96).
 __global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H, int32_t W, int32_t C1, int32_t C2,
                              const bf16 *__restrict__ low_mask, bf16 *__restrict__ dlow, bf16 *__restrict__ dskip) {
-    const int C = C1 + C2;
-    const int Hl = (H + 1) / 2, Wl = (W + 1) / 2;
-    const long long n_low = (long long)N * Hl * Wl * C1;
-    const long long n_skip = (long long)N * H * W * C2;
-    const long long total = n_low + n_skip;
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < total; i += (long long)gridDim.x * 256) {
+    const uint32_t C = (uint32_t)(C1 + C2), G1 = (uint32_t)C1 / 8, G2 = (uint32_t)C2 / 8;
+    const uint32_t Hl = (uint32_t)(H + 1) / 2, Wl = (uint32_t)(W + 1) / 2;
+    const uint32_t n_low = (uint32_t)N * Hl * Wl * G1;
+    const uint32_t total = n_low + (uint32_t)N * H * W * G2;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < total; i += gridDim.x * 256u) {
         if (i < n_low) {
-            const int c = (int)(i % C1);
-            const long long p = i / C1;
-            const int xl = (int)(p % Wl), yl = (int)((p / Wl) % Hl), n = (int)(p / ((long long)Wl * Hl));
-            float s = 0.0f;
+            const uint32_t g = i % G1, p = i / G1;
+            const uint32_t xl = p % Wl, yl = (p / Wl) % Hl, b = p / (Wl * Hl);
+            float s[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s[j] = 0.0f;
+#pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int y = 2 * yl + (k >> 1), x = 2 * xl + (k & 1);
-                if (y < H && x < W) s += b2f(dcat[(((size_t)n * H + y) * W + x) * C + c]);
+                const uint32_t y = 2 * yl + (k >> 1), x = 2 * xl + (k & 1);
+                if (y < (uint32_t)H && x < (uint32_t)W) {
+                    const bf16x8 v =
+                        *reinterpret_cast<const bf16x8 *>(dcat + (((size_t)b * H + y) * W + x) * C + 8 * g);
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) s[j] += b2f(v[j]);
+                }
             }
-            // dLow is the whole gradient of `low`, the output of a ReLU
-            dlow[i] = (low_mask && !(b2f(low_mask[i]) > 0.0f)) ? f2b(0.0f) : f2b(s);
+            const size_t off = (size_t)p * C1 + 8 * g;
+            bf16x8 m, o;
+            if (low_mask) m = *reinterpret_cast<const bf16x8 *>(low_mask + off);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) o[j] = (low_mask && !(b2f(m[j]) > 0.0f)) ? f2b(0.0f) : f2b(s[j]);
+            *reinterpret_cast<bf16x8 *>(dlow + off) = o;
         } else {
-            const long long j = i - n_low;
-            const int c = (int)(j % C2);
-            const long long p = j / C2;
-            dskip[j] = dcat[p * C + C1 + c];
+            const uint32_t j = i - n_low, g = j % G2, p = j / G2;
+            *reinterpret_cast<bf16x8 *>(dskip + (size_t)p * C2 + 8 * g) =
+                *reinterpret_cast<const bf16x8 *>(dcat + (size_t)p * C + C1 + 8 * g);
         }
     }
 }
@@ -465,74 +582,160 @@ __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *
 // gradients and the loss.
 constexpr float kRelEps = 1e-2f;
 
+// Eight lanes per pixel, each holding 4 of its 32 input channels (8-byte
+// loads and stores, 32 pixels per workgroup pass); the head sums are
+// completed across the eight lanes with shuffles, and the head gradients and
+// loss are reduced per wave, then per workgroup, into one partial per
+// workgroup (no atomics: thousands of workgroups adding into the same hundred
+// addresses serialise).
 __global__ __launch_bounds__(256) void k_head(const bf16 *__restrict__ x, long long npix, const float *__restrict__ wh,
                                                const float *__restrict__ bh, const float *__restrict__ feat,
                                                const float *__restrict__ target,
-                                               float *__restrict__ out, bf16 *__restrict__ dx, float *__restrict__ dwh,
-                                               float *__restrict__ dbh, double *__restrict__ loss, float inv_count) {
-    __shared__ float sw[kOut * 32 + kOut];
-    __shared__ float sg[256][kOut * 33];
-    __shared__ double sl[256];
-    for (int i = threadIdx.x; i < kOut * 32; i += 256) sw[i] = wh[i];
-    if (threadIdx.x < kOut) sw[kOut * 32 + threadIdx.x] = bh[threadIdx.x];
-    __syncthreads();
-    float gw[kOut * 33];
-    for (int k = 0; k < kOut * 33; ++k) gw[k] = 0.0f;
-    double l = 0.0;
-    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
-        float xi[32];
-        for (int c = 0; c < 32; ++c) xi[c] = b2f(x[p * 32 + c]);
-        float o[kOut];
-        for (int k = 0; k < kOut; ++k) {
-            float s = sw[kOut * 32 + k];
-            for (int c = 0; c < 32; ++c) s += sw[k * 32 + c] * xi[c];
-            o[k] = s;
+                                               float *__restrict__ out, bf16 *__restrict__ dx,
+                                               float *__restrict__ part, double *__restrict__ lpart, float inv_count) {
+    __shared__ float sg[4][kOut * 33];
+    __shared__ double sl[4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int q = threadIdx.x & 7;   // channels 4q .. 4q+3
+    float w[kOut][4], bias[kOut], gw[kOut][4], gb[kOut];
+#pragma unroll
+    for (int k = 0; k < kOut; ++k) {
+        bias[k] = bh[k];
+        gb[k] = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            w[k][j] = wh[k * 32 + 4 * q + j];
+            gw[k][j] = 0.0f;
         }
-        const bool miss = feat[p * kInChannels + 9] == 0.0f;
-        if (miss) {
-            for (int k = 0; k < kOut; ++k)
-                if (out) out[p * kOut + k] = feat[p * kInChannels + k];
-            for (int c = 0; c < 32; ++c) dx[p * 32 + c] = f2b(0.0f);
-            continue;
-        }
-        float g[kOut];
-        for (int k = 0; k < kOut; ++k) {
-            const float t = fmaxf(target[p * kOut + k], 0.0f);
-            // outputs beyond e^30 are treated as e^30 (keeps pred^2 finite)
-            const float oc = fminf(o[k], 30.0f);
-            const float pred = expm1f(oc);
-            const float d = pred - t;
-            const float den = pred * pred + kRelEps;
-            l += (double)(d * d / den);
-            g[k] = 2.0f * d / den * expf(oc) * inv_count;
-            if (out) out[p * kOut + k] = fmaxf(pred, 0.0f);
-        }
-        for (int c = 0; c < 32; ++c) {
-            float s = 0.0f;
-            for (int k = 0; k < kOut; ++k) {
-                s += sw[k * 32 + c] * g[k];
-                gw[k * 33 + c] += g[k] * xi[c];
-            }
-            // through the ReLU whose output x is: the gradient of the last
-            // convolution's pre-activation
-            dx[p * 32 + c] = (xi[c] > 0.0f) ? f2b(s) : f2b(0.0f);
-        }
-        for (int k = 0; k < kOut; ++k) gw[k * 33 + 32] += g[k];
     }
-    for (int k = 0; k < kOut * 33; ++k) sg[threadIdx.x][k] = gw[k];
-    sl[threadIdx.x] = l;
+    double l = 0.0;
+    // four pixels per lane group and pass, their loads issued together
+    constexpr int U = 4;
+    const long long stride = gridDim.x * 32LL;
+    for (long long p0 = blockIdx.x * 32LL + (threadIdx.x >> 3); p0 < npix; p0 += U * stride) {
+        bf16x4 xv[U];
+        float depth[U], tg[U][kOut];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long p = p0 + u * stride;
+            if (p < npix) {
+                xv[u] = *reinterpret_cast<const bf16x4 *>(x + p * 32 + 4 * q);
+                depth[u] = feat[p * kInChannels + 9];
+#pragma unroll
+                for (int k = 0; k < kOut; ++k) tg[u][k] = target[p * kOut + k];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long p = p0 + u * stride;
+            if (p >= npix) break;   // the same for the eight lanes of a pixel
+            float xi[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) xi[j] = b2f(xv[u][j]);
+            float o[kOut];
+#pragma unroll
+            for (int k = 0; k < kOut; ++k) {
+                float s = 0.0f;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) s += w[k][j] * xi[j];
+                // the eight lanes of this pixel
+                s += __shfl_xor(s, 1);
+                s += __shfl_xor(s, 2);
+                s += __shfl_xor(s, 4);
+                o[k] = s + bias[k];
+            }
+            bf16x4 dv;
+            if (depth[u] == 0.0f) {
+                if (out && q < kOut) out[p * kOut + q] = feat[p * kInChannels + q];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) dv[j] = f2b(0.0f);
+            } else {
+                float g[kOut];
+#pragma unroll
+                for (int k = 0; k < kOut; ++k) {
+                    const float t = fmaxf(tg[u][k], 0.0f);
+                    // outputs beyond e^30 are treated as e^30 (keeps pred^2 finite)
+                    const float oc = fminf(o[k], 30.0f);
+                    const float pred = expm1f(oc);
+                    const float d = pred - t;
+                    const float den = pred * pred + kRelEps;
+                    if (q == 0) l += (double)(d * d / den);
+                    g[k] = 2.0f * d / den * expf(oc) * inv_count;
+                    if (out && q == k) out[p * kOut + k] = fmaxf(pred, 0.0f);
+                    if (q == 0) gb[k] += g[k];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    float s = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < kOut; ++k) {
+                        s += w[k][j] * g[k];
+                        gw[k][j] += g[k] * xi[j];
+                    }
+                    // through the ReLU whose output x is: the gradient of the
+                    // last convolution's pre-activation
+                    dv[j] = (xi[j] > 0.0f) ? f2b(s) : f2b(0.0f);
+                }
+            }
+            *reinterpret_cast<bf16x4 *>(dx + p * 32 + 4 * q) = dv;
+        }
+    }
+    // lanes q, q + 8, ..., q + 56 hold the same channels
+#pragma unroll
+    for (int off = 8; off < 64; off <<= 1) {
+#pragma unroll
+        for (int k = 0; k < kOut; ++k) {
+            gb[k] += __shfl_xor(gb[k], off);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) gw[k][j] += __shfl_xor(gw[k][j], off);
+        }
+        l += __shfl_xor(l, off);
+    }
+    if (lane < 8) {
+#pragma unroll
+        for (int k = 0; k < kOut; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sg[wave][k * 33 + 4 * lane + j] = gw[k][j];
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < kOut; ++k) sg[wave][k * 33 + 32] = gb[k];
+        sl[wave] = l;
+    }
     __syncthreads();
-    if (threadIdx.x < kOut * 33) {
-        float s = 0.0f;
-        for (int t = 0; t < 256; ++t) s += sg[t][threadIdx.x];
-        const int k = threadIdx.x / 33, c = threadIdx.x % 33;
-        if (c < 32) atomicAdd(&dwh[k * 32 + c], s);
-        else atomicAdd(&dbh[k], s);
+    // this workgroup's sums, added up by k_head_reduce
+    if (threadIdx.x < kOut * 33)
+        part[(size_t)blockIdx.x * (kOut * 33) + threadIdx.x] =
+            sg[0][threadIdx.x] + sg[1][threadIdx.x] + sg[2][threadIdx.x] + sg[3][threadIdx.x];
+    if (threadIdx.x == 0) lpart[blockIdx.x] = sl[0] + sl[1] + sl[2] + sl[3];
+}
+
+// Sums k_head's per-workgroup partials: workgroup v < kOut * 33 adds head
+// gradient v (weight k * 32 + c, or bias k for c = 32) into dwh / dbh,
+// workgroup kOut * 33 adds the loss.  Fixed order, so the sums are
+// reproducible; one writer per value.
+__global__ __launch_bounds__(256) void k_head_reduce(const float *__restrict__ part, const double *__restrict__ lpart,
+                                                     int32_t nwg, float *__restrict__ dwh, float *__restrict__ dbh,
+                                                     double *__restrict__ loss) {
+    __shared__ double red[256];
+    const int v = blockIdx.x;
+    double s = 0.0;
+    for (int w = threadIdx.x; w < nwg; w += 256)
+        s += v < kOut * 33 ? (double)part[(size_t)w * (kOut * 33) + v] : lpart[w];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
     }
     if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int t = 0; t < 256; ++t) s += sl[t];
-        atomicAdd(loss, s);
+        if (v == kOut * 33) {
+            *loss += red[0];
+        } else {
+            const int k = v / 33, c = v % 33;
+            if (c < 32) dwh[k * 32 + c] += (float)red[0];
+            else dbh[k] += (float)red[0];
+        }
     }
 }
 
@@ -655,6 +858,20 @@ struct Acts {
     bf16 *x, *e0a, *e0b, *p0, *e1a, *e1b, *p1, *e2a, *e2b, *p2, *b0, *b1, *d2a, *d2b, *d1a, *d1b, *d0a, *d0b;
     // gradients
     bf16 *g_cat, *g_a, *g_b, *g_e0b, *g_e1b, *g_e2b, *g_low;
+    // head partial sums, one row per k_head workgroup
+    float *head_part;
+    double *loss_part;
+    // weight-gradient partial sums, one block per k_conv_dw slice
+    float *dw_part, *db_part;
+};
+
+constexpr int kHeadWGs = 2048;   // k_head workgroups at most
+constexpr int kDwWGs = 512;      // k_conv_dw workgroups (slices x channel blocks) at most
+constexpr long long kDwPartFloats = (long long)kDwWGs * 9 * 32 * 32;   // slices x 9 x Cout x Cin <= this
+constexpr long long kDbPartFloats = (long long)kDwWGs * 32;            // slices x Cout <= this
+
+struct DwParts {
+    float *w, *b;
 };
 
 size_t act_bytes(long long n0) {
@@ -668,7 +885,8 @@ size_t act_bytes(long long n0) {
     e += n2 * 96 * 2 + n1 * 64 * 2 + n0 * 32 * 2;  // decoder
     // gradients: concat buffer (largest: full res x 96), two layer buffers, three skips, low
     e += n0 * 224 + 2 * n0 * 96 + n0 * 32 + n1 * 64 + n2 * 96 + n0 * 128;
-    return (size_t)e * 2 + 64 * 256;
+    return (size_t)e * 2 + (size_t)kHeadWGs * (kOut * 33 * sizeof(float) + sizeof(double)) +
+           (size_t)(kDwPartFloats + kDbPartFloats) * sizeof(float) + 64 * 256;
 }
 
 Acts carve_acts(void *base, long long n0) {
@@ -691,12 +909,15 @@ Acts carve_acts(void *base, long long n0) {
     a.g_cat = take(n0 * 224); a.g_a = take(n0 * 96); a.g_b = take(n0 * 96);
     a.g_e0b = take(n0 * 32); a.g_e1b = take(n1 * 64); a.g_e2b = take(n2 * 96);
     a.g_low = take(n0 * 128);
+    a.head_part = reinterpret_cast<float *>(take((long long)kHeadWGs * kOut * 33 * 2));
+    a.loss_part = reinterpret_cast<double *>(take((long long)kHeadWGs * 4));
+    a.dw_part = reinterpret_cast<float *>(take(kDwPartFloats * 2));
+    a.db_part = reinterpret_cast<float *>(take(kDbPartFloats * 2));
     return a;
 }
 
-// Workgroups of a convolution up to which the prefetching kernel (one
-// workgroup per CU) is used: the device's CUs.
-int conv_prefetch_limit(s3dg_cnn *c) {
+// The device's CU count (cached per thread and device).
+int conv_cus(s3dg_cnn *c) {
     static thread_local int cached_device = -1, cached_cus = 0;
     if (cached_device != c->ctx->device) {
         int cus = 0;
@@ -714,7 +935,8 @@ void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, con
     hipStream_t s = c->ctx->stream;
     const unsigned tiles = (unsigned)(((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile));
     const int r = relu ? 1 : 0;
-    const long long limit = conv_prefetch_limit(c);
+    // the prefetching kernel (one workgroup per CU) up to one workgroup per CU
+    const long long limit = conv_cus(c);
     if (Cout % 64 == 0) {
         dim3 grid(tiles, N * (Cout / 64));
         if ((long long)grid.x * grid.y <= limit)
@@ -730,18 +952,22 @@ void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, con
     }
 }
 
-void conv_dw(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *dpre, int Cout, float *dw, float *db) {
+// Weight and bias gradients of one layer (Cin, Cout multiples of 32): one
+// workgroup per (32 output, 32 input channels) block and slice of the pixel
+// tiles, kDwWGs workgroups in all (two per CU on a 256-CU device), each
+// slice's sums into the partial buffers, then added up per weight.
+void conv_dw(s3dg_cnn *c, const DwParts &parts, const ConvIn &I, int N, int H, int W, const bf16 *dpre, int Cout,
+             float *dw, float *db) {
+    hipStream_t s = c->ctx->stream;
     const int Cin = I.C1 + I.C2;
-    const long long ntiles = (long long)N * ((W + kTile - 1) / kTile) * ((H + kDwRows - 1) / kDwRows);
-    const int cot = (Cout % 64 == 0) ? 64 : 32;
-    const int blocks_xy = (Cin / 32) * ((Cout + cot - 1) / cot);
-    // about 1024 workgroups, each walking at least a few tiles
-    const long long slices = std::max<long long>(1, std::min<long long>(ntiles, 1024 / blocks_xy + 1));
-    dim3 grid((unsigned)blocks_xy, (unsigned)slices);
-    if (cot == 64)
-        hipLaunchKernelGGL(k_conv_dw<64>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw, db);
-    else
-        hipLaunchKernelGGL(k_conv_dw<32>, grid, dim3(256), 0, c->ctx->stream, I, N, H, W, dpre, Cout, dw, db);
+    const long long ntiles = (long long)N * ((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile);
+    const int blocks_xy = (Cout / 32) * (Cin / 32);
+    const int slices = (int)std::max<long long>(1, std::min<long long>(ntiles, kDwWGs / blocks_xy));
+    hipLaunchKernelGGL(k_conv_dw, dim3((unsigned)blocks_xy, (unsigned)slices), dim3(256), 0, s, I, N, H, W, dpre, Cout,
+                       1, parts.w, parts.b);
+    const SliceSum sw{parts.w, slices, 9 * Cout * Cin, dw}, sb{parts.b, slices, Cout, db};
+    const int blocks_w = (sw.n / 4 + 15) / 16, blocks_b = (sb.n / 4 + 15) / 16;
+    hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)(blocks_w + blocks_b)), dim3(256), 0, s, sw, sb, blocks_w);
 }
 
 ConvIn cin1(const bf16 *a, int C) { return ConvIn{a, nullptr, C, 0, 1}; }
@@ -787,10 +1013,10 @@ void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
 // producers of dpre apply the ReLU mask): accumulates dW and db, and (if dx)
 // writes the input gradient, masked by dx_mask when the input is the output
 // of a ReLU whose whole gradient this is.
-void layer_back(s3dg_cnn *c, int l, const ConvIn &I, int N, int H, int W, const bf16 *dpre, bf16 *dx,
-                const bf16 *dx_mask) {
+void layer_back(s3dg_cnn *c, const DwParts &P, int l, const ConvIn &I, int N, int H, int W, const bf16 *dpre,
+                bf16 *dx, const bf16 *dx_mask) {
     ConvLayer &L = c->layer[l];
-    conv_dw(c, I, N, H, W, dpre, L.cout, c->g + L.w_off, c->g + L.b_off);
+    conv_dw(c, P, I, N, H, W, dpre, L.cout, c->g + L.w_off, c->g + L.b_off);
     if (dx) conv(c, cin1(dpre, L.cout), N, H, W, L.wb, nullptr, L.cin, false, dx, dx_mask);
 }
 
@@ -798,7 +1024,7 @@ void layer_back(s3dg_cnn *c, int l, const ConvIn &I, int N, int H, int W, const 
 // by low (a ReLU output whose whole gradient it is).
 void split(s3dg_cnn *c, const bf16 *dcat, int N, int H, int W, int C1, int C2, const bf16 *low, bf16 *dlow,
            bf16 *dskip) {
-    const long long total = (long long)N * ((H + 1) / 2) * ((W + 1) / 2) * C1 + (long long)N * H * W * C2;
+    const long long total = (long long)N * ((H + 1) / 2) * ((W + 1) / 2) * (C1 / 8) + (long long)N * H * W * (C2 / 8);
     hipLaunchKernelGGL(k_split_grad, dim3(grid1(total)), dim3(256), 0, c->ctx->stream, dcat, N, H, W, C1, C2, low,
                        dlow, dskip);
 }
@@ -806,7 +1032,7 @@ void split(s3dg_cnn *c, const bf16 *dcat, int N, int H, int W, int C1, int C2, c
 // dX += upsample(dY) / 4, the last contribution to dX, then masked by the
 // ReLU output `mask`.
 void pool_back(s3dg_cnn *c, const bf16 *dy, int N, int H, int W, int C, const bf16 *mask, bf16 *dx) {
-    hipLaunchKernelGGL(k_pool2_back, dim3(grid1((long long)N * H * W * C)), dim3(256), 0, c->ctx->stream, dy, N, H, W,
+    hipLaunchKernelGGL(k_pool2_back, dim3(grid1((long long)N * H * W * (C / 8))), dim3(256), 0, c->ctx->stream, dy, N, H, W,
                        C, mask, dx);
 }
 
@@ -815,27 +1041,28 @@ void pool_back(s3dg_cnn *c, const bf16 *dy, int N, int H, int W, int C, const bf
 // skip buffers are zeroed before), then the pooling path's, which applies
 // their ReLU mask.
 void backward(s3dg_cnn *c, Acts &a, int N, int H, int W) {
+    const DwParts P{a.dw_part, a.db_part};
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-    layer_back(c, L_D0B, cin1(a.d0a, 32), N, H, W, a.g_a, a.g_b, a.d0a);
-    layer_back(c, L_D0A, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, a.g_b, a.g_cat, nullptr);
+    layer_back(c, P, L_D0B, cin1(a.d0a, 32), N, H, W, a.g_a, a.g_b, a.d0a);
+    layer_back(c, P, L_D0A, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, a.g_b, a.g_cat, nullptr);
     split(c, a.g_cat, N, H, W, 64, 32, a.d1b, a.g_low, a.g_e0b);
-    layer_back(c, L_D1B, cin1(a.d1a, 64), N, H1, W1, a.g_low, a.g_a, a.d1a);
-    layer_back(c, L_D1A, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, a.g_a, a.g_cat, nullptr);
+    layer_back(c, P, L_D1B, cin1(a.d1a, 64), N, H1, W1, a.g_low, a.g_a, a.d1a);
+    layer_back(c, P, L_D1A, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, a.g_a, a.g_cat, nullptr);
     split(c, a.g_cat, N, H1, W1, 96, 64, a.d2b, a.g_low, a.g_e1b);
-    layer_back(c, L_D2B, cin1(a.d2a, 96), N, H2, W2, a.g_low, a.g_a, a.d2a);
-    layer_back(c, L_D2A, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, a.g_a, a.g_cat, nullptr);
+    layer_back(c, P, L_D2B, cin1(a.d2a, 96), N, H2, W2, a.g_low, a.g_a, a.d2a);
+    layer_back(c, P, L_D2A, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, a.g_a, a.g_cat, nullptr);
     split(c, a.g_cat, N, H2, W2, 128, 96, a.b1, a.g_low, a.g_e2b);
-    layer_back(c, L_B1, cin1(a.b0, 128), N, H3, W3, a.g_low, a.g_a, a.b0);
-    layer_back(c, L_B0, cin1(a.p2, 96), N, H3, W3, a.g_a, a.g_b, nullptr);
+    layer_back(c, P, L_B1, cin1(a.b0, 128), N, H3, W3, a.g_low, a.g_a, a.b0);
+    layer_back(c, P, L_B0, cin1(a.p2, 96), N, H3, W3, a.g_a, a.g_b, nullptr);
     pool_back(c, a.g_b, N, H2, W2, 96, a.e2b, a.g_e2b);
-    layer_back(c, L_E2B, cin1(a.e2a, 96), N, H2, W2, a.g_e2b, a.g_a, a.e2a);
-    layer_back(c, L_E2A, cin1(a.p1, 64), N, H2, W2, a.g_a, a.g_b, nullptr);
+    layer_back(c, P, L_E2B, cin1(a.e2a, 96), N, H2, W2, a.g_e2b, a.g_a, a.e2a);
+    layer_back(c, P, L_E2A, cin1(a.p1, 64), N, H2, W2, a.g_a, a.g_b, nullptr);
     pool_back(c, a.g_b, N, H1, W1, 64, a.e1b, a.g_e1b);
-    layer_back(c, L_E1B, cin1(a.e1a, 64), N, H1, W1, a.g_e1b, a.g_a, a.e1a);
-    layer_back(c, L_E1A, cin1(a.p0, 32), N, H1, W1, a.g_a, a.g_b, nullptr);
+    layer_back(c, P, L_E1B, cin1(a.e1a, 64), N, H1, W1, a.g_e1b, a.g_a, a.e1a);
+    layer_back(c, P, L_E1A, cin1(a.p0, 32), N, H1, W1, a.g_a, a.g_b, nullptr);
     pool_back(c, a.g_b, N, H, W, 32, a.e0b, a.g_e0b);
-    layer_back(c, L_E0B, cin1(a.e0a, 32), N, H, W, a.g_e0b, a.g_a, a.e0a);
-    layer_back(c, L_E0A, cin1(a.x, kInPadded), N, H, W, a.g_a, nullptr, nullptr);
+    layer_back(c, P, L_E0B, cin1(a.e0a, 32), N, H, W, a.g_e0b, a.g_a, a.e0a);
+    layer_back(c, P, L_E0A, cin1(a.x, kInPadded), N, H, W, a.g_a, nullptr, nullptr);
 }
 
 int check_dims(s3dg_context *ctx, int n, int h, int w) {
@@ -864,9 +1091,11 @@ int loss_and_grad(s3dg_cnn *c, int n, int h, int w, const float *d_feat, const f
     S3DG_HIP(ctx, hipMemsetAsync(c->d_loss, 0, sizeof(double), s));
     forward(c, a, d_feat, n, h, w);
     const float inv = 1.0f / (float)(n0 * kOut);
-    hipLaunchKernelGGL(k_head, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 2048)), dim3(256), 0, s, a.d0b, n0,
-                       c->p + c->head_w, c->p + c->head_b, d_feat, d_target, nullptr, a.g_a, c->g + c->head_w,
-                       c->g + c->head_b, c->d_loss, inv);
+    const int head_wgs = (int)std::min<long long>((n0 + 31) / 32, kHeadWGs);
+    hipLaunchKernelGGL(k_head, dim3((unsigned)head_wgs), dim3(256), 0, s, a.d0b, n0, c->p + c->head_w,
+                       c->p + c->head_b, d_feat, d_target, nullptr, a.g_a, a.head_part, a.loss_part, inv);
+    hipLaunchKernelGGL(k_head_reduce, dim3(kOut * 33 + 1), dim3(256), 0, s, a.head_part, a.loss_part, head_wgs,
+                       c->g + c->head_w, c->g + c->head_b, c->d_loss);
     backward(c, a, n, h, w);
     S3DG_HIP(ctx, hipGetLastError());
     double l = 0.0;
