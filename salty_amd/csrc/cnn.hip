@@ -76,67 +76,76 @@ __device__ __forceinline__ float load1(const ConvIn &I, int n, int y, int x, int
 
 // ---------------------------------------------------------------------------
 // 3x3 convolution (stride 1, zero padding) as an implicit GEMM: M = pixels,
-// N = output channels, K = 9 taps x Cin, on v_mfma_f32_32x32x16_bf16.  A
-// workgroup (four waves) computes a 16 x 16 pixel tile for COT output
-// channels.  Per 32-channel slice of the input it stages the 18 x 18 halo
-// tile and the 9 x COT x 32 weights in LDS (rows padded from 64 to 80 bytes,
-// so that the 16-byte reads of eight consecutive lanes hit distinct banks),
-// then each wave runs 9 taps x 2 K steps over its 4 pixel rows (two 32-pixel
-// A tiles) and COT / 32 B tiles.  Used for the forward pass (weights
-// [tap][cout][cin]) and for the data gradient (flipped transposed weights,
-// dY as input).
+// N = output channels, K = 9 taps x Cin, on v_mfma_f32_16x16x32_bf16 with one
+// 16-pixel row of the tile as M and a whole 32-channel input slice as K.  A
+// workgroup (four waves) computes a ROWS x 16 pixel tile for COT output
+// channels (ROWS x COT = 1024); each wave owns 8 rows x 32 channels.  Per
+// input slice it stages the (ROWS + 2) x 18 halo and the 9 x COT x 32
+// weights in LDS (pixel and channel rows padded from 64 to 80 bytes, so that
+// the 16-byte reads of consecutive lanes hit distinct banks).  For each column
+// shift dx a wave reads the 3 x 2 weight operands once and, per half of its
+// rows, the 6 input rows it needs, and runs the 3 x 4 x 2 MFMAs they make
+// (each input row serves up to three taps): 18 LDS reads per 48 MFMAs.  The
+// grid is persistent and each step's global loads (one tile and 32-channel
+// slice) are in flight while the previous step computes.  Used for the forward pass
+// (weights [tap][cout][cin]) and for the data gradient (flipped transposed
+// weights, dY as input; with `mask`, the result passes the ReLU whose output
+// mask is).  The weights are the MFMA's first operand, so each lane ends with
+// 4 consecutive output channels of one pixel and stores them directly.
 // ---------------------------------------------------------------------------
-constexpr int kTile = 16;                 // output tile edge (pixels)
+constexpr int kTile = 16;                 // tile width (pixels)
 constexpr int kHalo = kTile + 2;
 constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 padding
 
-template <int COT, bool PF>
-__global__ __launch_bounds__(256, PF ? 1 : (COT == 32 ? 3 : 2)) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
-                                                     const bf16 *__restrict__ wt, const float *__restrict__ bias,
-                                                     int32_t Cout, int32_t relu, const bf16 *__restrict__ mask,
-                                                     bf16 *__restrict__ out) {
-    __shared__ __attribute__((aligned(16))) bf16 s_in[kHalo * kHalo * kRow];
-    __shared__ __attribute__((aligned(16))) bf16 s_w[9 * COT * kRow];
-    constexpr int NT = COT / 32;
+template <int COT>
+__global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
+                                                    const bf16 *__restrict__ wt, const float *__restrict__ bias,
+                                                    int32_t Cout, int32_t relu, const bf16 *__restrict__ mask,
+                                                    bf16 *__restrict__ out) {
+    constexpr int ROWS = 1024 / COT;                 // tile rows: 32 (COT 32) or 16 (COT 64)
+    constexpr int WR = ROWS / 8;                     // waves along the rows
+    constexpr int kInElems = (ROWS + 2) * kHalo * kRow, kWElems = 9 * COT * kRow;
+    __shared__ __attribute__((aligned(16))) bf16 smem[kInElems + kWElems];
+    bf16 *const s_in = smem;
+    bf16 *const s_w = smem + kInElems;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tiles_x = (W + kTile - 1) / kTile;
-    const int x0 = (blockIdx.x % tiles_x) * kTile, y0 = (blockIdx.x / tiles_x) * kTile;
+    const int tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + ROWS - 1) / ROWS;
     const int co_blocks = (Cout + COT - 1) / COT;
-    const int n = blockIdx.y / co_blocks;
-    const int co0 = (blockIdx.y % co_blocks) * COT;
+    const int items = N * tiles_x * tiles_y * co_blocks;   // (tile, output-channel block) pairs
     const int Cin = I.C1 + I.C2;
-    // KS accumulators per block: with one B tile (COT = 32) the two K halves
-    // of a slice go to separate accumulators, so that four independent MFMA
-    // chains keep the matrix pipeline busy; they are summed in the epilogue
-    constexpr int KS = NT == 1 ? 2 : 1;
-    f32x16 acc[2][NT][KS];
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int t = 0; t < NT; ++t)
-#pragma unroll
-            for (int k = 0; k < KS; ++k)
-#pragma unroll
-                for (int i = 0; i < 16; ++i) acc[m][t][k][i] = 0.0f;
+    const int rw0 = (wave % WR) * 8;                 // this wave's first tile row
+    const int cw0 = (wave / WR) * 32;                // ... and first channel of the block
+    const int l16 = lane & 15, kq = 8 * (lane >> 4);
     bf16x8 zero;
 #pragma unroll
     for (int j = 0; j < 8; ++j) zero[j] = f2b(0.0f);
-    const int koff = 8 * (lane >> 5);
-    const int p = lane & 31;
-    // staging: every load of a slice is issued before the first LDS store, so
-    // that the slice costs one memory round trip
-    constexpr int kHalfCh = kHalo * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
+    // work item -> image, tile origin, first output channel (output-channel
+    // blocks innermost, so that consecutive items share the input tile)
+    struct Item {
+        int n, y0, x0, co0;
+    };
+    const auto item_of = [&](int it) {
+        Item r;
+        r.co0 = (it % co_blocks) * COT;
+        const int t = it / co_blocks;
+        r.x0 = (t % tiles_x) * kTile;
+        r.y0 = ((t / tiles_x) % tiles_y) * ROWS;
+        r.n = t / (tiles_x * tiles_y);
+        return r;
+    };
+    // staging: all global loads of a step (one item, one 32-channel slice)
+    // are in flight while the previous step computes
+    constexpr int kHalfCh = (ROWS + 2) * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
     constexpr int kWCh = 9 * COT * 4, kWPer = (kWCh + 255) / 256;
     bf16x8 hv[kHalfPer], wv[kWPer];
-    // global loads of the slice at channel c0 into registers
-    const auto issue = [&](int c0) {
+    const auto issue = [&](const Item &it, int c0) {
 #pragma unroll
         for (int k = 0; k < kHalfPer; ++k) {
             const int e = threadIdx.x + 256 * k;
             if (e < kHalfCh) {
                 const int pix = e >> 2, q = e & 3;
-                const int y = y0 + pix / kHalo - 1, x = x0 + pix % kHalo - 1;
-                hv[k] = load8(I, n, y, x, H, W, c0 + q * 8);
+                const int y = it.y0 + pix / kHalo - 1, x = it.x0 + pix % kHalo - 1;
+                hv[k] = load8(I, it.n, y, x, H, W, c0 + q * 8);
             }
         }
 #pragma unroll
@@ -144,97 +153,109 @@ __global__ __launch_bounds__(256, PF ? 1 : (COT == 32 ? 3 : 2)) void k_conv3x3(C
             const int e = threadIdx.x + 256 * k;
             if (e < kWCh) {
                 const int row = e >> 2, q = e & 3;
-                const int tap = row / COT, co = co0 + row % COT;
+                const int tap = row / COT, co = it.co0 + row % COT;
                 wv[k] = zero;
                 if (co < Cout) wv[k] = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + co) * Cin + c0 + q * 8);
             }
         }
     };
-    // With PF (grids of at most one workgroup per CU, where latency rather
-    // than occupancy decides) the next slice's loads are in flight while the
-    // current slice's MFMAs run; the registers this takes are free there.
-    if (PF) issue(0);
-    for (int c0 = 0; c0 < Cin; c0 += 32) {
-        if (!PF) issue(c0);
+    if ((int)blockIdx.x < items) issue(item_of(blockIdx.x), 0);
+    for (int itn = blockIdx.x; itn < items; itn += gridDim.x) {
+        const Item it = item_of(itn);
+        f32x4 acc[8][2];
 #pragma unroll
-        for (int k = 0; k < kHalfPer; ++k) {
-            const int e = threadIdx.x + 256 * k;
-            if (e < kHalfCh) *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = hv[k];
-        }
+        for (int i = 0; i < 8; ++i)
 #pragma unroll
-        for (int k = 0; k < kWPer; ++k) {
-            const int e = threadIdx.x + 256 * k;
-            if (e < kWCh) *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = wv[k];
-        }
-        if (PF && c0 + 32 < Cin) {
-            issue(c0 + 32);
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[i][t][e] = 0.0f;
+        for (int c0 = 0; c0 < Cin; c0 += 32) {
+#pragma unroll
+            for (int k = 0; k < kHalfPer; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                if (e < kHalfCh) *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = hv[k];
+            }
+#pragma unroll
+            for (int k = 0; k < kWPer; ++k) {
+                const int e = threadIdx.x + 256 * k;
+                if (e < kWCh) *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = wv[k];
+            }
+            // the next step: the next slice of this item, or slice 0 of the
+            // next item
+            {
+                const bool more = c0 + 32 < Cin;
+                const int nitem = more ? itn : itn + (int)gridDim.x;
+                if (nitem < items) issue(item_of(nitem), more ? c0 + 32 : 0);
+            }
             __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the taps
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll 1
-        for (int tap = 0; tap < 9; ++tap) {
-            const int dy = tap / 3, dx = tap % 3;
+            for (int dx = 0; dx < 3; ++dx) {
+                // operands: halo rows rw0 .. rw0+9 shifted by dx (A: lane =
+                // pixel column, 8 channels), weights of taps (dy, dx) (B: lane
+                // = output channel, 8 channels)
+                bf16x8 b[3][2];
 #pragma unroll
-            for (int kk = 0; kk < 32; kk += 16) {
-                bf16x8 a[2], b[NT];
+                for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-                for (int m = 0; m < 2; ++m) {
-                    const int r = wave * 4 + m * 2 + (p >> 4);
-                    a[m] = *reinterpret_cast<const bf16x8 *>(&s_in[((r + dy) * kHalo + (p & 15) + dx) * kRow + kk + koff]);
+                    for (int t = 0; t < 2; ++t)
+                        b[dy][t] = *reinterpret_cast<const bf16x8 *>(
+                            &s_w[((dy * 3 + dx) * COT + cw0 + 16 * t + l16) * kRow + kq]);
+                // two halves of 4 output rows (6 input rows each), so that
+                // consecutive MFMAs go to 8 different accumulators
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    bf16x8 a[6];
+#pragma unroll
+                    for (int i = 0; i < 6; ++i)
+                        a[i] = *reinterpret_cast<const bf16x8 *>(
+                            &s_in[((rw0 + 4 * h + i) * kHalo + l16 + dx) * kRow + kq]);
+#pragma unroll
+                    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                        for (int i = 0; i < 4; ++i)
+#pragma unroll
+                            for (int t = 0; t < 2; ++t)
+                                acc[4 * h + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[dy][t], a[i + dy],
+                                                                                           acc[4 * h + i][t], 0, 0, 0);
                 }
+            }
+            __syncthreads();
+        }
+        // Epilogue straight from the accumulators: with the weights as the
+        // MFMA's first operand the result is D[channel][pixel] and lane l holds
+        // channels 4*(l/16) .. +3 of pixel column l % 16, 8 contiguous bytes
+        // of the NHWC output; bias and ReLU applied, or the mask.
 #pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    b[t] = *reinterpret_cast<const bf16x8 *>(&s_w[(tap * COT + 32 * t + p) * kRow + kk + koff]);
+        for (int t = 0; t < 2; ++t) {
+            const int co = it.co0 + cw0 + 16 * t + 4 * (lane >> 4);
+            const int x = it.x0 + l16;
+            float bv[4];
 #pragma unroll
-                for (int m = 0; m < 2; ++m)
+            for (int e = 0; e < 4; ++e) bv[e] = (bias && co + e < Cout) ? bias[co + e] : 0.0f;
+            if (x >= W || co >= Cout) continue;
 #pragma unroll
-                    for (int t = 0; t < NT; ++t)
-                        acc[m][t][kk / 16 % KS] =
-                            __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[m], b[t], acc[m][t][kk / 16 % KS], 0, 0, 0);
+            for (int i = 0; i < 8; ++i) {
+                const int y = it.y0 + rw0 + i;
+                if (y >= H) break;
+                const size_t off = (((size_t)it.n * H + y) * W + x) * Cout + co;
+                bf16x4 v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float f = acc[i][t][e] + bv[e];
+                    if (relu) f = fmaxf(f, 0.0f);
+                    v[e] = f2b(f);
+                }
+                if (mask) {
+                    // gradient through the ReLU whose output is `mask` (same shape)
+                    const bf16x4 m = *reinterpret_cast<const bf16x4 *>(mask + off);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (!(b2f(m[e]) > 0.0f)) v[e] = f2b(0.0f);
+                }
+                *reinterpret_cast<bf16x4 *>(out + off) = v;
             }
         }
-        __syncthreads();
-    }
-    // Epilogue through LDS: bias and ReLU applied, the tile is transposed into
-    // [pixel][channel] rows in the weight buffer (free after the last slice),
-    // then written out 8 channels (16 bytes) per store.  D layout of a block:
-    // row (pixel of the 32-pixel tile) = 8*(i/4) + 4*(lane/32) + i%4, column
-    // (channel) = lane % 32.
-    constexpr int kOutRow = COT + 8;
-    static_assert(kTile * kTile * kOutRow <= 9 * COT * kRow, "output tile fits the weight buffer");
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = co0 + 32 * t + p;
-        const float bv = (bias && co < Cout) ? bias[co] : 0.0f;
-#pragma unroll
-        for (int m = 0; m < 2; ++m)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int row = 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-                const int pix = (wave * 4 + m * 2 + (row >> 4)) * kTile + (row & 15);
-                float v = acc[m][t][0][i];
-                if (KS == 2) v += acc[m][t][KS - 1][i];
-                v += bv;
-                if (relu) v = fmaxf(v, 0.0f);
-                s_w[pix * kOutRow + 32 * t + p] = f2b(v);
-            }
-    }
-    __syncthreads();
-    constexpr int kChunks = kTile * kTile * (COT / 8);
-    for (int e = threadIdx.x; e < kChunks; e += 256) {
-        const int pix = e / (COT / 8), q = e % (COT / 8);
-        const int y = y0 + pix / kTile, x = x0 + pix % kTile, co = co0 + 8 * q;
-        if (y >= H || x >= W || co >= Cout) continue;
-        const size_t off = (((size_t)n * H + y) * W + x) * Cout + co;
-        bf16x8 v = *reinterpret_cast<const bf16x8 *>(&s_w[pix * kOutRow + 8 * q]);
-        if (mask) {
-            // gradient through the ReLU whose output is `mask` (same shape)
-            const bf16x8 m = *reinterpret_cast<const bf16x8 *>(mask + off);
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (!(b2f(m[j]) > 0.0f)) v[j] = f2b(0.0f);
-        }
-        *reinterpret_cast<bf16x8 *>(out + off) = v;
     }
 }
 
@@ -929,26 +950,23 @@ int conv_cus(s3dg_cnn *c) {
 }
 
 // mask (optional): out is the gradient of a ReLU's output, passed on only
-// where that output (mask, same shape as out) is positive.
+// where that output (mask, same shape as out) is positive.  Persistent grid:
+// two workgroups per CU at most, each walking its share of the (tile,
+// channel block) items.
 void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
           bf16 *out, const bf16 *mask = nullptr) {
     hipStream_t s = c->ctx->stream;
-    const unsigned tiles = (unsigned)(((W + kTile - 1) / kTile) * ((H + kTile - 1) / kTile));
     const int r = relu ? 1 : 0;
-    // the prefetching kernel (one workgroup per CU) up to one workgroup per CU
-    const long long limit = conv_cus(c);
+    const long long wgs = 2LL * conv_cus(c);
+    const long long tx = (W + kTile - 1) / kTile;
     if (Cout % 64 == 0) {
-        dim3 grid(tiles, N * (Cout / 64));
-        if ((long long)grid.x * grid.y <= limit)
-            hipLaunchKernelGGL((k_conv3x3<64, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
-        else
-            hipLaunchKernelGGL((k_conv3x3<64, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
+        const long long items = (long long)N * tx * ((H + 15) / 16) * (Cout / 64);
+        hipLaunchKernelGGL((k_conv3x3<64>), dim3((unsigned)std::min(items, wgs)), dim3(256), 0, s, I, N, H, W, wt, bias,
+                           Cout, r, mask, out);
     } else {
-        dim3 grid(tiles, N * ((Cout + 31) / 32));
-        if ((long long)grid.x * grid.y <= limit)
-            hipLaunchKernelGGL((k_conv3x3<32, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
-        else
-            hipLaunchKernelGGL((k_conv3x3<32, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask, out);
+        const long long items = (long long)N * tx * ((H + 31) / 32) * ((Cout + 31) / 32);
+        hipLaunchKernelGGL((k_conv3x3<32>), dim3((unsigned)std::min(items, wgs)), dim3(256), 0, s, I, N, H, W, wt, bias,
+                           Cout, r, mask, out);
     }
 }
 
