@@ -103,7 +103,14 @@ def up_bwd(dy):
 REL_EPS = 1e-2
 
 
-def loss_and_grad(params, features, target, den=None):
+def bf16_round(x):
+    """Round-to-nearest-even to bfloat16 (through float32), returned as float64."""
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def loss_and_grad(params, features, target, den=None, bf16=False):
     """Returns (mean loss, gradient vector, head output in log space).
 
     The prediction is expm1 of the head output.  The loss per value is the
@@ -112,23 +119,34 @@ def loss_and_grad(params, features, target, den=None):
     (fixing it makes the loss a smooth function of the parameters, which is
     how the gradient is checked by finite differences).  Pixels whose depth
     guide is 0 (camera ray missed) are excluded from the loss: the network's
-    output is replaced there by the input radiance."""
+    output is replaced there by the input radiance.
+
+    With bf16=True every tensor the GPU kernels store in bfloat16 is rounded
+    to it at the same point: the packed input, each layer's and pooling's
+    output, the head's input gradient, each data-gradient convolution's
+    output, the 2x2 sums of the upsampled gradients and the skip gradients
+    after the pooling path's contribution is added (sums in between stay
+    float64)."""
     P = unpack(np.asarray(params, np.float64))
     lay, nparam = param_layout()
     cache = {}
+    q = bf16_round if bf16 else (lambda v: v)
 
     def layer(name, x):
         W, b = P[name]
         z, cols = conv_fwd(x, W, b)
-        y = np.maximum(z, 0)
+        y = q(np.maximum(z, 0))
         cache[name] = (x.shape, cols, y)
         return y
 
-    x = pack_features(np.asarray(features, np.float64))
+    def qpool(x):
+        return q(pool(x))
+
+    x = q(pack_features(np.asarray(features, np.float64)))
     e0 = layer("e0b", layer("e0a", x))
-    e1 = layer("e1b", layer("e1a", pool(e0)))
-    e2 = layer("e2b", layer("e2a", pool(e1)))
-    bt = layer("b1", layer("b0", pool(e2)))
+    e1 = layer("e1b", layer("e1a", qpool(e0)))
+    e2 = layer("e2b", layer("e2a", qpool(e1)))
+    bt = layer("b1", layer("b0", qpool(e2)))
     d2 = layer("d2b", layer("d2a", np.concatenate([up(bt), e2], -1)))
     d1 = layer("d1b", layer("d1a", np.concatenate([up(d2), e1], -1)))
     d0 = layer("d0b", layer("d0a", np.concatenate([up(d1), e0], -1)))
@@ -146,7 +164,8 @@ def loss_and_grad(params, features, target, den=None):
     dout = 2.0 * diff / (d + REL_EPS) * np.exp(out) / count
     g[lay["head_w"]:lay["head_w"] + OUT * 32] = np.einsum("nhwk,nhwc->kc", dout, d0).reshape(-1)
     g[lay["head_b"]:lay["head_b"] + OUT] = dout.sum(axis=(0, 1, 2))
-    dd0 = dout @ Wh
+    # the head's input gradient is stored masked by the ReLU (d0 > 0)
+    dd0 = q((dout @ Wh) * (d0 > 0)) if bf16 else dout @ Wh
 
     def back(name, dy):
         xshape, cols, y = cache[name]
@@ -156,20 +175,20 @@ def loss_and_grad(params, features, target, den=None):
         wo, bo, cin, cout = lay[name]
         g[wo:wo + dW.size] = dW.reshape(-1)
         g[bo:bo + cout] = db
-        return dx
+        return q(dx)
 
     dcat = back("d0a", back("d0b", dd0))
-    dd1, de0 = up_bwd(dcat[..., :64]), dcat[..., 64:]
+    dd1, de0 = q(up_bwd(dcat[..., :64])), dcat[..., 64:]
     dcat = back("d1a", back("d1b", dd1))
-    dd2, de1 = up_bwd(dcat[..., :96]), dcat[..., 96:]
+    dd2, de1 = q(up_bwd(dcat[..., :96])), dcat[..., 96:]
     dcat = back("d2a", back("d2b", dd2))
-    dbt, de2 = up_bwd(dcat[..., :128]), dcat[..., 128:]
+    dbt, de2 = q(up_bwd(dcat[..., :128])), dcat[..., 128:]
     dp2 = back("b0", back("b1", dbt))
-    de2 = de2 + pool_bwd(dp2)
+    de2 = q(de2 + pool_bwd(dp2))
     dp1 = back("e2a", back("e2b", de2))
-    de1 = de1 + pool_bwd(dp1)
+    de1 = q(de1 + pool_bwd(dp1))
     dp0 = back("e1a", back("e1b", de1))
-    de0 = de0 + pool_bwd(dp0)
+    de0 = q(de0 + pool_bwd(dp0))
     back("e0a", back("e0b", de0))
     return loss, g, out
 

@@ -1,18 +1,12 @@
 """Denoiser kernels (BF16 MFMA forward / backward-data / backward-weight, FP32
-accumulation) against the float64 NumPy restatement in oracle/unet.py."""
+accumulation) against the float64 NumPy restatement in oracle/unet.py, rounded
+to bfloat16 where the kernels store bfloat16."""
 import numpy as np
 import pytest
 
 from oracle import unet
 
 pytestmark = pytest.mark.gpu
-
-
-def bf16_round(x):
-    """Round-to-nearest-even to bfloat16, returned as float64."""
-    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
-    return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
 def oracle_params(p):
@@ -22,7 +16,7 @@ def oracle_params(p):
     lay, _ = unet.param_layout()
     for name, cin, cout in unet.LAYERS:
         wo, _, _, _ = lay[name]
-        q[wo:wo + 9 * cin * cout] = bf16_round(p[wo:wo + 9 * cin * cout])
+        q[wo:wo + 9 * cin * cout] = unet.bf16_round(p[wo:wo + 9 * cin * cout])
     return q
 
 
@@ -35,36 +29,47 @@ def case(seed, n=1, h=32, w=32):
 
 
 def test_gradients_match_float64_restatement(ctx):
+    """Every layer's weight and bias gradient against the float64 restatement
+    rounded to bfloat16 where the kernels store bfloat16 (activations, data
+    gradients).  Against the unrounded restatement the gradients differ by up
+    to 8% (the bf16 storage itself), which would hide a dropped tap or a
+    shifted halo row; against the rounded one what is left is FP32
+    accumulation order: 0.12-0.3% per layer on three cases (bound 1%).  On the
+    fourth the loss, a forward-pass quantity, already differs by 9e-4: an
+    accumulation-order difference flipped some activation roundings and
+    their effect propagates (4.2% per layer measured, bound 5%)."""
     import salty_amd
-    net = salty_amd.Denoiser(ctx, seed=3)
-    feat, target = case(0)
-    p = net.get_params()
-    loss, g = net.gradient(feat, target)
-    ref_loss, ref_g, _ = unet.loss_and_grad(oracle_params(p), feat.astype(np.float64), target.astype(np.float64))
-    assert abs(loss - ref_loss) <= 0.02 * ref_loss, (loss, ref_loss)
-    lay, _ = unet.param_layout()
-    for name, cin, cout in unet.LAYERS:
-        wo, bo, _, _ = lay[name]
-        for lo, hi, what in ((wo, wo + 9 * cin * cout, "W"), (bo, bo + cout, "b")):
-            a, b = g[lo:hi].astype(np.float64), ref_g[lo:hi]
-            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-30)
-            rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
-            assert cos > 0.98 and rel < 0.2, (name, what, cos, rel)
-    a = g[lay["head_w"]:].astype(np.float64)
-    b = ref_g[lay["head_w"]:]
-    assert np.linalg.norm(a - b) <= 0.05 * np.linalg.norm(b)
+    for seed, cs, tol_loss, tol in ((11, 1, 1e-4, 0.01), (5, 2, 1e-4, 0.01), (7, 3, 1e-4, 0.01), (3, 0, 2e-3, 0.05)):
+        net = salty_amd.Denoiser(ctx, seed=seed)
+        feat, target = case(cs)
+        p = net.get_params()
+        loss, g = net.gradient(feat, target)
+        ref_loss, ref_g, _ = unet.loss_and_grad(oracle_params(p), feat.astype(np.float64), target.astype(np.float64),
+                                                bf16=True)
+        assert abs(loss - ref_loss) <= tol_loss * ref_loss, (seed, loss, ref_loss)
+        lay, _ = unet.param_layout()
+        for name, cin, cout in unet.LAYERS:
+            wo, bo, _, _ = lay[name]
+            for lo, hi, what in ((wo, wo + 9 * cin * cout, "W"), (bo, bo + cout, "b")):
+                a, b = g[lo:hi].astype(np.float64), ref_g[lo:hi]
+                rel = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+                assert rel < tol, (seed, name, what, rel)
+        a = g[lay["head_w"]:].astype(np.float64)
+        b = ref_g[lay["head_w"]:]
+        assert np.linalg.norm(a - b) <= tol / 5 * np.linalg.norm(b), seed
 
 
 def test_inference_matches_float64_forward(ctx):
+    """Inference against the bf16-rounded float64 forward pass."""
     import salty_amd
     net = salty_amd.Denoiser(ctx, seed=4)
     feat, target = case(1, n=2, h=16, w=24)
     out = net.infer(feat)
     _, _, pred_log = unet.loss_and_grad(oracle_params(net.get_params()), feat.astype(np.float64),
-                                        target.astype(np.float64))
+                                        target.astype(np.float64), bf16=True)
     ref = np.maximum(np.expm1(pred_log), 0)
     err = np.abs(out - ref) / (np.abs(ref) + 0.05)
-    assert np.median(err) < 0.02 and np.percentile(err, 99) < 0.15
+    assert np.mean(err) < 0.01 and np.percentile(err, 99) < 0.05 and err.max() < 0.2
 
 
 def test_training_reduces_the_loss(ctx):

@@ -93,3 +93,19 @@ def test_miss_pixels_do_not_enter_the_loss():
     t3[0, 0, 0] += 5.0
     l2, _, _ = unet.loss_and_grad(p, feat, t3)
     assert l2 != l0
+
+
+def test_bf16_rounding_points():
+    """bf16_round is round-to-nearest-even on the top 16 bits of float32
+    (known answers, ties both ways), and the rounded restatement stays close
+    to the exact one (its gradient differs by the bf16 storage error only)."""
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -8 + 2 ** -20, -2.5, 0.0, 2.0 - 2 ** -12,
+                  -(1.0 + 2 ** -7 + 2 ** -9)])
+    # halfway cases go to the even mantissa; 2 - 2^-12 carries into the exponent
+    want = [1.0, 1.0, 1.0 + 2 ** -6, 1.0 + 2 ** -7, -2.5, 0.0, 2.0, -(1.0 + 2 ** -7)]
+    assert unet.bf16_round(x).tolist() == want
+    p, feat, target = make_case(2)
+    l0, g0, _ = unet.loss_and_grad(p, feat, target)
+    l1, g1, _ = unet.loss_and_grad(p, feat, target, bf16=True)
+    assert l1 != l0 and abs(l1 - l0) < 0.02 * l0
+    assert 0 < np.linalg.norm(g1 - g0) < 0.1 * np.linalg.norm(g0)
