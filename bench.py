@@ -65,6 +65,9 @@ def parse():
     p.add_argument("--passes-in-flight", type=int, default=0,
                    help="passes traced together per wavefront (0: the library's automatic choice)")
     p.add_argument("--no-autotune", action="store_true", help="keep the library's default kernel settings")
+    p.add_argument("--tuning", default=None,
+                   help="mesh workload: kernel settings 'variant,refill,tri_batch,fetch_chunk,tail_paths' (as a "
+                        "default run's config.tuning reports them) instead of autotuning, e.g. for profiling")
     p.add_argument("--materials", choices=("c2", "all"), default="c2",
                    help="mesh workload materials: Lambert/mirror/glass, or all five (Phong and Plastic too)")
     p.add_argument("--lens-radius", type=float, default=0.0, help="mesh workload: thin-lens camera when > 0")
@@ -606,7 +609,13 @@ def main():
     ext_spills = cst.extend_stack_spills / max(cst.extension_rays, 1)
 
     tuning = None
-    if not args.no_autotune:
+    if args.tuning:
+        v, r, tb, fc, tail = (int(x) for x in args.tuning.split(","))
+        base = ctx.get_tuning()
+        ctx.set_tuning(abi.Tuning(v, r, tb, base.passes_in_flight, fc, base.leaf_size, tail))
+        tuning = {"traversal_variant": v, "refill": r, "tri_batch": tb, "fetch_chunk": fc, "tail_paths": tail,
+                  "source": "--tuning"}
+    elif not args.no_autotune:
         t, _ = scene.autotune(cfg(args.spp))
         tuning = {"traversal_variant": t.traversal_variant, "refill": t.refill, "tri_batch": t.tri_batch,
                   "fetch_chunk": t.fetch_chunk, "tail_paths": t.tail_paths}
