@@ -41,14 +41,15 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_extend_mesh_scene_hbm_traffi
 def extend_traffic_bytes():
     """HBM bytes per k_extend launch of the default workload, from the committed
     counter profile (FETCH_SIZE and WRITE_SIZE scaled by a plain copy), or None."""
+    rows = []
     try:
         with open(TRAFFIC_PROFILE) as f:
-            for line in f:
-                if line.startswith("| `k_extend<false"):
-                    return float(line.strip().strip("|").split("|")[-1]) * 1e6
+            rows = [line for line in f if line.startswith("| `k_extend<false")]
     except OSError:
         pass
-    return None
+    # the instrumented frame's row (the frame the kernel split is timed on)
+    rows.sort(key=lambda line: "instrumented frame" not in line)
+    return float(rows[0].strip().strip("|").split("|")[-1]) * 1e6 if rows else None
 
 
 def parse():
