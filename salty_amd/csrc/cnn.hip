@@ -38,7 +38,8 @@ __device__ __forceinline__ bf16 f2b(float x) { return (bf16)x; }
 
 // ---------------------------------------------------------------------------
 // Virtual convolution input: concat(upsample_s1(in1), in2) along channels.
-// in1: [N, H/s1, W/s1, C1], in2: [N, H, W, C2] (may be null with C2 = 0).
+// in1: [N, H/s1, W/s1, C1], in2: [N, H, W, C2] (may be null with C2 = 0);
+// C1 and C2 are multiples of 32, so that a 32-channel slice lies in one source.
 // ---------------------------------------------------------------------------
 struct ConvIn {
     const bf16 *in1;
@@ -46,23 +47,61 @@ struct ConvIn {
     int32_t C1, C2, s1;   // s1 is 1 or 2
 };
 
-// 8 consecutive channels c..c+7 (all in the same source) at pixel (n, y, x);
-// zero outside the image.
+// Workgroup barrier for LDS only: this wave's LDS operations complete
+// (lgkmcnt(0); gfx9 s_waitcnt fields: vmcnt 63, expcnt 7, lgkmcnt 0), then
+// s_barrier.  __syncthreads() also fences global memory, i.e. waits for every
+// outstanding global load and store of the wave (vmcnt(0)), which would make
+// each barrier wait for the next step's prefetched loads and the previous
+// step's output stores.  No wave here reads global memory another wave of the
+// workgroup writes, so the LDS ordering is all a barrier needs.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+}
+
+// A wave-uniform pointer as a value held in scalar registers.
+__device__ __forceinline__ const bf16 *uniform_ptr(const bf16 *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<const bf16 *>(((uint64_t)hi << 32) | lo);
+}
+
+// 8 consecutive channels c..c+7 (all in the same source) at pixel (n, y, x),
+// without the zeroing outside the image: `inside` says whether the value is
+// the pixel's (outside, an in-image address is read).  One load from whichever
+// source holds the channels: with a load on each side of a branch the
+// compiler has to wait for the first before the second may write the same
+// registers, which serialised the staging loads.
+__device__ __forceinline__ bf16x8 load8_raw(const ConvIn &I, int n, int y, int x, int H, int W, int c,
+                                            bool &inside) {
+    inside = y >= 0 && y < H && x >= 0 && x < W;
+    const int yy = inside ? y : 0, xx = inside ? x : 0;
+    const bool up = c < I.C1;
+    // upsampled input: coordinates halved when s1 = 2
+    const int sh = I.s1 >> 1;
+    const int h1 = (H + sh) >> sh, w1 = (W + sh) >> sh;
+    const size_t off = up ? (((size_t)n * h1 + (yy >> sh)) * w1 + (xx >> sh)) * I.C1 + c
+                          : (((size_t)n * H + yy) * W + xx) * I.C2 + (c - I.C1);
+    // the two source pointers as values in scalar registers: selecting
+    // between the struct's fields became a per-lane load of the pointer,
+    // with a wait for every load before it
+    const bf16 *base = up ? uniform_ptr(I.in1) : uniform_ptr(I.in2);
+    // a global-memory load (through a generic pointer it became a flat load,
+    // whose wait also waits for LDS and scalar traffic)
+    typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
+    return *(gbf16x8 *)(base + off);
+}
+
+// The same, zero outside the image.
 __device__ __forceinline__ bf16x8 load8(const ConvIn &I, int n, int y, int x, int H, int W, int c) {
-    bf16x8 v;
-    if (y < 0 || y >= H || x < 0 || x >= W) {
+    bool inside;
+    bf16x8 v = load8_raw(I, n, y, x, H, W, c, inside);
+    if (!inside) {
+#pragma unroll
         for (int j = 0; j < 8; ++j) v[j] = f2b(0.0f);
-        return v;
     }
-    if (c < I.C1) {
-        // upsampled input: coordinates halved when s1 = 2 (y, x >= 0 here)
-        const int sh = I.s1 >> 1;
-        const int h1 = (H + sh) >> sh, w1 = (W + sh) >> sh;
-        const size_t off = (((size_t)n * h1 + (y >> sh)) * w1 + (x >> sh)) * I.C1 + c;
-        return *reinterpret_cast<const bf16x8 *>(I.in1 + off);
-    }
-    const size_t off = (((size_t)n * H + y) * W + x) * I.C2 + (c - I.C1);
-    return *reinterpret_cast<const bf16x8 *>(I.in2 + off);
+    return v;
 }
 
 __device__ __forceinline__ float load1(const ConvIn &I, int n, int y, int x, int H, int W, int c) {
@@ -91,11 +130,17 @@ __device__ __forceinline__ float load1(const ConvIn &I, int n, int y, int x, int
 // (weights [tap][cout][cin]) and for the data gradient (flipped transposed
 // weights, dY as input; with `mask`, the result passes the ReLU whose output
 // mask is).  The weights are the MFMA's first operand, so each lane ends with
-// 4 consecutive output channels of one pixel and stores them directly.
+// 8 consecutive output channels of one pixel (with the rows of its two
+// channel tiles mapped by chan_of) and stores them directly, 16 bytes.
 // ---------------------------------------------------------------------------
 constexpr int kTile = 16;                 // tile width (pixels)
 constexpr int kHalo = kTile + 2;
 constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 padding
+
+// Output channel (within a wave's 32) of row m of channel tile t: rows
+// 4g .. 4g+3 of tile 0 and of tile 1 are channels 8g .. 8g+7, so that the lane
+// holding rows 4g .. 4g+3 of both tiles holds 8 consecutive channels.
+__device__ __forceinline__ int chan_of(int t, int m) { return 8 * (m >> 2) + 4 * t + (m & 3); }
 
 template <int COT>
 __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
@@ -138,25 +183,40 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
     constexpr int kHalfCh = (ROWS + 2) * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
     constexpr int kWCh = 9 * COT * 4, kWPer = (kWCh + 255) / 256;
     bf16x8 hv[kHalfPer], wv[kWPer];
+    uint32_t hin = 0, win = 0;   // chunks whose value is real (inside the image / an existing channel)
     const auto issue = [&](const Item &it, int c0) {
+        // every chunk load is issued unconditionally and raw (the last
+        // round's spare threads load the last chunk again and do not store
+        // it; out-of-range chunks are zeroed when stored), so that nothing
+        // between the loads and the next step's LDS stores waits for them
+        hin = win = 0;
+        // the slice lies in one source (C1 is a multiple of 32), chosen once
+        // per step; pixel indices in 32 bits (N*H*W <= 2^27)
+        const bool up = c0 < I.C1;
+        const bf16 *src = up ? uniform_ptr(I.in1) : uniform_ptr(I.in2);
+        const uint32_t C = (uint32_t)(up ? I.C1 : I.C2);
+        const int cs = up ? c0 : c0 - I.C1;
+        const int sh = up ? (I.s1 >> 1) : 0;   // upsampled input: coordinates halved
+        const uint32_t hs = (uint32_t)(H + sh) >> sh, ws = (uint32_t)(W + sh) >> sh;
+        typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
 #pragma unroll
         for (int k = 0; k < kHalfPer; ++k) {
-            const int e = threadIdx.x + 256 * k;
-            if (e < kHalfCh) {
-                const int pix = e >> 2, q = e & 3;
-                const int y = it.y0 + pix / kHalo - 1, x = it.x0 + pix % kHalo - 1;
-                hv[k] = load8(I, it.n, y, x, H, W, c0 + q * 8);
-            }
+            const int e = min((int)threadIdx.x + 256 * k, kHalfCh - 1);
+            const int pix = e >> 2, q = e & 3;
+            const int y = it.y0 + pix / kHalo - 1, x = it.x0 + pix % kHalo - 1;
+            const bool inside = y >= 0 && y < H && x >= 0 && x < W;
+            const uint32_t sp = ((uint32_t)it.n * hs + ((uint32_t)(inside ? y : 0) >> sh)) * ws +
+                                ((uint32_t)(inside ? x : 0) >> sh);
+            hv[k] = *(gbf16x8 *)(src + (size_t)sp * C + cs + q * 8);
+            hin |= (uint32_t)inside << k;
         }
 #pragma unroll
         for (int k = 0; k < kWPer; ++k) {
-            const int e = threadIdx.x + 256 * k;
-            if (e < kWCh) {
-                const int row = e >> 2, q = e & 3;
-                const int tap = row / COT, co = it.co0 + row % COT;
-                wv[k] = zero;
-                if (co < Cout) wv[k] = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + co) * Cin + c0 + q * 8);
-            }
+            const int e = min((int)threadIdx.x + 256 * k, kWCh - 1);
+            const int row = e >> 2, q = e & 3;
+            const int tap = row / COT, co = it.co0 + row % COT;
+            wv[k] = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + min(co, Cout - 1)) * Cin + c0 + q * 8);
+            win |= (uint32_t)(co < Cout) << k;
         }
     };
     if ((int)blockIdx.x < items) issue(item_of(blockIdx.x), 0);
@@ -173,12 +233,14 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
 #pragma unroll
             for (int k = 0; k < kHalfPer; ++k) {
                 const int e = threadIdx.x + 256 * k;
-                if (e < kHalfCh) *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = hv[k];
+                if (e < kHalfCh)
+                    *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = ((hin >> k) & 1u) ? hv[k] : zero;
             }
 #pragma unroll
             for (int k = 0; k < kWPer; ++k) {
                 const int e = threadIdx.x + 256 * k;
-                if (e < kWCh) *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = wv[k];
+                if (e < kWCh)
+                    *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = ((win >> k) & 1u) ? wv[k] : zero;
             }
             // the next step: the next slice of this item, or slice 0 of the
             // next item
@@ -188,7 +250,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                 if (nitem < items) issue(item_of(nitem), more ? c0 + 32 : 0);
             }
             __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the taps
-            __syncthreads();
+            lds_barrier();
 #pragma unroll 1
             for (int dx = 0; dx < 3; ++dx) {
                 // operands: halo rows rw0 .. rw0+9 shifted by dx (A: lane =
@@ -200,7 +262,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
 #pragma unroll
                     for (int t = 0; t < 2; ++t)
                         b[dy][t] = *reinterpret_cast<const bf16x8 *>(
-                            &s_w[((dy * 3 + dx) * COT + cw0 + 16 * t + l16) * kRow + kq]);
+                            &s_w[((dy * 3 + dx) * COT + cw0 + chan_of(t, l16)) * kRow + kq]);
                 // two halves of 4 output rows (6 input rows each), so that
                 // consecutive MFMAs go to 8 different accumulators
 #pragma unroll
@@ -220,40 +282,51 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                                                                                            acc[4 * h + i][t], 0, 0, 0);
                 }
             }
-            __syncthreads();
+            lds_barrier();
         }
         // Epilogue straight from the accumulators: with the weights as the
-        // MFMA's first operand the result is D[channel][pixel] and lane l holds
-        // channels 4*(l/16) .. +3 of pixel column l % 16, 8 contiguous bytes
-        // of the NHWC output; bias and ReLU applied, or the mask.
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int co = it.co0 + cw0 + 16 * t + 4 * (lane >> 4);
+        // MFMA's first operand the result is D[channel][pixel]: lane l holds
+        // rows 4*(l/16) .. +3 of both channel tiles for pixel column l % 16,
+        // and with the rows mapped by chan_of those are channels
+        // 8*(l/16) .. +7, 16 contiguous bytes of the NHWC output; bias and
+        // ReLU applied, or the mask.
+        {
+            const int co = it.co0 + cw0 + 8 * (lane >> 4);
             const int x = it.x0 + l16;
-            float bv[4];
+            float bv[8];
 #pragma unroll
-            for (int e = 0; e < 4; ++e) bv[e] = (bias && co + e < Cout) ? bias[co + e] : 0.0f;
-            if (x >= W || co >= Cout) continue;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int y = it.y0 + rw0 + i;
-                if (y >= H) break;
-                const size_t off = (((size_t)it.n * H + y) * W + x) * Cout + co;
-                bf16x4 v;
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float f = acc[i][t][e] + bv[e];
-                    if (relu) f = fmaxf(f, 0.0f);
-                    v[e] = f2b(f);
-                }
+            for (int e = 0; e < 8; ++e) bv[e] = (bias && co + e < Cout) ? bias[co + e] : 0.0f;
+            if (x < W && co < Cout) {
+                // the ReLU mask of all 8 rows loaded before any is used (rows
+                // past the image read row H - 1 and are not stored)
+                bf16x8 m[8];
                 if (mask) {
-                    // gradient through the ReLU whose output is `mask` (same shape)
-                    const bf16x4 m = *reinterpret_cast<const bf16x4 *>(mask + off);
 #pragma unroll
-                    for (int e = 0; e < 4; ++e)
-                        if (!(b2f(m[e]) > 0.0f)) v[e] = f2b(0.0f);
+                    for (int i = 0; i < 8; ++i) {
+                        const int y = min(it.y0 + rw0 + i, H - 1);
+                        m[i] = *reinterpret_cast<const bf16x8 *>(mask + (((size_t)it.n * H + y) * W + x) * Cout + co);
+                    }
                 }
-                *reinterpret_cast<bf16x4 *>(out + off) = v;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int y = it.y0 + rw0 + i;
+                    if (y >= H) break;
+                    const size_t off = (((size_t)it.n * H + y) * W + x) * Cout + co;
+                    bf16x8 v;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        float f = acc[i][e >> 2][e & 3] + bv[e];
+                        if (relu) f = fmaxf(f, 0.0f);
+                        v[e] = f2b(f);
+                    }
+                    if (mask) {
+                        // gradient through the ReLU whose output is `mask` (same shape)
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            if (!(b2f(m[i][e]) > 0.0f)) v[e] = f2b(0.0f);
+                    }
+                    *reinterpret_cast<bf16x8 *>(out + off) = v;
+                }
             }
         }
     }
@@ -347,13 +420,13 @@ __global__ __launch_bounds__(256, 2) void k_conv_dw(ConvIn I, int32_t N, int32_t
             dv[k] = zero;
             if (y < H && x < W) dv[k] = *reinterpret_cast<const bf16x8 *>(dpre + (((size_t)n * H + y) * W + x) * Cout + co0 + 8 * q);
         }
+        // unconditional halo loads (spare threads of the last round reload
+        // the last chunk and do not store it): no load waits in a branch
 #pragma unroll
         for (int k = 0; k < kXPer; ++k) {
-            const int e = threadIdx.x + 256 * k;
-            if (e < kXCh) {
-                const int hp = e >> 2, q = e & 3;
-                xv[k] = load8(I, n, y0 + hp / kHalo - 1, x0 + hp % kHalo - 1, H, W, ci0 + 8 * q);
-            }
+            const int e = min((int)threadIdx.x + 256 * k, kXCh - 1);
+            const int hp = e >> 2, q = e & 3;
+            xv[k] = load8(I, n, y0 + hp / kHalo - 1, x0 + hp % kHalo - 1, H, W, ci0 + 8 * q);
         }
 #pragma unroll
         for (int k = 0; k < kDPer; ++k) {
@@ -572,15 +645,32 @@ __global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H
 }
 
 // Input features -> padded bf16: log1p radiance, albedo, normal, log1p depth.
+template <bool PAIRS>
 __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *__restrict__ x) {
-    // one thread per pixel: its 10 features in, 32 bf16 out as four 16-byte stores
+    // one thread per pixel: its 10 features in (with PAIRS, for 8-byte
+    // aligned features, as five 8-byte loads), 32 bf16 out as four 16-byte
+    // stores
+    static_assert(kInChannels == 10, "five float2 per pixel");
     for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
+        float in[kInChannels];
+        if (PAIRS) {
+            const float2 *src = reinterpret_cast<const float2 *>(f + p * kInChannels);
+#pragma unroll
+            for (int k = 0; k < kInChannels / 2; ++k) {
+                const float2 v = src[k];
+                in[2 * k] = v.x;
+                in[2 * k + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < kInChannels; ++c) in[c] = f[p * kInChannels + c];
+        }
         bf16x8 o[kInPadded / 8];
 #pragma unroll
         for (int c = 0; c < kInPadded; ++c) {
             float v = 0.0f;
             if (c < kInChannels) {
-                v = f[p * kInChannels + c];
+                v = in[c];
                 if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
             }
             o[c / 8][c % 8] = f2b(v);
@@ -1005,7 +1095,10 @@ void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
     hipStream_t s = c->ctx->stream;
     const long long n0 = (long long)N * H * W;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-    hipLaunchKernelGGL(k_pack_input, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
+    if (((uintptr_t)feat & 7u) == 0)
+        hipLaunchKernelGGL(k_pack_input<true>, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
+    else
+        hipLaunchKernelGGL(k_pack_input<false>, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
     ConvLayer *L = c->layer;
     auto B = [&](int l) { return c->p + L[l].b_off; };
     conv(c, cin1(a.x, kInPadded), N, H, W, L[L_E0A].wf, B(L_E0A), 32, true, a.e0a);
