@@ -142,11 +142,24 @@ constexpr int kRow = 40;                  // LDS row: 32 channels + 8 bf16 paddi
 // holding rows 4g .. 4g+3 of both tiles holds 8 consecutive channels.
 __device__ __forceinline__ int chan_of(int t, int m) { return 8 * (m >> 2) + 4 * t + (m & 3); }
 
+// The 1x1 inference head fused into the last convolution (Cout = 32): with
+// `out` set, that convolution's output is not stored; each pixel's 32
+// channels (bf16-rounded, as stored) go through the head instead, and the
+// prediction expm1(head), clamped at 0, or the input radiance where the camera
+// ray missed (depth guide 0), is written as 3 floats.  The sums of the
+// training head (k_head), in another order.
+struct HeadArgs {
+    const float *w;      // [3][32]
+    const float *b;      // [3]
+    const float *feat;   // [pixels][10]
+    float *out;          // [pixels][3], or null: no head
+};
+
 template <int COT>
 __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                     const bf16 *__restrict__ wt, const float *__restrict__ bias,
                                                     int32_t Cout, int32_t relu, const bf16 *__restrict__ mask,
-                                                    bf16 *__restrict__ out) {
+                                                    bf16 *__restrict__ out, HeadArgs head) {
     constexpr int ROWS = 1024 / COT;                 // tile rows: 32 (COT 32) or 16 (COT 64)
     constexpr int WR = ROWS / 8;                     // waves along the rows
     constexpr int kInElems = (ROWS + 2) * kHalo * kRow, kWElems = 9 * COT * kRow;
@@ -296,7 +309,53 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
             float bv[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) bv[e] = (bias && co + e < Cout) ? bias[co + e] : 0.0f;
-            if (x < W && co < Cout) {
+            if (COT == 32 && head.out) {
+                // fused head: this lane's 8 channels' share of the 3 sums,
+                // completed across the 4 lanes of the pixel (lanes l, l ^ 16,
+                // l ^ 32, l ^ 48 share the pixel column and the control flow)
+                float hw[kOut][8];
+#pragma unroll
+                for (int k = 0; k < kOut; ++k)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) hw[k][e] = head.w[k * 32 + 8 * (lane >> 4) + e];
+                const int g = lane >> 4;
+                const float hb = g < kOut ? head.b[g] : 0.0f;
+                if (x < W) {
+                    // the 8 rows' depth guides and input radiance loaded
+                    // before any is used (rows past the image read row H - 1)
+                    float depth[8], rad[8];
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const size_t p = ((size_t)it.n * H + min(it.y0 + rw0 + i, H - 1)) * W + x;
+                        depth[i] = head.feat[p * kInChannels + 9];
+                        rad[i] = head.feat[p * kInChannels + min(g, kOut - 1)];
+                    }
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const int y = it.y0 + rw0 + i;
+                        if (y >= H) break;
+                        float sum[kOut] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+                        for (int e = 0; e < 8; ++e) {
+                            float f = acc[i][e >> 2][e & 3] + bv[e];
+                            if (relu) f = fmaxf(f, 0.0f);
+                            f = b2f(f2b(f));
+#pragma unroll
+                            for (int k = 0; k < kOut; ++k) sum[k] += hw[k][e] * f;
+                        }
+#pragma unroll
+                        for (int k = 0; k < kOut; ++k) {
+                            sum[k] += __shfl_xor(sum[k], 16);
+                            sum[k] += __shfl_xor(sum[k], 32);
+                        }
+                        if (g < kOut) {
+                            const size_t p = ((size_t)it.n * H + y) * W + x;
+                            const float mine = g == 0 ? sum[0] : (g == 1 ? sum[1] : sum[2]);
+                            head.out[p * kOut + g] = depth[i] == 0.0f ? rad[i] : fmaxf(expm1f(hb + mine), 0.0f);
+                        }
+                    }
+                }
+            } else if (x < W && co < Cout) {
                 // the ReLU mask of all 8 rows loaded before any is used (rows
                 // past the image read row H - 1 and are not stored)
                 bf16x8 m[8];
@@ -850,39 +909,6 @@ __global__ __launch_bounds__(256) void k_head_reduce(const float *__restrict__ p
     }
 }
 
-// Inference head: the 1x1 convolution to radiance (the same sums as k_head),
-// expm1 back from log space, misses passed through; one thread per pixel with
-// 16-byte loads of its 32 channels.
-__global__ __launch_bounds__(256) void k_head_infer(const bf16 *__restrict__ x, long long npix,
-                                                    const float *__restrict__ wh, const float *__restrict__ bh,
-                                                    const float *__restrict__ feat, float *__restrict__ out) {
-    __shared__ float sw[kOut * 32 + kOut];
-    for (int i = threadIdx.x; i < kOut * 32; i += 256) sw[i] = wh[i];
-    if (threadIdx.x < kOut) sw[kOut * 32 + threadIdx.x] = bh[threadIdx.x];
-    __syncthreads();
-    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
-        if (feat[p * kInChannels + 9] == 0.0f) {
-            for (int k = 0; k < kOut; ++k) out[p * kOut + k] = feat[p * kInChannels + k];
-            continue;
-        }
-        const bf16x8 *src = reinterpret_cast<const bf16x8 *>(x + p * 32);
-        float xi[32];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const bf16x8 v = src[q];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) xi[8 * q + j] = b2f(v[j]);
-        }
-#pragma unroll
-        for (int k = 0; k < kOut; ++k) {
-            float sum = sw[kOut * 32 + k];
-#pragma unroll
-            for (int c = 0; c < 32; ++c) sum += sw[k * 32 + c] * xi[c];
-            out[p * kOut + k] = fmaxf(expm1f(sum), 0.0f);
-        }
-    }
-}
-
 // Adam (Kingma & Ba 2015) on the FP32 master parameters.
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
                        long long n, float lr, float b1, float b2, float eps, float c1, float c2) {
@@ -1044,7 +1070,7 @@ int conv_cus(s3dg_cnn *c) {
 // two workgroups per CU at most, each walking its share of the (tile,
 // channel block) items.
 void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
-          bf16 *out, const bf16 *mask = nullptr) {
+          bf16 *out, const bf16 *mask = nullptr, HeadArgs head = HeadArgs{nullptr, nullptr, nullptr, nullptr}) {
     hipStream_t s = c->ctx->stream;
     const int r = relu ? 1 : 0;
     const long long wgs = 2LL * conv_cus(c);
@@ -1052,11 +1078,11 @@ void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, con
     if (Cout % 64 == 0) {
         const long long items = (long long)N * tx * ((H + 15) / 16) * (Cout / 64);
         hipLaunchKernelGGL((k_conv3x3<64>), dim3((unsigned)std::min(items, wgs)), dim3(256), 0, s, I, N, H, W, wt, bias,
-                           Cout, r, mask, out);
+                           Cout, r, mask, out, head);
     } else {
         const long long items = (long long)N * tx * ((H + 31) / 32) * ((Cout + 31) / 32);
         hipLaunchKernelGGL((k_conv3x3<32>), dim3((unsigned)std::min(items, wgs)), dim3(256), 0, s, I, N, H, W, wt, bias,
-                           Cout, r, mask, out);
+                           Cout, r, mask, out, head);
     }
 }
 
@@ -1091,7 +1117,9 @@ void pack_weights(s3dg_cnn *c) {
 }
 
 // Forward pass; returns the decoder output (d0b) for the head.
-void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
+// Forward pass; with d_out (inference) the last convolution runs the fused
+// head into d_out instead of storing d0b.
+void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W, float *d_out = nullptr) {
     hipStream_t s = c->ctx->stream;
     const long long n0 = (long long)N * H * W;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
@@ -1117,7 +1145,8 @@ void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W) {
     conv(c, cin_cat(a.d2b, 96, a.e1b, 64), N, H1, W1, L[L_D1A].wf, B(L_D1A), 64, true, a.d1a);
     conv(c, cin1(a.d1a, 64), N, H1, W1, L[L_D1B].wf, B(L_D1B), 64, true, a.d1b);
     conv(c, cin_cat(a.d1b, 64, a.e0b, 32), N, H, W, L[L_D0A].wf, B(L_D0A), 32, true, a.d0a);
-    conv(c, cin1(a.d0a, 32), N, H, W, L[L_D0B].wf, B(L_D0B), 32, true, a.d0b);
+    conv(c, cin1(a.d0a, 32), N, H, W, L[L_D0B].wf, B(L_D0B), 32, true, a.d0b, nullptr,
+         HeadArgs{c->p + c->head_w, c->p + c->head_b, feat, d_out});
 }
 
 // Backward of one conv layer whose pre-activation gradient is in dpre (the
@@ -1322,11 +1351,7 @@ extern "C" int s3dg_cnn_infer_device(s3dg_cnn *c, int32_t n, int32_t h, int32_t 
     rc = prepare(c, n0, a);
     if (rc) return rc;
     hipStream_t s = c->ctx->stream;
-    auto launch = [&]() {
-        forward(c, a, d_feat, n, h, w);
-        hipLaunchKernelGGL(k_head_infer, dim3((unsigned)std::min<long long>((n0 + 255) / 256, 8192)), dim3(256), 0,
-                           s, a.d0b, n0, c->p + c->head_w, c->p + c->head_b, d_feat, d_out);
-    };
+    auto launch = [&]() { forward(c, a, d_feat, n, h, w, d_out); };
     // The ~20 launches of one inference as one graph launch once the same
     // shape and buffers come back (a denoiser runs frame after frame on the
     // same buffers).  The kernels read the weights through fixed pointers,
