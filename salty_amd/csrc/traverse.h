@@ -113,9 +113,15 @@ __device__ __forceinline__ DRay to_local(const GInst &in, const DRay &r) {
     const float nz = (r.dx * m[0][2]) + (r.dy * m[1][2]) + (r.dz * m[2][2]);
     const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
     DRay l;
-    l.ox = X / W;
-    l.oy = Y / W;
-    l.oz = Z / W;
+    l.ox = X;
+    l.oy = Y;
+    l.oz = Z;
+    // an affine matrix gives W = 1 exactly, and X / 1 is X
+    if (W != 1.0f) {
+        l.ox = X / W;
+        l.oy = Y / W;
+        l.oz = Z / W;
+    }
     l.dx = nx / mag;
     l.dy = ny / mag;
     l.dz = nz / mag;
@@ -154,11 +160,14 @@ __device__ __forceinline__ RayPre precompute(const DRay &r) {
 }
 
 // Stack entries: a group of a node's hit inner children still to visit, a
-// group of a top-level node's hit shapes still to enter, the return to the
-// world ray after an instance, and a group of a node's hit leaves whose
-// triangles are still to test.  A top-level node thus takes at most two
-// entries, like any other node.
-enum { ENT_GROUP = 0u, ENT_INST = 1u, ENT_RESTORE = 2u, ENT_LEAVES = 3u };
+// group of a top-level node's hit shapes still to enter, and a group of a
+// node's hit leaves whose triangles are still to test.  A top-level node thus
+// takes at most two entries, like any other node.  Entries a top-level node
+// pushes carry kWorldEntry: popping one while the ray is an instance's local
+// ray first returns to the world ray (in the same step, so leaving an
+// instance costs no step of its own).
+enum { ENT_GROUP = 0u, ENT_INST = 1u, ENT_LEAVES = 3u };
+constexpr uint32_t kWorldEntry = 1u << 29;
 
 // Traversal stack: the first LDS entries of a lane live in LDS (stride
 // kBlock), the rest in a global overflow area (stride gstride).  The pointers
@@ -268,6 +277,7 @@ struct Tracer {
     uint32_t leaf_base, leaf_lo, leaf_hi;   // that node's leaf_base and count words
     int32_t cur_shape;
     bool have_node;
+    bool in_local;       // `ray` is an instance's local ray
     uint32_t visits, tests;
 
     // Starts a ray at the root of scene `rec`.
@@ -283,6 +293,7 @@ struct Tracer {
         node = rec.root;
         cur_shape = rec.single_shape;
         have_node = true;
+        in_local = false;
         tri_cur = tri_end = 0;
         leaf_mask = 0;
         leaf_base = leaf_lo = leaf_hi = 0;
@@ -327,13 +338,18 @@ struct Tracer {
     }
 
     // Pops one entry: a node group yields its next child, an instance entry
-    // enters the instance (or tests the sphere), a restore entry returns to
-    // the world ray, a leaf group becomes the pending triangles.  Returns true
-    // when an any-hit search is over.
+    // enters the instance (or tests the sphere), a leaf group becomes the
+    // pending triangles; a top-level entry popped inside an instance returns
+    // to the world ray first.  Returns true when an any-hit search is over.
     template <class ST>
     __device__ __forceinline__ bool pop(const SceneDev &S, ST &st) {
         const uint2 e = st.top();
         const uint32_t kind = e.y >> 30;
+        if ((e.y & kWorldEntry) && in_local) {
+            in_local = false;
+            ray = wray;
+            pre = precompute(ray);
+        }
         if (kind == ENT_GROUP) {
             const uint32_t imask = e.y & 0xffu;
             uint32_t rem = (e.y >> 8) & 0xffu;
@@ -373,17 +389,13 @@ struct Tracer {
             } else {
                 cur_shape = in.shape;
                 if (in.type == INST_TRANSFORMED) {
-                    st.push(make_uint2(0u, ENT_RESTORE << 30));
+                    in_local = true;
                     ray = to_local(in, wray);
                     pre = precompute(ray);
                 }
                 node = in.blas_root;
                 have_node = true;
             }
-        } else {   // ENT_RESTORE
-            st.sp--;
-            ray = wray;
-            pre = precompute(ray);
         }
         return false;
     }
@@ -463,7 +475,7 @@ struct Tracer {
                 // first; each top-level leaf slot holds one shape record, so
                 // a slot's record is lbase plus the leaf slots below it
                 const uint32_t lslots = valid & ~imask;
-                st.push(make_uint2(lbase, (ENT_INST << 30) | (lslots << 8) | to_korder(leaves, pre.oct)));
+                st.push(make_uint2(lbase, (ENT_INST << 30) | kWorldEntry | (lslots << 8) | to_korder(leaves, pre.oct)));
             } else if (tri_cur == tri_end && leaf_mask == 0) {
                 leaf_base = lbase;
                 leaf_lo = cnt_lo;
@@ -478,7 +490,8 @@ struct Tracer {
         if (inner) {
             const uint32_t s0 = (uint32_t)__builtin_ctz(inner) ^ pre.oct;
             const uint32_t rem = inner & (inner - 1u);
-            if (rem) st.push(make_uint2(child_base, (ENT_GROUP << 30) | (rem << 8) | imask));
+            static_assert((kTopLevelBit >> 2) == kWorldEntry, "a top-level node's group entry is a world entry");
+            if (rem) st.push(make_uint2(child_base, (ENT_GROUP << 30) | (top >> 2) | (rem << 8) | imask));
             node = child_base + (uint32_t)__builtin_popcount(imask & ((1u << s0) - 1u));
             have_node = true;
         } else {
