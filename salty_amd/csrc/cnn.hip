@@ -45,6 +45,9 @@ struct ConvIn {
     const bf16 *in1;
     const bf16 *in2;
     int32_t C1, C2, s1;   // s1 is 1 or 2
+    // the first layer at inference: the 10 FP32 input features per pixel
+    // (8-byte aligned), packed while staged as k_pack_input packs them
+    const float *feat;
 };
 
 // Workgroup barrier for LDS only: this wave's LDS operations complete
@@ -155,11 +158,16 @@ struct HeadArgs {
     float *out;          // [pixels][3], or null: no head
 };
 
-template <int COT>
+// Epilogue of a convolution: bias + ReLU (+ 2x2 pooling), the fused head, or
+// the data gradient's ReLU mask; one kernel instantiation each, so that each
+// carries only its own registers.
+enum { EPI_FORWARD = 0, EPI_HEAD = 1, EPI_MASK = 2 };
+
+template <int COT, int EPI, bool FEAT>
 __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                     const bf16 *__restrict__ wt, const float *__restrict__ bias,
                                                     int32_t Cout, int32_t relu, const bf16 *__restrict__ mask,
-                                                    bf16 *__restrict__ out, HeadArgs head) {
+                                                    bf16 *__restrict__ out, HeadArgs head, bf16 *__restrict__ pool) {
     constexpr int ROWS = 1024 / COT;                 // tile rows: 32 (COT 32) or 16 (COT 64)
     constexpr int WR = ROWS / 8;                     // waves along the rows
     constexpr int kInElems = (ROWS + 2) * kHalo * kRow, kWElems = 9 * COT * kRow;
@@ -195,7 +203,10 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
     // are in flight while the previous step computes
     constexpr int kHalfCh = (ROWS + 2) * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
     constexpr int kWCh = 9 * COT * 4, kWPer = (kWCh + 255) / 256;
-    bf16x8 hv[kHalfPer], wv[kWPer];
+    // FEAT: one halo pixel (10 floats) per thread and round instead
+    constexpr int kFeatPix = (ROWS + 2) * kHalo, kFeatPer = (kFeatPix + 255) / 256;
+    bf16x8 hv[FEAT ? 1 : kHalfPer], wv[kWPer];
+    float fv[FEAT ? kFeatPer : 1][kInChannels];
     uint32_t hin = 0, win = 0;   // chunks whose value is real (inside the image / an existing channel)
     const auto issue = [&](const Item &it, int c0) {
         // every chunk load is issued unconditionally and raw (the last
@@ -203,6 +214,27 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
         // it; out-of-range chunks are zeroed when stored), so that nothing
         // between the loads and the next step's LDS stores waits for them
         hin = win = 0;
+        if constexpr (FEAT) {
+            typedef float f32x2 __attribute__((ext_vector_type(2)));
+            typedef __attribute__((address_space(1))) const f32x2 gfloat2;
+            const float *feat = reinterpret_cast<const float *>(uniform_ptr(reinterpret_cast<const bf16 *>(I.feat)));
+#pragma unroll
+            for (int k = 0; k < kFeatPer; ++k) {
+                const int pix = min((int)threadIdx.x + 256 * k, kFeatPix - 1);
+                const int y = it.y0 + pix / kHalo - 1, x = it.x0 + pix % kHalo - 1;
+                const bool inside = y >= 0 && y < H && x >= 0 && x < W;
+                const uint32_t sp = ((uint32_t)it.n * (uint32_t)H + (uint32_t)(inside ? y : 0)) * (uint32_t)W +
+                                    (uint32_t)(inside ? x : 0);
+                const gfloat2 *src = (const gfloat2 *)(feat + (size_t)sp * kInChannels);
+#pragma unroll
+                for (int j = 0; j < kInChannels / 2; ++j) {
+                    const f32x2 v = src[j];
+                    fv[k][2 * j] = v.x;
+                    fv[k][2 * j + 1] = v.y;
+                }
+                hin |= (uint32_t)inside << k;
+            }
+        }
         // the slice lies in one source (C1 is a multiple of 32), chosen once
         // per step; pixel indices in 32 bits (N*H*W <= 2^27)
         const bool up = c0 < I.C1;
@@ -213,7 +245,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
         const uint32_t hs = (uint32_t)(H + sh) >> sh, ws = (uint32_t)(W + sh) >> sh;
         typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
 #pragma unroll
-        for (int k = 0; k < kHalfPer; ++k) {
+        for (int k = 0; k < (FEAT ? 0 : kHalfPer); ++k) {
             const int e = min((int)threadIdx.x + 256 * k, kHalfCh - 1);
             const int pix = e >> 2, q = e & 3;
             const int y = it.y0 + pix / kHalo - 1, x = it.x0 + pix % kHalo - 1;
@@ -243,8 +275,31 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
 #pragma unroll
                 for (int e = 0; e < 4; ++e) acc[i][t][e] = 0.0f;
         for (int c0 = 0; c0 < Cin; c0 += 32) {
+            if constexpr (FEAT) {
+                // k_pack_input's packing: log1p of radiance and depth, bf16,
+                // zero padding to 32 channels (and outside the image)
 #pragma unroll
-            for (int k = 0; k < kHalfPer; ++k) {
+                for (int k = 0; k < kFeatPer; ++k) {
+                    const int pix = threadIdx.x + 256 * k;
+                    if (pix < kFeatPix) {
+                        bf16x8 o[kInPadded / 8];
+#pragma unroll
+                        for (int c = 0; c < kInPadded; ++c) {
+                            float v = 0.0f;
+                            if (c < kInChannels) {
+                                v = fv[k][c];
+                                if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
+                            }
+                            o[c / 8][c % 8] = ((hin >> k) & 1u) ? f2b(v) : f2b(0.0f);
+                        }
+#pragma unroll
+                        for (int q = 0; q < kInPadded / 8; ++q)
+                            *reinterpret_cast<bf16x8 *>(&s_in[pix * kRow + q * 8]) = o[q];
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < (FEAT ? 0 : kHalfPer); ++k) {
                 const int e = threadIdx.x + 256 * k;
                 if (e < kHalfCh)
                     *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = ((hin >> k) & 1u) ? hv[k] : zero;
@@ -309,63 +364,64 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
             float bv[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) bv[e] = (bias && co + e < Cout) ? bias[co + e] : 0.0f;
-            if (COT == 32 && head.out) {
-                // fused head: this lane's 8 channels' share of the 3 sums,
-                // completed across the 4 lanes of the pixel (lanes l, l ^ 16,
-                // l ^ 32, l ^ 48 share the pixel column and the control flow)
-                float hw[kOut][8];
-#pragma unroll
-                for (int k = 0; k < kOut; ++k)
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) hw[k][e] = head.w[k * 32 + 8 * (lane >> 4) + e];
+            if (EPI == EPI_HEAD) {
+                // fused head as one GEMM per row on the matrix cores: the
+                // weights are the first operand with output k in row 4k (so
+                // that lane group k ends with output k of its pixel column),
+                // split into bf16 high and low parts (two MFMAs, about 16
+                // bits of the FP32 weights); the second operand is this
+                // lane's 8 channels of the pixel, bf16 as stored
                 const int g = lane >> 4;
+                bf16x8 whi, wlo;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const float wv = (l16 & 3) == 0 && l16 < 4 * kOut ? head.w[(l16 >> 2) * 32 + 8 * g + e] : 0.0f;
+                    whi[e] = f2b(wv);
+                    wlo[e] = f2b(wv - b2f(whi[e]));
+                }
                 const float hb = g < kOut ? head.b[g] : 0.0f;
-                if (x < W) {
-                    // the 8 rows' depth guides and input radiance loaded
-                    // before any is used (rows past the image read row H - 1)
-                    float depth[8], rad[8];
+                // the 8 rows' depth guides and input radiance loaded before
+                // any is used (rows past the image read row H - 1, columns
+                // past it column W - 1)
+                const int xc = min(x, W - 1);
+                float depth[8], rad[8];
 #pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const size_t p = ((size_t)it.n * H + min(it.y0 + rw0 + i, H - 1)) * W + x;
-                        depth[i] = head.feat[p * kInChannels + 9];
-                        rad[i] = head.feat[p * kInChannels + min(g, kOut - 1)];
+                for (int i = 0; i < 8; ++i) {
+                    const size_t p = ((size_t)it.n * H + min(it.y0 + rw0 + i, H - 1)) * W + xc;
+                    depth[i] = head.feat[p * kInChannels + 9];
+                    rad[i] = head.feat[p * kInChannels + min(g, kOut - 1)];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int y = it.y0 + rw0 + i;
+                    if (y >= H) break;
+                    bf16x8 v;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        float f = acc[i][e >> 2][e & 3] + bv[e];
+                        if (relu) f = fmaxf(f, 0.0f);
+                        v[e] = f2b(f);
                     }
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const int y = it.y0 + rw0 + i;
-                        if (y >= H) break;
-                        float sum[kOut] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) {
-                            float f = acc[i][e >> 2][e & 3] + bv[e];
-                            if (relu) f = fmaxf(f, 0.0f);
-                            f = b2f(f2b(f));
-#pragma unroll
-                            for (int k = 0; k < kOut; ++k) sum[k] += hw[k][e] * f;
-                        }
-#pragma unroll
-                        for (int k = 0; k < kOut; ++k) {
-                            sum[k] += __shfl_xor(sum[k], 16);
-                            sum[k] += __shfl_xor(sum[k], 32);
-                        }
-                        if (g < kOut) {
-                            const size_t p = ((size_t)it.n * H + y) * W + x;
-                            const float mine = g == 0 ? sum[0] : (g == 1 ? sum[1] : sum[2]);
-                            head.out[p * kOut + g] = depth[i] == 0.0f ? rad[i] : fmaxf(expm1f(hb + mine), 0.0f);
-                        }
+                    f32x4 hs = {0.0f, 0.0f, 0.0f, 0.0f};
+                    hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi, v, hs, 0, 0, 0);
+                    hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlo, v, hs, 0, 0, 0);
+                    if (g < kOut && x < W) {
+                        const size_t p = ((size_t)it.n * H + y) * W + x;
+                        head.out[p * kOut + g] = depth[i] == 0.0f ? rad[i] : fmaxf(expm1f(hb + hs[0]), 0.0f);
                     }
                 }
             } else if (x < W && co < Cout) {
                 // the ReLU mask of all 8 rows loaded before any is used (rows
                 // past the image read row H - 1 and are not stored)
                 bf16x8 m[8];
-                if (mask) {
+                if (EPI == EPI_MASK) {
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
                         const int y = min(it.y0 + rw0 + i, H - 1);
                         m[i] = *reinterpret_cast<const bf16x8 *>(mask + (((size_t)it.n * H + y) * W + x) * Cout + co);
                     }
                 }
+                bf16x8 prev;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) {
                     const int y = it.y0 + rw0 + i;
@@ -378,13 +434,42 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                         if (relu) f = fmaxf(f, 0.0f);
                         v[e] = f2b(f);
                     }
-                    if (mask) {
+                    if (EPI == EPI_MASK) {
                         // gradient through the ReLU whose output is `mask` (same shape)
 #pragma unroll
                         for (int e = 0; e < 8; ++e)
                             if (!(b2f(m[i][e]) > 0.0f)) v[e] = f2b(0.0f);
                     }
                     *reinterpret_cast<bf16x8 *>(out + off) = v;
+                    if (EPI == EPI_FORWARD && pool) {
+                        // 2x2 average pooling of the stored values, summed top
+                        // left, top right, bottom left, bottom right, times
+                        // 0.25: the tile's rows pair up within a lane (y0 and
+                        // rw0 are even), its columns with lane ^ 1 (x0 is even);
+                        // H and W are even, so both lanes of a pair are here
+                        if (i & 1) {
+                            bf16x8 tr, br;
+                            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                            const u32x4 pv = __builtin_bit_cast(u32x4, prev), cv = __builtin_bit_cast(u32x4, v);
+                            u32x4 ptr, pbr;
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                ptr[j] = (uint32_t)__shfl_xor((int)pv[j], 1);
+                                pbr[j] = (uint32_t)__shfl_xor((int)cv[j], 1);
+                            }
+                            tr = __builtin_bit_cast(bf16x8, ptr);
+                            br = __builtin_bit_cast(bf16x8, pbr);
+                            if ((x & 1) == 0) {
+                                bf16x8 o;
+#pragma unroll
+                                for (int e = 0; e < 8; ++e)
+                                    o[e] = f2b(0.25f * (b2f(prev[e]) + b2f(tr[e]) + b2f(v[e]) + b2f(br[e])));
+                                const size_t po = (((size_t)it.n * (H / 2) + (y >> 1)) * (W / 2) + (x >> 1)) * Cout + co;
+                                *reinterpret_cast<bf16x8 *>(pool + po) = o;
+                            }
+                        }
+                        prev = v;
+                    }
                 }
             }
         }
@@ -613,26 +698,6 @@ __global__ __launch_bounds__(256) void k_slice_sum(SliceSum a, SliceSum b, int32
     }
 }
 
-// 2x2 average pooling, NHWC.
-// 2x2 average pooling; one thread per output pixel and group of 8 channels
-// (16-byte loads and stores), 32-bit indexing (the host checks the range).
-__global__ void k_pool2(const bf16 *__restrict__ x, int32_t N, int32_t H, int32_t W, int32_t C, bf16 *__restrict__ y) {
-    const uint32_t Ho = (uint32_t)H / 2, Wo = (uint32_t)W / 2, G = (uint32_t)C / 8;
-    const uint32_t n_out = (uint32_t)N * Ho * Wo * G;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n_out; i += gridDim.x * 256u) {
-        const uint32_t g = i % G, p = i / G;
-        const uint32_t xo = p % Wo, yo = (p / Wo) % Ho, n = p / (Wo * Ho);
-        const size_t b = (((size_t)n * H + 2 * yo) * W + 2 * xo) * C + 8 * g;
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(x + b);
-        const bf16x8 bb = *reinterpret_cast<const bf16x8 *>(x + b + C);
-        const bf16x8 c = *reinterpret_cast<const bf16x8 *>(x + b + (size_t)W * C);
-        const bf16x8 d = *reinterpret_cast<const bf16x8 *>(x + b + (size_t)W * C + C);
-        bf16x8 o;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) o[k] = f2b(0.25f * (b2f(a[k]) + b2f(bb[k]) + b2f(c[k]) + b2f(d[k])));
-        *reinterpret_cast<bf16x8 *>(y + (size_t)p * C + 8 * g) = o;
-    }
-}
 // dX += upsample(dY) / 4, the last contribution to dX, then (with mask) the
 // gradient through the ReLU whose output is `mask`.  One thread per pixel and
 // group of 8 channels (16-byte accesses), 32-bit indexing (N*H*W <= 2^27,
@@ -1070,19 +1135,37 @@ int conv_cus(s3dg_cnn *c) {
 // two workgroups per CU at most, each walking its share of the (tile,
 // channel block) items.
 void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, const float *bias, int Cout, bool relu,
-          bf16 *out, const bf16 *mask = nullptr, HeadArgs head = HeadArgs{nullptr, nullptr, nullptr, nullptr}) {
+          bf16 *out, const bf16 *mask = nullptr, HeadArgs head = HeadArgs{nullptr, nullptr, nullptr, nullptr},
+          bf16 *pool = nullptr) {
     hipStream_t s = c->ctx->stream;
     const int r = relu ? 1 : 0;
     const long long wgs = 2LL * conv_cus(c);
     const long long tx = (W + kTile - 1) / kTile;
-    if (Cout % 64 == 0) {
+    // the fused head needs the 32-channel tiles (all of a pixel's channels in one wave)
+    if (Cout % 64 == 0 && !head.out) {
         const long long items = (long long)N * tx * ((H + 15) / 16) * (Cout / 64);
-        hipLaunchKernelGGL((k_conv3x3<64>), dim3((unsigned)std::min(items, wgs)), dim3(256), 0, s, I, N, H, W, wt, bias,
-                           Cout, r, mask, out, head);
+        const dim3 grid((unsigned)std::min(items, wgs));
+        if (mask)
+            hipLaunchKernelGGL((k_conv3x3<64, EPI_MASK, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
+                               out, head, pool);
+        else
+            hipLaunchKernelGGL((k_conv3x3<64, EPI_FORWARD, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
+                               out, head, pool);
     } else {
         const long long items = (long long)N * tx * ((H + 31) / 32) * ((Cout + 31) / 32);
-        hipLaunchKernelGGL((k_conv3x3<32>), dim3((unsigned)std::min(items, wgs)), dim3(256), 0, s, I, N, H, W, wt, bias,
-                           Cout, r, mask, out, head);
+        const dim3 grid((unsigned)std::min(items, wgs));
+        if (head.out)
+            hipLaunchKernelGGL((k_conv3x3<32, EPI_HEAD, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
+                               out, head, pool);
+        else if (mask)
+            hipLaunchKernelGGL((k_conv3x3<32, EPI_MASK, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
+                               out, head, pool);
+        else if (I.feat)
+            hipLaunchKernelGGL((k_conv3x3<32, EPI_FORWARD, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r,
+                               mask, out, head, pool);
+        else
+            hipLaunchKernelGGL((k_conv3x3<32, EPI_FORWARD, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r,
+                               mask, out, head, pool);
     }
 }
 
@@ -1104,8 +1187,10 @@ void conv_dw(s3dg_cnn *c, const DwParts &parts, const ConvIn &I, int N, int H, i
     hipLaunchKernelGGL(k_slice_sum, dim3((unsigned)(blocks_w + blocks_b)), dim3(256), 0, s, sw, sb, blocks_w);
 }
 
-ConvIn cin1(const bf16 *a, int C) { return ConvIn{a, nullptr, C, 0, 1}; }
-ConvIn cin_cat(const bf16 *low, int C1, const bf16 *skip, int C2) { return ConvIn{low, skip, C1, C2, 2}; }
+ConvIn cin1(const bf16 *a, int C) { return ConvIn{a, nullptr, C, 0, 1, nullptr}; }
+ConvIn cin_cat(const bf16 *low, int C1, const bf16 *skip, int C2) { return ConvIn{low, skip, C1, C2, 2, nullptr}; }
+// the input features, packed by the first convolution itself
+ConvIn cin_feat(const float *feat) { return ConvIn{nullptr, nullptr, kInPadded, 0, 1, feat}; }
 
 void pack_weights(s3dg_cnn *c) {
     for (int l = 0; l < L_COUNT; ++l) {
@@ -1123,21 +1208,27 @@ void forward(s3dg_cnn *c, Acts &a, const float *feat, int N, int H, int W, float
     hipStream_t s = c->ctx->stream;
     const long long n0 = (long long)N * H * W;
     const int H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4, H3 = H / 8, W3 = W / 8;
-    if (((uintptr_t)feat & 7u) == 0)
-        hipLaunchKernelGGL(k_pack_input<true>, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
-    else
-        hipLaunchKernelGGL(k_pack_input<false>, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
     ConvLayer *L = c->layer;
     auto B = [&](int l) { return c->p + L[l].b_off; };
-    conv(c, cin1(a.x, kInPadded), N, H, W, L[L_E0A].wf, B(L_E0A), 32, true, a.e0a);
-    conv(c, cin1(a.e0a, 32), N, H, W, L[L_E0B].wf, B(L_E0B), 32, true, a.e0b);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H1 * W1 * 4)), dim3(256), 0, s, a.e0b, N, H, W, 32, a.p0);
+    // inference with 8-byte aligned features: the first convolution packs
+    // them while staging (training keeps the packed input for its gradient)
+    const bool fuse_pack = d_out && ((uintptr_t)feat & 7u) == 0;
+    if (fuse_pack) {
+        conv(c, cin_feat(feat), N, H, W, L[L_E0A].wf, B(L_E0A), 32, true, a.e0a);
+    } else {
+        if (((uintptr_t)feat & 7u) == 0)
+            hipLaunchKernelGGL(k_pack_input<true>, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
+        else
+            hipLaunchKernelGGL(k_pack_input<false>, dim3(grid1(n0)), dim3(256), 0, s, feat, n0, a.x);
+        conv(c, cin1(a.x, kInPadded), N, H, W, L[L_E0A].wf, B(L_E0A), 32, true, a.e0a);
+    }
+    // the encoder's second convolutions also write their 2x2-pooled output
+    const HeadArgs nohead{nullptr, nullptr, nullptr, nullptr};
+    conv(c, cin1(a.e0a, 32), N, H, W, L[L_E0B].wf, B(L_E0B), 32, true, a.e0b, nullptr, nohead, a.p0);
     conv(c, cin1(a.p0, 32), N, H1, W1, L[L_E1A].wf, B(L_E1A), 64, true, a.e1a);
-    conv(c, cin1(a.e1a, 64), N, H1, W1, L[L_E1B].wf, B(L_E1B), 64, true, a.e1b);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H2 * W2 * 8)), dim3(256), 0, s, a.e1b, N, H1, W1, 64, a.p1);
+    conv(c, cin1(a.e1a, 64), N, H1, W1, L[L_E1B].wf, B(L_E1B), 64, true, a.e1b, nullptr, nohead, a.p1);
     conv(c, cin1(a.p1, 64), N, H2, W2, L[L_E2A].wf, B(L_E2A), 96, true, a.e2a);
-    conv(c, cin1(a.e2a, 96), N, H2, W2, L[L_E2B].wf, B(L_E2B), 96, true, a.e2b);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1((long long)N * H3 * W3 * 12)), dim3(256), 0, s, a.e2b, N, H2, W2, 96, a.p2);
+    conv(c, cin1(a.e2a, 96), N, H2, W2, L[L_E2B].wf, B(L_E2B), 96, true, a.e2b, nullptr, nohead, a.p2);
     conv(c, cin1(a.p2, 96), N, H3, W3, L[L_B0].wf, B(L_B0), 128, true, a.b0);
     conv(c, cin1(a.b0, 128), N, H3, W3, L[L_B1].wf, B(L_B1), 128, true, a.b1);
     conv(c, cin_cat(a.b1, 128, a.e2b, 96), N, H2, W2, L[L_D2A].wf, B(L_D2A), 96, true, a.d2a);

@@ -145,3 +145,28 @@ def test_batched_inference_equals_single_images(ctx):
     both = net.infer(np.concatenate([a, b]))
     assert np.array_equal(both[0:1].view(np.uint32), net.infer(a).view(np.uint32))
     assert np.array_equal(both[1:2].view(np.uint32), net.infer(b).view(np.uint32))
+
+
+def test_inference_first_layer_packing_equals_packing_kernel(ctx):
+    """Inference on 8-byte aligned features packs them inside the first
+    convolution's staging; on features 4 bytes off that alignment the packing
+    kernel runs first.  Both give the same output bit for bit (log1p of
+    radiance and depth, bf16 rounding, zero padding and the image border)."""
+    import salty_amd
+    net = salty_amd.Denoiser(ctx, seed=8)
+    feat, _ = case(13, n=2, h=40, w=48)
+    feat[0, 3:7, 5:30, 9] = 0.0   # a few miss pixels
+    n, h, w, _ = feat.shape
+    d_f = ctx.malloc(feat.nbytes + 16)
+    d_o = ctx.malloc(n * h * w * 3 * 4)
+    try:
+        outs = []
+        for off in (0, 4):
+            ctx.copy_to_device(d_f + off, feat)
+            net.infer_device(n, h, w, d_f + off, d_o)
+            outs.append(ctx.download(d_o, (n, h, w, 3), np.float32))
+        assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+        assert np.isfinite(outs[0]).all() and outs[0].max() > 0
+    finally:
+        ctx.free(d_f)
+        ctx.free(d_o)
