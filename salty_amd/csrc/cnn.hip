@@ -15,6 +15,7 @@
 #include <cstring>
 #include <new>
 #include <random>
+#include <type_traits>
 #include <vector>
 
 #include "internal.h"
@@ -158,16 +159,18 @@ struct HeadArgs {
     float *out;          // [pixels][3], or null: no head
 };
 
-// Epilogue of a convolution: bias + ReLU (+ 2x2 pooling), the fused head, or
-// the data gradient's ReLU mask; one kernel instantiation each, so that each
-// carries only its own registers.
-enum { EPI_FORWARD = 0, EPI_HEAD = 1, EPI_MASK = 2 };
+// Epilogue of a convolution: bias + ReLU (+ 2x2 pooling), the fused head, the
+// data gradient's ReLU mask, or the plain data gradient; one kernel
+// instantiation each, so that each carries only its own registers.
+enum { EPI_FORWARD = 0, EPI_HEAD = 1, EPI_MASK = 2, EPI_LINEAR = 3 };
 
 template <int COT, int EPI, bool FEAT>
 __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                     const bf16 *__restrict__ wt, const float *__restrict__ bias,
-                                                    int32_t Cout, int32_t relu, const bf16 *__restrict__ mask,
+                                                    int32_t Cout, const bf16 *__restrict__ mask,
                                                     bf16 *__restrict__ out, HeadArgs head, bf16 *__restrict__ pool) {
+    // bias and ReLU in the forward pass (and the head), neither in the data gradient
+    constexpr bool kAct = EPI == EPI_FORWARD || EPI == EPI_HEAD;
     constexpr int ROWS = 1024 / COT;                 // tile rows: 32 (COT 32) or 16 (COT 64)
     constexpr int WR = ROWS / 8;                     // waves along the rows
     constexpr int kInElems = (ROWS + 2) * kHalo * kRow, kWElems = 9 * COT * kRow;
@@ -244,15 +247,27 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
         const int sh = up ? (I.s1 >> 1) : 0;   // upsampled input: coordinates halved
         const uint32_t hs = (uint32_t)(H + sh) >> sh, ws = (uint32_t)(W + sh) >> sh;
         typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
+        // addresses: one uniform base, the source pixel left of and above
+        // the tile origin's (it may lie before the tensor; nothing reads
+        // it), plus a 32-bit byte offset per chunk.  Tile origins are even,
+        // so a chunk's offset from the base is the same for every tile; a
+        // chunk outside the image reads the tile origin's pixel instead (in
+        // the image) and is zeroed when stored.
+        const int sy0 = it.y0 >> sh, sx0 = it.x0 >> sh;
+        // signed 64-bit throughout: the base pixel's row and column can be -1
+        const int64_t bpix = ((int64_t)it.n * (int64_t)hs + sy0 - 1) * (int64_t)ws + sx0 - 1;
+        const uint64_t hbase = (uint64_t)(uintptr_t)src + (uint64_t)((bpix * (int64_t)C + cs) * 2);
+        const char *hb = (const char *)(uintptr_t)hbase;
+        const uint32_t origin_off = ((ws + 1u) * C) * 2u;
 #pragma unroll
         for (int k = 0; k < (FEAT ? 0 : kHalfPer); ++k) {
             const int e = min((int)threadIdx.x + 256 * k, kHalfCh - 1);
             const int pix = e >> 2, q = e & 3;
-            const int y = it.y0 + pix / kHalo - 1, x = it.x0 + pix % kHalo - 1;
-            const bool inside = y >= 0 && y < H && x >= 0 && x < W;
-            const uint32_t sp = ((uint32_t)it.n * hs + ((uint32_t)(inside ? y : 0) >> sh)) * ws +
-                                ((uint32_t)(inside ? x : 0) >> sh);
-            hv[k] = *(gbf16x8 *)(src + (size_t)sp * C + cs + q * 8);
+            const int dy = pix / kHalo - 1, dx = pix % kHalo - 1;
+            const bool inside = (uint32_t)(it.y0 + dy) < (uint32_t)H && (uint32_t)(it.x0 + dx) < (uint32_t)W;
+            const uint32_t ry = (uint32_t)((dy >> sh) + 1), rx = (uint32_t)((dx >> sh) + 1);
+            const uint32_t off = (inside ? ((ry * ws + rx) * C) * 2u : origin_off) + (uint32_t)q * 16u;
+            hv[k] = *(const gbf16x8 *)(hb + off);
             hin |= (uint32_t)inside << k;
         }
 #pragma unroll
@@ -260,20 +275,18 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
             const int e = min((int)threadIdx.x + 256 * k, kWCh - 1);
             const int row = e >> 2, q = e & 3;
             const int tap = row / COT, co = it.co0 + row % COT;
-            wv[k] = *reinterpret_cast<const bf16x8 *>(wt + ((size_t)tap * Cout + min(co, Cout - 1)) * Cin + c0 + q * 8);
+            const uint32_t off = (((uint32_t)tap * (uint32_t)Cout + (uint32_t)min(co, Cout - 1)) * (uint32_t)Cin +
+                                  (uint32_t)q * 8u) * 2u;
+            wv[k] = *(const gbf16x8 *)((const char *)uniform_ptr(wt + c0) + off);
             win |= (uint32_t)(co < Cout) << k;
         }
     };
     if ((int)blockIdx.x < items) issue(item_of(blockIdx.x), 0);
     for (int itn = blockIdx.x; itn < items; itn += gridDim.x) {
         const Item it = item_of(itn);
+        // written first by the MFMAs of the first slice's taps (0, 0) to (2, 0),
+        // which add to zero
         f32x4 acc[8][2];
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-#pragma unroll
-            for (int t = 0; t < 2; ++t)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) acc[i][t][e] = 0.0f;
         for (int c0 = 0; c0 < Cin; c0 += 32) {
             if constexpr (FEAT) {
                 // k_pack_input's packing: log1p of radiance and depth, bf16,
@@ -319,8 +332,11 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
             }
             __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the taps
             lds_barrier();
-#pragma unroll 1
-            for (int dx = 0; dx < 3; ++dx) {
+            // the taps of column shift dx; FIRST: the item's first (dy = 0
+            // starts the sums from zero)
+            const auto taps = [&](int dx, auto first) {
+                constexpr bool FIRST = decltype(first)::value;
+                const f32x4 zacc = {0.0f, 0.0f, 0.0f, 0.0f};
                 // operands: halo rows rw0 .. rw0+9 shifted by dx (A: lane =
                 // pixel column, 8 channels), weights of taps (dy, dx) (B: lane
                 // = output channel, 8 channels)
@@ -346,10 +362,17 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                         for (int i = 0; i < 4; ++i)
 #pragma unroll
                             for (int t = 0; t < 2; ++t)
-                                acc[4 * h + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b[dy][t], a[i + dy],
-                                                                                           acc[4 * h + i][t], 0, 0, 0);
+                                acc[4 * h + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    b[dy][t], a[i + dy], (FIRST && dy == 0) ? zacc : acc[4 * h + i][t], 0, 0, 0);
                 }
+            };
+            int dx0 = 0;
+            if (c0 == 0) {
+                taps(0, std::true_type{});
+                dx0 = 1;
             }
+#pragma unroll 1
+            for (int dx = dx0; dx < 3; ++dx) taps(dx, std::false_type{});
             lds_barrier();
         }
         // Epilogue straight from the accumulators: with the weights as the
@@ -363,7 +386,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
             const int x = it.x0 + l16;
             float bv[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) bv[e] = (bias && co + e < Cout) ? bias[co + e] : 0.0f;
+            for (int e = 0; e < 8; ++e) bv[e] = (kAct && co + e < Cout) ? bias[co + e] : 0.0f;
             if (EPI == EPI_HEAD) {
                 // fused head as one GEMM per row on the matrix cores: the
                 // weights are the first operand with output k in row 4k (so
@@ -398,9 +421,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                     bf16x8 v;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
-                        float f = acc[i][e >> 2][e & 3] + bv[e];
-                        if (relu) f = fmaxf(f, 0.0f);
-                        v[e] = f2b(f);
+                        v[e] = f2b(fmaxf(acc[i][e >> 2][e & 3] + bv[e], 0.0f));
                     }
                     f32x4 hs = {0.0f, 0.0f, 0.0f, 0.0f};
                     hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi, v, hs, 0, 0, 0);
@@ -411,14 +432,24 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                     }
                 }
             } else if (x < W && co < Cout) {
+                // addresses: the tile origin's element (uniform) plus this
+                // lane's 32-bit offset (row rw0, column l16, its 8 channels)
+                // and a row stride (W <= 65536 keeps them in 32 bits)
+                typedef __attribute__((address_space(1))) bf16x8 gbf16x8w;
+                typedef __attribute__((address_space(1))) const bf16x8 gbf16x8r;
+                const size_t t0 = (((size_t)it.n * H + it.y0) * W + it.x0) * Cout;
+                const uint32_t lo = ((uint32_t)rw0 * (uint32_t)W + (uint32_t)l16) * (uint32_t)Cout + (uint32_t)co;
+                const uint32_t rs = (uint32_t)W * (uint32_t)Cout;
+                char *const ob = (char *)(out + t0);
                 // the ReLU mask of all 8 rows loaded before any is used (rows
                 // past the image read row H - 1 and are not stored)
                 bf16x8 m[8];
                 if (EPI == EPI_MASK) {
+                    const char *mb = (const char *)(mask + t0);
 #pragma unroll
                     for (int i = 0; i < 8; ++i) {
-                        const int y = min(it.y0 + rw0 + i, H - 1);
-                        m[i] = *reinterpret_cast<const bf16x8 *>(mask + (((size_t)it.n * H + y) * W + x) * Cout + co);
+                        const uint32_t r = (uint32_t)min(rw0 + i, H - 1 - it.y0) - (uint32_t)rw0;
+                        m[i] = *(const gbf16x8r *)(mb + (lo + r * rs) * 2u);
                     }
                 }
                 bf16x8 prev;
@@ -426,12 +457,11 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                 for (int i = 0; i < 8; ++i) {
                     const int y = it.y0 + rw0 + i;
                     if (y >= H) break;
-                    const size_t off = (((size_t)it.n * H + y) * W + x) * Cout + co;
                     bf16x8 v;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
                         float f = acc[i][e >> 2][e & 3] + bv[e];
-                        if (relu) f = fmaxf(f, 0.0f);
+                        if (kAct) f = fmaxf(f, 0.0f);
                         v[e] = f2b(f);
                     }
                     if (EPI == EPI_MASK) {
@@ -440,32 +470,33 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                         for (int e = 0; e < 8; ++e)
                             if (!(b2f(m[i][e]) > 0.0f)) v[e] = f2b(0.0f);
                     }
-                    *reinterpret_cast<bf16x8 *>(out + off) = v;
+                    *(gbf16x8w *)(ob + (lo + (uint32_t)i * rs) * 2u) = v;
                     if (EPI == EPI_FORWARD && pool) {
                         // 2x2 average pooling of the stored values, summed top
                         // left, top right, bottom left, bottom right, times
                         // 0.25: the tile's rows pair up within a lane (y0 and
-                        // rw0 are even), its columns with lane ^ 1 (x0 is even);
-                        // H and W are even, so both lanes of a pair are here
+                        // rw0 are even), its columns with lane ^ 1 (x0 is even,
+                        // DPP quad permutation 1, 0, 3, 2); H and W are even,
+                        // so both lanes of a pair are here
                         if (i & 1) {
-                            bf16x8 tr, br;
                             typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
                             const u32x4 pv = __builtin_bit_cast(u32x4, prev), cv = __builtin_bit_cast(u32x4, v);
                             u32x4 ptr, pbr;
 #pragma unroll
                             for (int j = 0; j < 4; ++j) {
-                                ptr[j] = (uint32_t)__shfl_xor((int)pv[j], 1);
-                                pbr[j] = (uint32_t)__shfl_xor((int)cv[j], 1);
+                                ptr[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)pv[j], 0xB1, 0xF, 0xF, false);
+                                pbr[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)cv[j], 0xB1, 0xF, 0xF, false);
                             }
-                            tr = __builtin_bit_cast(bf16x8, ptr);
-                            br = __builtin_bit_cast(bf16x8, pbr);
-                            if ((x & 1) == 0) {
+                            const bf16x8 tr = __builtin_bit_cast(bf16x8, ptr), br = __builtin_bit_cast(bf16x8, pbr);
+                            if ((l16 & 1) == 0) {
                                 bf16x8 o;
 #pragma unroll
                                 for (int e = 0; e < 8; ++e)
                                     o[e] = f2b(0.25f * (b2f(prev[e]) + b2f(tr[e]) + b2f(v[e]) + b2f(br[e])));
-                                const size_t po = (((size_t)it.n * (H / 2) + (y >> 1)) * (W / 2) + (x >> 1)) * Cout + co;
-                                *reinterpret_cast<bf16x8 *>(pool + po) = o;
+                                const size_t p0 = (((size_t)it.n * (H / 2) + (it.y0 >> 1)) * (W / 2) + (it.x0 >> 1)) * Cout;
+                                const uint32_t plo = ((uint32_t)((rw0 + i) >> 1) * (uint32_t)(W / 2) + (uint32_t)(l16 >> 1)) *
+                                                         (uint32_t)Cout + (uint32_t)co;
+                                *(gbf16x8w *)((char *)(pool + p0) + plo * 2u) = o;
                             }
                         }
                         prev = v;
@@ -1138,35 +1169,31 @@ void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, con
           bf16 *out, const bf16 *mask = nullptr, HeadArgs head = HeadArgs{nullptr, nullptr, nullptr, nullptr},
           bf16 *pool = nullptr) {
     hipStream_t s = c->ctx->stream;
-    const int r = relu ? 1 : 0;
     const long long wgs = 2LL * conv_cus(c);
     const long long tx = (W + kTile - 1) / kTile;
+    // the epilogue: forward (bias + ReLU, optional pooling), fused head,
+    // masked or plain data gradient (no bias, no ReLU)
+    const int epi = head.out ? EPI_HEAD : (mask ? EPI_MASK : (relu ? EPI_FORWARD : EPI_LINEAR));
+#define S3DG_CONV(COT_, EPI_, FEAT_)                                                                                   \
+    hipLaunchKernelGGL((k_conv3x3<COT_, EPI_, FEAT_>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, mask, out, \
+                       head, pool)
     // the fused head needs the 32-channel tiles (all of a pixel's channels in one wave)
-    if (Cout % 64 == 0 && !head.out) {
+    if (Cout % 64 == 0 && epi != EPI_HEAD) {
         const long long items = (long long)N * tx * ((H + 15) / 16) * (Cout / 64);
         const dim3 grid((unsigned)std::min(items, wgs));
-        if (mask)
-            hipLaunchKernelGGL((k_conv3x3<64, EPI_MASK, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
-                               out, head, pool);
-        else
-            hipLaunchKernelGGL((k_conv3x3<64, EPI_FORWARD, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
-                               out, head, pool);
+        if (epi == EPI_MASK) S3DG_CONV(64, EPI_MASK, false);
+        else if (epi == EPI_LINEAR) S3DG_CONV(64, EPI_LINEAR, false);
+        else S3DG_CONV(64, EPI_FORWARD, false);
     } else {
         const long long items = (long long)N * tx * ((H + 31) / 32) * ((Cout + 31) / 32);
         const dim3 grid((unsigned)std::min(items, wgs));
-        if (head.out)
-            hipLaunchKernelGGL((k_conv3x3<32, EPI_HEAD, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
-                               out, head, pool);
-        else if (mask)
-            hipLaunchKernelGGL((k_conv3x3<32, EPI_MASK, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r, mask,
-                               out, head, pool);
-        else if (I.feat)
-            hipLaunchKernelGGL((k_conv3x3<32, EPI_FORWARD, true>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r,
-                               mask, out, head, pool);
-        else
-            hipLaunchKernelGGL((k_conv3x3<32, EPI_FORWARD, false>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, r,
-                               mask, out, head, pool);
+        if (epi == EPI_HEAD) S3DG_CONV(32, EPI_HEAD, false);
+        else if (epi == EPI_MASK) S3DG_CONV(32, EPI_MASK, false);
+        else if (epi == EPI_LINEAR) S3DG_CONV(32, EPI_LINEAR, false);
+        else if (I.feat) S3DG_CONV(32, EPI_FORWARD, true);
+        else S3DG_CONV(32, EPI_FORWARD, false);
     }
+#undef S3DG_CONV
 }
 
 // Weight and bias gradients of one layer (Cin, Cout multiples of 32): one
@@ -1300,6 +1327,8 @@ int check_dims(s3dg_context *ctx, int n, int h, int w) {
     if (n <= 0 || h <= 0 || w <= 0 || h % 8 || w % 8)
         return set_error(ctx, S3DG_ERR_INVALID, "denoiser input: N, H, W > 0 and H, W multiples of 8");
     if ((long long)n * h * w > (1LL << 27)) return set_error(ctx, S3DG_ERR_UNSUPPORTED, "denoiser batch too large");
+    // the convolutions address a tile's rows with 32-bit offsets
+    if (w > 65536) return set_error(ctx, S3DG_ERR_UNSUPPORTED, "denoiser input wider than 65536 pixels");
     return S3DG_OK;
 }
 
