@@ -134,6 +134,61 @@ def make_dataset(ctx, seeds, width=256, height=256, levels=INPUT_LEVELS, spp_ref
     return Dataset(np.stack(noisy), np.stack(guides), np.stack(tgts), levels)
 
 
+def make_dataset_batched(ctx, seeds, width=256, height=256, levels=INPUT_LEVELS, spp_ref=4096, max_depth=8,
+                         group=64, log=None):
+    """make_dataset's pairs, rendered `group` scenes at a time as one scene
+    set: each level of the input ladder and the reference are one batched
+    render over the group's frames (a frame's sum is the one its scene alone
+    would give, so the data equal make_dataset's), the guides one call per
+    scene.  Small frames are launch-bound one at a time; batched, the
+    4,096-spp references of 2,000 scenes take minutes instead of half an
+    hour."""
+    import salty_amd
+    seeds = [int(s) for s in seeds]
+    n, npx = len(seeds), width * height
+    noisy = np.empty((n, len(levels), height, width, 3), np.float32)
+    guides = np.empty((n, height, width, 7), np.float32)
+    target = np.empty((n, height, width, 3), np.float32)
+    zero = np.zeros(npx * 3, np.float32)
+    for g0 in range(0, n, group):
+        gs = seeds[g0:g0 + group]
+        scs = [salty_amd.Scene(ctx, scenes.cornell_variant(s)) for s in gs]
+        sset = salty_amd.SceneSet(ctx, scs)
+        acc = [ctx.malloc(npx * 12) for _ in gs]
+        d_alb, d_nrm, d_dep = ctx.malloc(npx * 12), ctx.malloc(npx * 12), ctx.malloc(npx * 4)
+        try:
+            # the input ladder: passes 0 .. levels[-1] - 1 of one progressive render
+            for a in acc:
+                ctx.copy_to_device(a, zero)
+            done = 0
+            for li, spp in enumerate(levels):
+                sset.render_batch_device([(j, width, height, spp - done, s, done, acc[j]) for j, s in enumerate(gs)],
+                                         max_depth)
+                done = spp
+                for j in range(len(gs)):
+                    noisy[g0 + j, li] = ctx.download(acc[j], (height, width, 3), np.float32) / np.float32(spp)
+            # the reference: passes disjoint from the inputs'
+            for a in acc:
+                ctx.copy_to_device(a, zero)
+            sset.render_batch_device([(j, width, height, spp_ref, s, 1 << 20, acc[j]) for j, s in enumerate(gs)],
+                                     max_depth)
+            for j, (sc, s) in enumerate(zip(scs, gs)):
+                target[g0 + j] = ctx.download(acc[j], (height, width, 3), np.float32) / np.float32(spp_ref)
+                sc.guides_device(abi.RenderConfig(width, height, max_depth, 1, s, 0, 0, 0, 0), d_alb, d_nrm, d_dep)
+                guides[g0 + j, ..., 0:3] = ctx.download(d_alb, (height, width, 3), np.float32)
+                guides[g0 + j, ..., 3:6] = ctx.download(d_nrm, (height, width, 3), np.float32)
+                guides[g0 + j, ..., 6:7] = ctx.download(d_dep, (height, width, 1), np.float32)
+        finally:
+            for a in acc + [d_alb, d_nrm, d_dep]:
+                ctx.free(a)
+            sset.close()
+            for sc in scs:
+                sc.close()
+        if log:
+            log(f"dataset {min(g0 + group, n)}/{n}")
+    return Dataset(noisy, guides, target, levels)
+
+
 def train(net, ctx, data: Dataset, steps=3000, batch=16, lr=1e-3, crop=None, seed=0, log=None):
     """Adam with a cosine learning-rate schedule down to 10 %.  The dataset is
     held in device memory (through torch, which only moves and indexes it);

@@ -1,13 +1,15 @@
 """Trains the U-Net denoiser on path-traced Cornell-style scenes and reports
 the time-to-relMSE comparison on held-out scenes.
 
-usage: python -m salty_amd.train_denoiser [--train-scenes N] [--steps S] [--out PATH]
+usage: python -m salty_amd.train_denoiser [--train-scenes N] [--spp-ref S] [--steps S] [--out PATH]
 
 Training scenes use seeds 1000.., held-out scenes seeds 0..(held-1) (seed 0 is
 the canonical 36-triangle Cornell box).  Inputs are the means after 1, 2, 4,
 ..., 256 samples per pixel of one progressive render; the reference is an
-independent spp-ref render.  Prints progress lines and one final JSON line;
-writes the weights to --out (default salty_amd/weights/unet_cornell.npz).
+independent spp-ref render (2,000 scenes with 4,096-spp references by
+default, rendered 64 scenes at a time as scene sets).  Prints progress lines
+and one final JSON line; writes the weights to --out (default
+salty_amd/weights/unet_cornell.npz).
 """
 from __future__ import annotations
 
@@ -21,9 +23,11 @@ import numpy as np
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train-scenes", type=int, default=1024)
+    ap.add_argument("--train-scenes", type=int, default=2000)
     ap.add_argument("--held-out", type=int, default=4)
-    ap.add_argument("--spp-ref", type=int, default=1024)
+    ap.add_argument("--spp-ref", type=int, default=4096)
+    ap.add_argument("--per-scene", action="store_true",
+                    help="render the training pairs scene by scene instead of 64 scenes per batch")
     ap.add_argument("--steps", type=int, default=60000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--crop", type=int, default=0, help="random crop side (0: whole frames)")
@@ -45,8 +49,8 @@ def main():
 
     ctx = salty_amd.Context(0)
     t = time.perf_counter()
-    data = denoise.make_dataset(ctx, range(1000, 1000 + args.train_scenes), args.size, args.size,
-                                spp_ref=args.spp_ref, log=log)
+    make = denoise.make_dataset if args.per_scene else denoise.make_dataset_batched
+    data = make(ctx, range(1000, 1000 + args.train_scenes), args.size, args.size, spp_ref=args.spp_ref, log=log)
     data_s = time.perf_counter() - t
     log(f"dataset: {len(data)} scenes in {data_s:.1f}s")
     net = salty_amd.Denoiser(ctx, seed=0)

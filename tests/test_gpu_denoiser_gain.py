@@ -1,7 +1,7 @@
 """What the trained network gains over the methods without it.
 
 The shipped weights (salty_amd/weights/unet_cornell.npz, trained on seeds
-1000-2023) on held-out Cornell-style scenes (seed 0, the canonical box, and
+1000-2999) on held-out Cornell-style scenes (seed 0, the canonical box, and
 seeds 4 and 5), rendered at 256x256, depth 8: from a 4-spp input plus the
 guide buffers the U-Net reaches relMSE <= 2e-3 against a 65,536-spp ground
 truth, below NLM on the same input at its best strength (FilterNLM's weights,
