@@ -180,12 +180,14 @@ __device__ __forceinline__ const Seg &find_seg(const Seg *__restrict__ segs, uin
     return segs[lo];
 }
 
-// Paths traced together when the caller leaves passes_in_flight at 0 (about
-// 16 GB of path state: all 64 passes of a 1080p frame at once, which keeps the
-// deep-bounce launches full), and the most ever held at once.  The automatic
-// choice also stays below a quarter of the free device memory.
-constexpr uint32_t kAutoPathsInFlight = 1u << 27;
-constexpr uint32_t kMaxPathsInFlight = 1u << 28;
+// Paths traced together when the caller leaves passes_in_flight at 0, and the
+// most ever held at once: 2^30 paths are 155 GB of path state, so that a
+// batch of many frames runs in few, large batches (the 64-scene
+// configuration's 1.93 G paths in two; each batch ends with a thin tail of
+// deep bounces).  The automatic choice also stays below half of the free
+// device memory.
+constexpr uint32_t kAutoPathsInFlight = 1u << 30;
+constexpr uint32_t kMaxPathsInFlight = 1u << 30;
 
 // Appends of a workgroup to two queues, staged in LDS over several loop
 // iterations: a wave reserves its entries in the stage with one LDS atomic,
@@ -807,7 +809,7 @@ int render_jobs(s3dg_context *ctx, const SceneDev &S, Workspace &ws, const Rende
     if (!passes_in_flight) {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
-            cap = std::min<uint64_t>(cap, (free_b / 4 + ws.render_bytes) / kPathBytes);
+            cap = std::min<uint64_t>(cap, (free_b / 2 + ws.render_bytes) / kPathBytes);
     }
     cap = std::min<uint64_t>(cap, kMaxPathsInFlight);
     cap = std::min<uint64_t>(cap, std::max<uint64_t>(total_paths, 1));
