@@ -251,11 +251,17 @@ int main() {
         printf("scene_grid %d", (int)grid.size());
         for (const Vertex &v : grid) p3(v.Position);
         printf("\n");
+        std::vector<RaySet> sets;
+        std::vector<HitRecord> singles;
+        std::vector<int> single_hit;
         for (int i = 0; i < 200; ++i) {
             const f32 x = uni(-0.5f, 0.5f), y = uni(-0.5f, 0.5f);
             const Ray r = scene.GetRay(x, y);
             HitRecord rec;
             const bool hit = scene.Intersect(MakeRaySet(r.pos, r.dir), rec);
+            sets.push_back(MakeRaySet(r.pos, r.dir));
+            singles.push_back(rec);
+            single_hit.push_back(hit ? 1 : 0);
             int kind = -1;
             if (hit) kind = rec.pShape == lamp ? 1 : 0;
             Vector3 nrm;
@@ -267,6 +273,22 @@ int main() {
             printf(" %d %.9g %d", kind, hit ? rec.distance : 0.0f, hit && rec.pMaterial == (kind ? glow : white));
             p3(nrm);
             printf("\n");
+        }
+        // the drop-in's batched overload (an extension, not a reference
+        // signature): the same 200 rays in one trace give the same records
+        {
+            std::vector<HitRecord> batch(sets.size());
+            const size_t hits = scene.Intersect(sets.data(), batch.data(), sets.size());
+            int equal = 0, nhit = 0;
+            for (size_t i = 0; i < sets.size(); ++i) {
+                const HitRecord &a = singles[i], &b = batch[i];
+                nhit += single_hit[i];
+                equal += !single_hit[i] ? (b.pShape == nullptr)
+                                        : (a.distance == b.distance && a.barycentric.x == b.barycentric.x &&
+                                           a.barycentric.y == b.barycentric.y && a.pShape == b.pShape &&
+                                           a.pMaterial == b.pMaterial);
+            }
+            printf("scene_batch %d %d %d %d\n", equal, (int)sets.size(), (int)hits, nhit);
         }
         const Color3 env = scene.SampleIBL(Vector3(0, 1, 0));
         PCG lr(5);

@@ -1135,17 +1135,53 @@ public:
     Ray GetRay(f32 x, f32 y) { return m_pCamera->GetRay(x, y); }
 
     // Scene::Intersect (s3d_scene.h:63-75) with the reference's
-    // [F_HIT_MIN, F_HIT_MAX] and the record's distance as the far end.  Batch
-    // callers should use s3dg_trace on GetDeviceScene().
+    // [F_HIT_MIN, F_HIT_MAX] and the record's distance as the far end: one
+    // s3dg_trace of one ray (one host round trip).
     bool Intersect(const RaySet &raySet, HitRecord &record) {
         std::lock_guard<std::recursive_mutex> lock(detail::Lock());
         if (!Build()) return false;
-        const Ray &ray = raySet.ray;
-        s3dg_ray r{{ray.pos.x, ray.pos.y, ray.pos.z}, F_HIT_MIN, {ray.dir.x, ray.dir.y, ray.dir.z},
-                   detail::TriangleTmax(record.distance)};
+        const s3dg_ray r = TraceRay(raySet, record);
         s3dg_hit h;
         int32_t shape = -1;
         if (s3dg_trace(m_Scene, &r, 1, S3DG_TRACE_CLOSEST, &h, &shape) != S3DG_OK || h.prim == -1) return false;
+        Fill(h, shape, record);
+        return true;
+    }
+    bool Intersect(const Ray &ray, HitRecord &record) { return Intersect(MakeRaySet(ray.pos, ray.dir), record); }
+
+    // Many rays in one s3dg_trace (no counterpart in the reference, whose
+    // Intersect takes one ray; for callers with batches, which would
+    // otherwise pay a host round trip per ray): records[i] ends as
+    // Intersect(raySets[i], records[i]) would leave it.  Returns the number
+    // of hits, or 0 with the records untouched when the scene cannot be
+    // built or traced.
+    size_t Intersect(const RaySet *raySets, HitRecord *records, size_t count) {
+        std::lock_guard<std::recursive_mutex> lock(detail::Lock());
+        if (!count || !Build() || count > 0xffffffffu) return 0;
+        std::vector<s3dg_ray> r(count);
+        std::vector<s3dg_hit> h(count);
+        std::vector<int32_t> shape(count, -1);
+        for (size_t i = 0; i < count; ++i) r[i] = TraceRay(raySets[i], records[i]);
+        if (s3dg_trace(m_Scene, r.data(), (uint32_t)count, S3DG_TRACE_CLOSEST, h.data(), shape.data()) != S3DG_OK)
+            return 0;
+        size_t hits = 0;
+        for (size_t i = 0; i < count; ++i) {
+            if (h[i].prim == -1) continue;
+            Fill(h[i], shape[i], records[i]);
+            ++hits;
+        }
+        return hits;
+    }
+
+private:
+    static s3dg_ray TraceRay(const RaySet &raySet, const HitRecord &record) {
+        const Ray &ray = raySet.ray;
+        return s3dg_ray{{ray.pos.x, ray.pos.y, ray.pos.z}, F_HIT_MIN, {ray.dir.x, ray.dir.y, ray.dir.z},
+                        detail::TriangleTmax(record.distance)};
+    }
+    // The record of a hit: distance, barycentrics, the hit shape (the
+    // triangle's view for meshes) and its material.
+    void Fill(const s3dg_hit &h, int32_t shape, HitRecord &record) const {
         record.distance = h.t;
         record.barycentric = Vector2(h.u, h.v);
         record.shape = shape;
@@ -1163,9 +1199,9 @@ public:
             record.pShape = m->View((u32)h.prim);
             record.pMaterial = m->TriangleMaterial((u32)h.prim);
         }
-        return true;
     }
-    bool Intersect(const Ray &ray, HitRecord &record) { return Intersect(MakeRaySet(ray.pos, ray.dir), record); }
+
+public:
 
     // Scene::SampleIBL (s3d_scene.h:80-82)
     Color3 SampleIBL(const Vector3 &dir) {

@@ -255,3 +255,10 @@ def test_tonemap_and_nlm_signatures(lines):
     src = np.concatenate([hdr * np.float32(0.25), np.ones((16, 24, 1), np.float32)], -1)
     got = f32(lines["nlm"][0]).reshape(16, 24, 4)
     assert np.allclose(got, O.nlm(src, 0.2, 6, 2), rtol=1e-4, atol=1e-6)
+
+
+def test_scene_batched_intersect_equals_single_rays(lines):
+    """The drop-in's batched Scene::Intersect (one trace for many rays) leaves
+    every record as the single-ray Intersect does."""
+    equal, n, hits, single_hits = (int(x) for x in lines["scene_batch"][0])
+    assert n == 200 and equal == n and hits == single_hits and hits > 0
