@@ -31,7 +31,8 @@
 // s3dg_camera_rays, s3dg_sphere_sample, s3dg_scene_calc_param and
 // s3dg_scene_environment.  These per-call paths carry one host round trip
 // each; whole frames go through Scene + PathTracer (s3dg_render), and batches
-// of rays through s3dg_trace on Scene::GetDeviceScene().
+// of rays through the batched Scene::Intersect overload or s3dg_trace on
+// Scene::GetDeviceScene().
 //
 // All device work uses one process-wide context (DefaultContext), and the
 // calls are serialised by one mutex, so the const methods stay reentrant as
