@@ -35,6 +35,17 @@ constexpr int kInPadded = 32;       // first layer input padded to one MFMA K st
 constexpr int kOut = 3;
 
 __device__ __forceinline__ float b2f(bf16 x) { return (float)x; }
+
+// log(1 + x) for the input features (x >= 0), by Goldberg's method: with
+// y = 1 + x rounded, log(y) * x / (y - 1) cancels the rounding of y (a few
+// ulps, against the library log1pf's ~25 instructions; the packing kernel
+// and the first convolution's own packing both use it, so they agree bit for
+// bit, and bf16 rounding of the result leaves a few ulps of FP32 unseen).
+__device__ __forceinline__ float log1p_feature(float x) {
+    const float y = 1.0f + x;
+    const float d = y - 1.0f;
+    return d == 0.0f ? x : __logf(y) * __fdividef(x, d);
+}
 __device__ __forceinline__ bf16 f2b(float x) { return (bf16)x; }
 
 // ---------------------------------------------------------------------------
@@ -301,7 +312,7 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
                             float v = 0.0f;
                             if (c < kInChannels) {
                                 v = fv[k][c];
-                                if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
+                                if (c < 3 || c == 9) v = log1p_feature(fmaxf(v, 0.0f));
                             }
                             o[c / 8][c % 8] = ((hin >> k) & 1u) ? f2b(v) : f2b(0.0f);
                         }
@@ -826,7 +837,7 @@ __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *
             float v = 0.0f;
             if (c < kInChannels) {
                 v = in[c];
-                if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.0f));
+                if (c < 3 || c == 9) v = log1p_feature(fmaxf(v, 0.0f));
             }
             o[c / 8][c % 8] = f2b(v);
         }
