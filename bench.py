@@ -32,7 +32,16 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 NODE_BYTES = 128               # compressed 8-wide node, one line
 TRI_BYTES = 36                 # v0, e0, e1 as Triangle::IsHit reads them
-EXT_IO_BYTES = 4 + 24 + 20     # queue index, ray origin+direction, hit record
+HIT_BYTES = 20                 # hit record written per extension ray
+RAY_IN_BYTES = 4 + 24          # queue index, ray origin+direction (read from the
+                               # second bounce on; the first bounce computes its
+                               # camera rays from the path slot)
+
+
+def ext_io_bytes(extension_rays, paths):
+    """Average bytes per extension ray besides nodes and triangles."""
+    later = max(extension_rays - paths, 0)
+    return HIT_BYTES + RAY_IN_BYTES * later / max(extension_rays, 1)
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_extend_mesh_scene_hbm_traffic.md")
@@ -516,7 +525,7 @@ def batch_run(args, steps, warmup):
     kst = sset.render_batch_device(frames, depth, flags=abi.RENDER_TIME_KERNELS)
     launches = kst.extend_launches
     avg_ext_ms = kst.extend_ms / max(launches, 1)
-    bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + EXT_IO_BYTES
+    bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + ext_io_bytes(kst.extension_rays, kst.paths)
     achieved = (bytes_per_ray * kst.extension_rays / max(launches, 1) / (avg_ext_ms * 1e-3) / 1e9
                 if avg_ext_ms else 0.0)
     line = {
@@ -647,7 +656,7 @@ def main():
     default_workload = (W, H, args.spp, args.depth, args.seed) == (1920, 1080, 64, 12, 0)
     traffic = extend_traffic_bytes() if default_workload else None
     avg_ext_ms = kst.extend_ms / max(launches, 1)
-    bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + EXT_IO_BYTES
+    bytes_per_ray = ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES + ext_io_bytes(kst.extension_rays, kst.paths)
     bytes_per_launch = bytes_per_ray * kst.extension_rays / max(launches, 1)
     achieved = bytes_per_launch / (avg_ext_ms * 1e-3) / 1e9 if avg_ext_ms > 0 else 0.0
 

@@ -374,9 +374,10 @@ int s3dg_render_guides_device(s3dg_scene *scene, const s3dg_render_config *cfg,
 
 /* ---------------------------------------------------------------------------
  * Tuning of the traversal and render kernels (no reference counterpart), per
- * context.  s3dg_autotune times trial renders of `cfg` (at most 16 passes of
- * it) on the scene for the candidate settings and keeps the fastest in the
- * context; results do not depend on the tuning, only the speed does.
+ * context.  s3dg_autotune times trial renders of `cfg` (its samples reduced
+ * only beyond 2^27 paths) on the scene for the candidate settings and keeps
+ * the fastest in the context; results do not depend on the tuning, only the
+ * speed does.
  * ------------------------------------------------------------------------- */
 typedef struct s3dg_tuning {
     uint32_t traversal_variant;  /* 0: 16 LDS stack entries, 5 waves/SIMD; 1: 12, 6; 2: 8, 6 */
@@ -388,8 +389,9 @@ typedef struct s3dg_tuning {
                                     created afterwards (1..8); 0: traverse the reference-shaped
                                     tree.  Tied hits are ranked by the reference tree either way */
     uint32_t tail_paths;         /* the wavefront's remaining bounces run thread per path in one
-                                    launch once its queue holds at most this many paths (capped
-                                    at the traversal grid's threads; 0: never)               */
+                                    launch once its queue holds at most this many paths
+                                    (0xFFFFFFFF, the default: two per thread of the traversal
+                                    grid; 0: never)                                           */
 } s3dg_tuning;
 
 int s3dg_get_tuning(s3dg_context *ctx, s3dg_tuning *out);

@@ -64,7 +64,7 @@ def test_tail_launch_changes_nothing(ctx, rr_start):
     sc = salty_amd.Scene(ctx, desc)
     try:
         outs = []
-        for tail in (0, 50, 2000, 1 << 30):
+        for tail in (0, 50, 2000, 0xFFFFFFFF, 1 << 30):
             t = ctx.get_tuning()
             t.tail_paths = tail
             ctx.set_tuning(t)
