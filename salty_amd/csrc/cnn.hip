@@ -141,7 +141,10 @@ __device__ __forceinline__ float load1(const ConvIn &I, int n, int y, int x, int
 // rows, the 6 input rows it needs, and runs the 3 x 4 x 2 MFMAs they make
 // (each input row serves up to three taps): 18 LDS reads per 48 MFMAs.  The
 // grid is persistent and each step's global loads (one tile and 32-channel
-// slice) are in flight while the previous step computes.  Used for the forward pass
+// slice) are in flight while the previous step computes; with one slice and
+// one output-channel block (SINGLE) the weights are staged once per
+// workgroup and the halos of the next two tiles are in flight, in two
+// register stages used in turn.  Used for the forward pass
 // (weights [tap][cout][cin]) and for the data gradient (flipped transposed
 // weights, dY as input; with `mask`, the result passes the ReLU whose output
 // mask is).  The weights are the MFMA's first operand, so each lane ends with
@@ -175,7 +178,7 @@ struct HeadArgs {
 // instantiation each, so that each carries only its own registers.
 enum { EPI_FORWARD = 0, EPI_HEAD = 1, EPI_MASK = 2, EPI_LINEAR = 3 };
 
-template <int COT, int EPI, bool FEAT>
+template <int COT, int EPI, bool FEAT, bool SINGLE>
 __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t H, int32_t W,
                                                     const bf16 *__restrict__ wt, const float *__restrict__ bias,
                                                     int32_t Cout, const bf16 *__restrict__ mask,
@@ -214,20 +217,39 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
         return r;
     };
     // staging: all global loads of a step (one item, one 32-channel slice)
-    // are in flight while the previous step computes
+    // are in flight while the previous step computes; SINGLE (one slice, one
+    // output-channel block: the weights never change) keeps the weights in
+    // LDS for the whole launch and two items' halo loads in flight
     constexpr int kHalfCh = (ROWS + 2) * kHalo * 4, kHalfPer = (kHalfCh + 255) / 256;
     constexpr int kWCh = 9 * COT * 4, kWPer = (kWCh + 255) / 256;
     // FEAT: one halo pixel (10 floats) per thread and round instead
     constexpr int kFeatPix = (ROWS + 2) * kHalo, kFeatPer = (kFeatPix + 255) / 256;
-    bf16x8 hv[FEAT ? 1 : kHalfPer], wv[kWPer];
-    float fv[FEAT ? kFeatPer : 1][kInChannels];
-    uint32_t hin = 0, win = 0;   // chunks whose value is real (inside the image / an existing channel)
-    const auto issue = [&](const Item &it, int c0) {
-        // every chunk load is issued unconditionally and raw (the last
-        // round's spare threads load the last chunk again and do not store
-        // it; out-of-range chunks are zeroed when stored), so that nothing
-        // between the loads and the next step's LDS stores waits for them
-        hin = win = 0;
+    struct Stage {
+        bf16x8 hv[FEAT ? 1 : kHalfPer];
+        float fv[FEAT ? kFeatPer : 1][kInChannels];
+        bf16x8 wv[kWPer];
+        uint32_t hin, win;   // chunks whose value is real (inside the image / an existing channel)
+    };
+    // every chunk load is issued unconditionally and raw (the last round's
+    // spare threads load the last chunk again and do not store it;
+    // out-of-range chunks are zeroed when stored), so that nothing between
+    // the loads and the next step's LDS stores waits for them
+    const auto issue_weights = [&](Stage &sg, const Item &it, int c0) {
+        typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
+        sg.win = 0;
+#pragma unroll
+        for (int k = 0; k < kWPer; ++k) {
+            const int e = min((int)threadIdx.x + 256 * k, kWCh - 1);
+            const int row = e >> 2, q = e & 3;
+            const int tap = row / COT, co = it.co0 + row % COT;
+            const uint32_t off = (((uint32_t)tap * (uint32_t)Cout + (uint32_t)min(co, Cout - 1)) * (uint32_t)Cin +
+                                  (uint32_t)q * 8u) * 2u;
+            sg.wv[k] = *(const gbf16x8 *)((const char *)uniform_ptr(wt + c0) + off);
+            sg.win |= (uint32_t)(co < Cout) << k;
+        }
+    };
+    const auto issue_halo = [&](Stage &sg, const Item &it, int c0) {
+        sg.hin = 0;
         if constexpr (FEAT) {
             typedef float f32x2 __attribute__((ext_vector_type(2)));
             typedef __attribute__((address_space(1))) const f32x2 gfloat2;
@@ -243,277 +265,316 @@ __global__ __launch_bounds__(256, 2) void k_conv3x3(ConvIn I, int32_t N, int32_t
 #pragma unroll
                 for (int j = 0; j < kInChannels / 2; ++j) {
                     const f32x2 v = src[j];
-                    fv[k][2 * j] = v.x;
-                    fv[k][2 * j + 1] = v.y;
+                    sg.fv[k][2 * j] = v.x;
+                    sg.fv[k][2 * j + 1] = v.y;
                 }
-                hin |= (uint32_t)inside << k;
+                sg.hin |= (uint32_t)inside << k;
             }
-        }
-        // the slice lies in one source (C1 is a multiple of 32), chosen once
-        // per step; pixel indices in 32 bits (N*H*W <= 2^27)
-        const bool up = c0 < I.C1;
-        const bf16 *src = up ? uniform_ptr(I.in1) : uniform_ptr(I.in2);
-        const uint32_t C = (uint32_t)(up ? I.C1 : I.C2);
-        const int cs = up ? c0 : c0 - I.C1;
-        const int sh = up ? (I.s1 >> 1) : 0;   // upsampled input: coordinates halved
-        const uint32_t hs = (uint32_t)(H + sh) >> sh, ws = (uint32_t)(W + sh) >> sh;
-        typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
-        // addresses: one uniform base, the source pixel left of and above
-        // the tile origin's (it may lie before the tensor; nothing reads
-        // it), plus a 32-bit byte offset per chunk.  Tile origins are even,
-        // so a chunk's offset from the base is the same for every tile; a
-        // chunk outside the image reads the tile origin's pixel instead (in
-        // the image) and is zeroed when stored.
-        const int sy0 = it.y0 >> sh, sx0 = it.x0 >> sh;
-        // signed 64-bit throughout: the base pixel's row and column can be -1
-        const int64_t bpix = ((int64_t)it.n * (int64_t)hs + sy0 - 1) * (int64_t)ws + sx0 - 1;
-        const uint64_t hbase = (uint64_t)(uintptr_t)src + (uint64_t)((bpix * (int64_t)C + cs) * 2);
-        const char *hb = (const char *)(uintptr_t)hbase;
-        const uint32_t origin_off = ((ws + 1u) * C) * 2u;
+        } else {
+            // the slice lies in one source (C1 is a multiple of 32), chosen
+            // once per step; pixel indices in 32 bits (N*H*W <= 2^27)
+            const bool up = c0 < I.C1;
+            const bf16 *src = up ? uniform_ptr(I.in1) : uniform_ptr(I.in2);
+            const uint32_t C = (uint32_t)(up ? I.C1 : I.C2);
+            const int cs = up ? c0 : c0 - I.C1;
+            const int sh = up ? (I.s1 >> 1) : 0;   // upsampled input: coordinates halved
+            const uint32_t hs = (uint32_t)(H + sh) >> sh, ws = (uint32_t)(W + sh) >> sh;
+            typedef __attribute__((address_space(1))) const bf16x8 gbf16x8;
+            // addresses: one uniform base, the source pixel left of and above
+            // the tile origin's (it may lie before the tensor; nothing reads
+            // it), plus a 32-bit byte offset per chunk.  Tile origins are
+            // even, so a chunk's offset from the base is the same for every
+            // tile; a chunk outside the image reads the tile origin's pixel
+            // instead (in the image) and is zeroed when stored.
+            const int sy0 = it.y0 >> sh, sx0 = it.x0 >> sh;
+            // signed 64-bit throughout: the base pixel's row and column can be -1
+            const int64_t bpix = ((int64_t)it.n * (int64_t)hs + sy0 - 1) * (int64_t)ws + sx0 - 1;
+            const uint64_t hbase = (uint64_t)(uintptr_t)src + (uint64_t)((bpix * (int64_t)C + cs) * 2);
+            const char *hb = (const char *)(uintptr_t)hbase;
+            const uint32_t origin_off = ((ws + 1u) * C) * 2u;
 #pragma unroll
-        for (int k = 0; k < (FEAT ? 0 : kHalfPer); ++k) {
-            const int e = min((int)threadIdx.x + 256 * k, kHalfCh - 1);
-            const int pix = e >> 2, q = e & 3;
-            const int dy = pix / kHalo - 1, dx = pix % kHalo - 1;
-            const bool inside = (uint32_t)(it.y0 + dy) < (uint32_t)H && (uint32_t)(it.x0 + dx) < (uint32_t)W;
-            const uint32_t ry = (uint32_t)((dy >> sh) + 1), rx = (uint32_t)((dx >> sh) + 1);
-            const uint32_t off = (inside ? ((ry * ws + rx) * C) * 2u : origin_off) + (uint32_t)q * 16u;
-            hv[k] = *(const gbf16x8 *)(hb + off);
-            hin |= (uint32_t)inside << k;
-        }
-#pragma unroll
-        for (int k = 0; k < kWPer; ++k) {
-            const int e = min((int)threadIdx.x + 256 * k, kWCh - 1);
-            const int row = e >> 2, q = e & 3;
-            const int tap = row / COT, co = it.co0 + row % COT;
-            const uint32_t off = (((uint32_t)tap * (uint32_t)Cout + (uint32_t)min(co, Cout - 1)) * (uint32_t)Cin +
-                                  (uint32_t)q * 8u) * 2u;
-            wv[k] = *(const gbf16x8 *)((const char *)uniform_ptr(wt + c0) + off);
-            win |= (uint32_t)(co < Cout) << k;
+            for (int k = 0; k < kHalfPer; ++k) {
+                const int e = min((int)threadIdx.x + 256 * k, kHalfCh - 1);
+                const int pix = e >> 2, q = e & 3;
+                const int dy = pix / kHalo - 1, dx = pix % kHalo - 1;
+                const bool inside = (uint32_t)(it.y0 + dy) < (uint32_t)H && (uint32_t)(it.x0 + dx) < (uint32_t)W;
+                const uint32_t ry = (uint32_t)((dy >> sh) + 1), rx = (uint32_t)((dx >> sh) + 1);
+                const uint32_t off = (inside ? ((ry * ws + rx) * C) * 2u : origin_off) + (uint32_t)q * 16u;
+                sg.hv[k] = *(const gbf16x8 *)(hb + off);
+                sg.hin |= (uint32_t)inside << k;
+            }
         }
     };
-    if ((int)blockIdx.x < items) issue(item_of(blockIdx.x), 0);
-    for (int itn = blockIdx.x; itn < items; itn += gridDim.x) {
-        const Item it = item_of(itn);
-        // written first by the MFMAs of the first slice's taps (0, 0) to (2, 0),
-        // which add to zero
-        f32x4 acc[8][2];
-        for (int c0 = 0; c0 < Cin; c0 += 32) {
-            if constexpr (FEAT) {
-                // k_pack_input's packing: log1p of radiance and depth, bf16,
-                // zero padding to 32 channels (and outside the image)
+    const auto store_weights = [&](const Stage &sg) {
 #pragma unroll
-                for (int k = 0; k < kFeatPer; ++k) {
-                    const int pix = threadIdx.x + 256 * k;
-                    if (pix < kFeatPix) {
-                        bf16x8 o[kInPadded / 8];
+        for (int k = 0; k < kWPer; ++k) {
+            const int e = threadIdx.x + 256 * k;
+            if (e < kWCh)
+                *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = ((sg.win >> k) & 1u) ? sg.wv[k] : zero;
+        }
+    };
+    const auto store_halo = [&](const Stage &sg) {
+        if constexpr (FEAT) {
+            // k_pack_input's packing: log1p of radiance and depth, bf16,
+            // zero padding to 32 channels (and outside the image)
 #pragma unroll
-                        for (int c = 0; c < kInPadded; ++c) {
-                            float v = 0.0f;
-                            if (c < kInChannels) {
-                                v = fv[k][c];
-                                if (c < 3 || c == 9) v = log1p_feature(fmaxf(v, 0.0f));
-                            }
-                            o[c / 8][c % 8] = ((hin >> k) & 1u) ? f2b(v) : f2b(0.0f);
+            for (int k = 0; k < kFeatPer; ++k) {
+                const int pix = threadIdx.x + 256 * k;
+                if (pix < kFeatPix) {
+                    bf16x8 o[kInPadded / 8];
+#pragma unroll
+                    for (int c = 0; c < kInPadded; ++c) {
+                        float v = 0.0f;
+                        if (c < kInChannels) {
+                            v = sg.fv[k][c];
+                            if (c < 3 || c == 9) v = log1p_feature(fmaxf(v, 0.0f));
                         }
-#pragma unroll
-                        for (int q = 0; q < kInPadded / 8; ++q)
-                            *reinterpret_cast<bf16x8 *>(&s_in[pix * kRow + q * 8]) = o[q];
+                        o[c / 8][c % 8] = ((sg.hin >> k) & 1u) ? f2b(v) : f2b(0.0f);
                     }
+#pragma unroll
+                    for (int q = 0; q < kInPadded / 8; ++q)
+                        *reinterpret_cast<bf16x8 *>(&s_in[pix * kRow + q * 8]) = o[q];
                 }
             }
+        } else {
 #pragma unroll
-            for (int k = 0; k < (FEAT ? 0 : kHalfPer); ++k) {
+            for (int k = 0; k < kHalfPer; ++k) {
                 const int e = threadIdx.x + 256 * k;
                 if (e < kHalfCh)
-                    *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) = ((hin >> k) & 1u) ? hv[k] : zero;
+                    *reinterpret_cast<bf16x8 *>(&s_in[(e >> 2) * kRow + (e & 3) * 8]) =
+                        ((sg.hin >> k) & 1u) ? sg.hv[k] : zero;
             }
+        }
+    };
+    // written first by the MFMAs of the first slice's taps (0, 0) to (2, 0),
+    // which add to zero
+    f32x4 acc[8][2];
+    // the taps of one staged slice (the LDS barrier before them makes the
+    // stores visible, the one after frees the stage for the next stores)
+    const auto compute = [&](int c0) {
+        __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the taps
+        lds_barrier();
+        // the taps of column shift dx; FIRST: the item's first (dy = 0
+        // starts the sums from zero)
+        const auto taps = [&](int dx, auto first) {
+            constexpr bool FIRST = decltype(first)::value;
+            const f32x4 zacc = {0.0f, 0.0f, 0.0f, 0.0f};
+            // operands: halo rows rw0 .. rw0+9 shifted by dx (A: lane =
+            // pixel column, 8 channels), weights of taps (dy, dx) (B: lane
+            // = output channel, 8 channels)
+            bf16x8 b[3][2];
 #pragma unroll
-            for (int k = 0; k < kWPer; ++k) {
-                const int e = threadIdx.x + 256 * k;
-                if (e < kWCh)
-                    *reinterpret_cast<bf16x8 *>(&s_w[(e >> 2) * kRow + (e & 3) * 8]) = ((win >> k) & 1u) ? wv[k] : zero;
-            }
-            // the next step: the next slice of this item, or slice 0 of the
-            // next item
-            {
-                const bool more = c0 + 32 < Cin;
-                const int nitem = more ? itn : itn + (int)gridDim.x;
-                if (nitem < items) issue(item_of(nitem), more ? c0 + 32 : 0);
-            }
-            __builtin_amdgcn_sched_barrier(0);   // keep the loads ahead of the taps
-            lds_barrier();
-            // the taps of column shift dx; FIRST: the item's first (dy = 0
-            // starts the sums from zero)
-            const auto taps = [&](int dx, auto first) {
-                constexpr bool FIRST = decltype(first)::value;
-                const f32x4 zacc = {0.0f, 0.0f, 0.0f, 0.0f};
-                // operands: halo rows rw0 .. rw0+9 shifted by dx (A: lane =
-                // pixel column, 8 channels), weights of taps (dy, dx) (B: lane
-                // = output channel, 8 channels)
-                bf16x8 b[3][2];
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    b[dy][t] = *reinterpret_cast<const bf16x8 *>(
+                        &s_w[((dy * 3 + dx) * COT + cw0 + chan_of(t, l16)) * kRow + kq]);
+            // two halves of 4 output rows (6 input rows each), so that
+            // consecutive MFMAs go to 8 different accumulators
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                bf16x8 a[6];
+#pragma unroll
+                for (int i = 0; i < 6; ++i)
+                    a[i] = *reinterpret_cast<const bf16x8 *>(&s_in[((rw0 + 4 * h + i) * kHalo + l16 + dx) * kRow + kq]);
 #pragma unroll
                 for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-                    for (int t = 0; t < 2; ++t)
-                        b[dy][t] = *reinterpret_cast<const bf16x8 *>(
-                            &s_w[((dy * 3 + dx) * COT + cw0 + chan_of(t, l16)) * kRow + kq]);
-                // two halves of 4 output rows (6 input rows each), so that
-                // consecutive MFMAs go to 8 different accumulators
+                    for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    bf16x8 a[6];
-#pragma unroll
-                    for (int i = 0; i < 6; ++i)
-                        a[i] = *reinterpret_cast<const bf16x8 *>(
-                            &s_in[((rw0 + 4 * h + i) * kHalo + l16 + dx) * kRow + kq]);
-#pragma unroll
-                    for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-                        for (int i = 0; i < 4; ++i)
-#pragma unroll
-                            for (int t = 0; t < 2; ++t)
-                                acc[4 * h + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    b[dy][t], a[i + dy], (FIRST && dy == 0) ? zacc : acc[4 * h + i][t], 0, 0, 0);
-                }
-            };
-            int dx0 = 0;
-            if (c0 == 0) {
-                taps(0, std::true_type{});
-                dx0 = 1;
+                        for (int t = 0; t < 2; ++t)
+                            acc[4 * h + i][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                b[dy][t], a[i + dy], (FIRST && dy == 0) ? zacc : acc[4 * h + i][t], 0, 0, 0);
             }
-#pragma unroll 1
-            for (int dx = dx0; dx < 3; ++dx) taps(dx, std::false_type{});
-            lds_barrier();
+        };
+        int dx0 = 0;
+        if (c0 == 0) {
+            taps(0, std::true_type{});
+            dx0 = 1;
         }
-        // Epilogue straight from the accumulators: with the weights as the
-        // MFMA's first operand the result is D[channel][pixel]: lane l holds
-        // rows 4*(l/16) .. +3 of both channel tiles for pixel column l % 16,
-        // and with the rows mapped by chan_of those are channels
-        // 8*(l/16) .. +7, 16 contiguous bytes of the NHWC output; bias and
-        // ReLU applied, or the mask.
-        {
-            const int co = it.co0 + cw0 + 8 * (lane >> 4);
-            const int x = it.x0 + l16;
-            float bv[8];
+#pragma unroll 1
+        for (int dx = dx0; dx < 3; ++dx) taps(dx, std::false_type{});
+        lds_barrier();
+    };
+    // Epilogue straight from the accumulators: with the weights as the
+    // MFMA's first operand the result is D[channel][pixel]: lane l holds
+    // rows 4*(l/16) .. +3 of both channel tiles for pixel column l % 16,
+    // and with the rows mapped by chan_of those are channels
+    // 8*(l/16) .. +7, 16 contiguous bytes of the NHWC output; bias and
+    // ReLU applied, or the mask.
+    const auto epilogue = [&](const Item &it) {
+        const int co = it.co0 + cw0 + 8 * (lane >> 4);
+        const int x = it.x0 + l16;
+        float bv[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) bv[e] = (kAct && co + e < Cout) ? bias[co + e] : 0.0f;
-            if (EPI == EPI_HEAD) {
-                // fused head as one GEMM per row on the matrix cores: the
-                // weights are the first operand with output k in row 4k (so
-                // that lane group k ends with output k of its pixel column),
-                // split into bf16 high and low parts (two MFMAs, about 16
-                // bits of the FP32 weights); the second operand is this
-                // lane's 8 channels of the pixel, bf16 as stored
-                const int g = lane >> 4;
-                bf16x8 whi, wlo;
+        for (int e = 0; e < 8; ++e) bv[e] = (kAct && co + e < Cout) ? bias[co + e] : 0.0f;
+        if (EPI == EPI_HEAD) {
+            // fused head as one GEMM per row on the matrix cores: the
+            // weights are the first operand with output k in row 4k (so
+            // that lane group k ends with output k of its pixel column),
+            // split into bf16 high and low parts (two MFMAs, about 16
+            // bits of the FP32 weights); the second operand is this
+            // lane's 8 channels of the pixel, bf16 as stored
+            const int g = lane >> 4;
+            bf16x8 whi, wlo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float wv = (l16 & 3) == 0 && l16 < 4 * kOut ? head.w[(l16 >> 2) * 32 + 8 * g + e] : 0.0f;
+                whi[e] = f2b(wv);
+                wlo[e] = f2b(wv - b2f(whi[e]));
+            }
+            const float hb = g < kOut ? head.b[g] : 0.0f;
+            // the 8 rows' depth guides and input radiance loaded before
+            // any is used (rows past the image read row H - 1, columns
+            // past it column W - 1)
+            const int xc = min(x, W - 1);
+            float depth[8], rad[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const size_t p = ((size_t)it.n * H + min(it.y0 + rw0 + i, H - 1)) * W + xc;
+                depth[i] = head.feat[p * kInChannels + 9];
+                rad[i] = head.feat[p * kInChannels + min(g, kOut - 1)];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int y = it.y0 + rw0 + i;
+                if (y >= H) break;
+                bf16x8 v;
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
-                    const float wv = (l16 & 3) == 0 && l16 < 4 * kOut ? head.w[(l16 >> 2) * 32 + 8 * g + e] : 0.0f;
-                    whi[e] = f2b(wv);
-                    wlo[e] = f2b(wv - b2f(whi[e]));
+                    v[e] = f2b(fmaxf(acc[i][e >> 2][e & 3] + bv[e], 0.0f));
                 }
-                const float hb = g < kOut ? head.b[g] : 0.0f;
-                // the 8 rows' depth guides and input radiance loaded before
-                // any is used (rows past the image read row H - 1, columns
-                // past it column W - 1)
-                const int xc = min(x, W - 1);
-                float depth[8], rad[8];
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const size_t p = ((size_t)it.n * H + min(it.y0 + rw0 + i, H - 1)) * W + xc;
-                    depth[i] = head.feat[p * kInChannels + 9];
-                    rad[i] = head.feat[p * kInChannels + min(g, kOut - 1)];
-                }
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int y = it.y0 + rw0 + i;
-                    if (y >= H) break;
-                    bf16x8 v;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        v[e] = f2b(fmaxf(acc[i][e >> 2][e & 3] + bv[e], 0.0f));
-                    }
-                    f32x4 hs = {0.0f, 0.0f, 0.0f, 0.0f};
-                    hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi, v, hs, 0, 0, 0);
-                    hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlo, v, hs, 0, 0, 0);
-                    if (g < kOut && x < W) {
-                        const size_t p = ((size_t)it.n * H + y) * W + x;
-                        head.out[p * kOut + g] = depth[i] == 0.0f ? rad[i] : fmaxf(expm1f(hb + hs[0]), 0.0f);
-                    }
-                }
-            } else if (x < W && co < Cout) {
-                // addresses: the tile origin's element (uniform) plus this
-                // lane's 32-bit offset (row rw0, column l16, its 8 channels)
-                // and a row stride (W <= 65536 keeps them in 32 bits)
-                typedef __attribute__((address_space(1))) bf16x8 gbf16x8w;
-                typedef __attribute__((address_space(1))) const bf16x8 gbf16x8r;
-                const size_t t0 = (((size_t)it.n * H + it.y0) * W + it.x0) * Cout;
-                const uint32_t lo = ((uint32_t)rw0 * (uint32_t)W + (uint32_t)l16) * (uint32_t)Cout + (uint32_t)co;
-                const uint32_t rs = (uint32_t)W * (uint32_t)Cout;
-                char *const ob = (char *)(out + t0);
-                // the ReLU mask of all 8 rows loaded before any is used (rows
-                // past the image read row H - 1 and are not stored)
-                bf16x8 m[8];
-                if (EPI == EPI_MASK) {
-                    const char *mb = (const char *)(mask + t0);
-#pragma unroll
-                    for (int i = 0; i < 8; ++i) {
-                        const uint32_t r = (uint32_t)min(rw0 + i, H - 1 - it.y0) - (uint32_t)rw0;
-                        m[i] = *(const gbf16x8r *)(mb + (lo + r * rs) * 2u);
-                    }
-                }
-                bf16x8 prev;
-#pragma unroll
-                for (int i = 0; i < 8; ++i) {
-                    const int y = it.y0 + rw0 + i;
-                    if (y >= H) break;
-                    bf16x8 v;
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        float f = acc[i][e >> 2][e & 3] + bv[e];
-                        if (kAct) f = fmaxf(f, 0.0f);
-                        v[e] = f2b(f);
-                    }
-                    if (EPI == EPI_MASK) {
-                        // gradient through the ReLU whose output is `mask` (same shape)
-#pragma unroll
-                        for (int e = 0; e < 8; ++e)
-                            if (!(b2f(m[i][e]) > 0.0f)) v[e] = f2b(0.0f);
-                    }
-                    *(gbf16x8w *)(ob + (lo + (uint32_t)i * rs) * 2u) = v;
-                    if (EPI == EPI_FORWARD && pool) {
-                        // 2x2 average pooling of the stored values, summed top
-                        // left, top right, bottom left, bottom right, times
-                        // 0.25: the tile's rows pair up within a lane (y0 and
-                        // rw0 are even), its columns with lane ^ 1 (x0 is even,
-                        // DPP quad permutation 1, 0, 3, 2); H and W are even,
-                        // so both lanes of a pair are here
-                        if (i & 1) {
-                            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-                            const u32x4 pv = __builtin_bit_cast(u32x4, prev), cv = __builtin_bit_cast(u32x4, v);
-                            u32x4 ptr, pbr;
-#pragma unroll
-                            for (int j = 0; j < 4; ++j) {
-                                ptr[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)pv[j], 0xB1, 0xF, 0xF, false);
-                                pbr[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)cv[j], 0xB1, 0xF, 0xF, false);
-                            }
-                            const bf16x8 tr = __builtin_bit_cast(bf16x8, ptr), br = __builtin_bit_cast(bf16x8, pbr);
-                            if ((l16 & 1) == 0) {
-                                bf16x8 o;
-#pragma unroll
-                                for (int e = 0; e < 8; ++e)
-                                    o[e] = f2b(0.25f * (b2f(prev[e]) + b2f(tr[e]) + b2f(v[e]) + b2f(br[e])));
-                                const size_t p0 = (((size_t)it.n * (H / 2) + (it.y0 >> 1)) * (W / 2) + (it.x0 >> 1)) * Cout;
-                                const uint32_t plo = ((uint32_t)((rw0 + i) >> 1) * (uint32_t)(W / 2) + (uint32_t)(l16 >> 1)) *
-                                                         (uint32_t)Cout + (uint32_t)co;
-                                *(gbf16x8w *)((char *)(pool + p0) + plo * 2u) = o;
-                            }
-                        }
-                        prev = v;
-                    }
+                f32x4 hs = {0.0f, 0.0f, 0.0f, 0.0f};
+                hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(whi, v, hs, 0, 0, 0);
+                hs = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wlo, v, hs, 0, 0, 0);
+                if (g < kOut && x < W) {
+                    const size_t p = ((size_t)it.n * H + y) * W + x;
+                    head.out[p * kOut + g] = depth[i] == 0.0f ? rad[i] : fmaxf(expm1f(hb + hs[0]), 0.0f);
                 }
             }
+        } else if (x < W && co < Cout) {
+            // addresses: the tile origin's element (uniform) plus this
+            // lane's 32-bit offset (row rw0, column l16, its 8 channels)
+            // and a row stride (W <= 65536 keeps them in 32 bits)
+            typedef __attribute__((address_space(1))) bf16x8 gbf16x8w;
+            typedef __attribute__((address_space(1))) const bf16x8 gbf16x8r;
+            const size_t t0 = (((size_t)it.n * H + it.y0) * W + it.x0) * Cout;
+            const uint32_t lo = ((uint32_t)rw0 * (uint32_t)W + (uint32_t)l16) * (uint32_t)Cout + (uint32_t)co;
+            const uint32_t rs = (uint32_t)W * (uint32_t)Cout;
+            char *const ob = (char *)(out + t0);
+            // the ReLU mask of all 8 rows loaded before any is used (rows
+            // past the image read row H - 1 and are not stored)
+            bf16x8 m[8];
+            if (EPI == EPI_MASK) {
+                const char *mb = (const char *)(mask + t0);
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const uint32_t r = (uint32_t)min(rw0 + i, H - 1 - it.y0) - (uint32_t)rw0;
+                    m[i] = *(const gbf16x8r *)(mb + (lo + r * rs) * 2u);
+                }
+            }
+            bf16x8 prev;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int y = it.y0 + rw0 + i;
+                if (y >= H) break;
+                bf16x8 v;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    float f = acc[i][e >> 2][e & 3] + bv[e];
+                    if (kAct) f = fmaxf(f, 0.0f);
+                    v[e] = f2b(f);
+                }
+                if (EPI == EPI_MASK) {
+                    // gradient through the ReLU whose output is `mask` (same shape)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        if (!(b2f(m[i][e]) > 0.0f)) v[e] = f2b(0.0f);
+                }
+                *(gbf16x8w *)(ob + (lo + (uint32_t)i * rs) * 2u) = v;
+                if (EPI == EPI_FORWARD && pool) {
+                    // 2x2 average pooling of the stored values, summed top
+                    // left, top right, bottom left, bottom right, times
+                    // 0.25: the tile's rows pair up within a lane (y0 and
+                    // rw0 are even), its columns with lane ^ 1 (x0 is even,
+                    // DPP quad permutation 1, 0, 3, 2); H and W are even,
+                    // so both lanes of a pair are here
+                    if (i & 1) {
+                        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                        const u32x4 pv = __builtin_bit_cast(u32x4, prev), cv = __builtin_bit_cast(u32x4, v);
+                        u32x4 ptr, pbr;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            ptr[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)pv[j], 0xB1, 0xF, 0xF, false);
+                            pbr[j] = (uint32_t)__builtin_amdgcn_mov_dpp((int)cv[j], 0xB1, 0xF, 0xF, false);
+                        }
+                        const bf16x8 tr = __builtin_bit_cast(bf16x8, ptr), br = __builtin_bit_cast(bf16x8, pbr);
+                        if ((l16 & 1) == 0) {
+                            bf16x8 o;
+#pragma unroll
+                            for (int e = 0; e < 8; ++e)
+                                o[e] = f2b(0.25f * (b2f(prev[e]) + b2f(tr[e]) + b2f(v[e]) + b2f(br[e])));
+                            const size_t p0 = (((size_t)it.n * (H / 2) + (it.y0 >> 1)) * (W / 2) + (it.x0 >> 1)) * Cout;
+                            const uint32_t plo = ((uint32_t)((rw0 + i) >> 1) * (uint32_t)(W / 2) + (uint32_t)(l16 >> 1)) *
+                                                     (uint32_t)Cout + (uint32_t)co;
+                            *(gbf16x8w *)((char *)(pool + p0) + plo * 2u) = o;
+                        }
+                    }
+                    prev = v;
+                }
+            }
+        }
+    };
+    if constexpr (SINGLE) {
+        // one slice and one output-channel block (Cin = 32, Cout <= COT): the
+        // weights are staged once; items alternate between two halo stages,
+        // each loaded two items ahead
+        const int grid = (int)gridDim.x;
+        int itn = blockIdx.x;
+        if (itn >= items) return;
+        Stage sa, sb;
+        issue_weights(sa, item_of(itn), 0);
+        issue_halo(sa, item_of(itn), 0);
+        if (itn + grid < items) issue_halo(sb, item_of(itn + grid), 0);
+        store_weights(sa);
+        for (;;) {
+            store_halo(sa);
+            if (itn + 2 * grid < items) issue_halo(sa, item_of(itn + 2 * grid), 0);
+            compute(0);
+            epilogue(item_of(itn));
+            itn += grid;
+            if (itn >= items) break;
+            store_halo(sb);
+            if (itn + 2 * grid < items) issue_halo(sb, item_of(itn + 2 * grid), 0);
+            compute(0);
+            epilogue(item_of(itn));
+            itn += grid;
+            if (itn >= items) break;
+        }
+    } else {
+        Stage sg;
+        if ((int)blockIdx.x < items) {
+            issue_halo(sg, item_of(blockIdx.x), 0);
+            issue_weights(sg, item_of(blockIdx.x), 0);
+        }
+        for (int itn = blockIdx.x; itn < items; itn += gridDim.x) {
+            const Item it = item_of(itn);
+            for (int c0 = 0; c0 < Cin; c0 += 32) {
+                store_halo(sg);
+                store_weights(sg);
+                // the next step: the next slice of this item, or slice 0 of
+                // the next item
+                {
+                    const bool more = c0 + 32 < Cin;
+                    const int nitem = more ? itn : itn + (int)gridDim.x;
+                    if (nitem < items) {
+                        issue_halo(sg, item_of(nitem), more ? c0 + 32 : 0);
+                        issue_weights(sg, item_of(nitem), more ? c0 + 32 : 0);
+                    }
+                }
+                compute(c0);
+            }
+            epilogue(it);
         }
     }
 }
@@ -1185,24 +1246,32 @@ void conv(s3dg_cnn *c, const ConvIn &I, int N, int H, int W, const bf16 *wt, con
     // the epilogue: forward (bias + ReLU, optional pooling), fused head,
     // masked or plain data gradient (no bias, no ReLU)
     const int epi = head.out ? EPI_HEAD : (mask ? EPI_MASK : (relu ? EPI_FORWARD : EPI_LINEAR));
-#define S3DG_CONV(COT_, EPI_, FEAT_)                                                                                   \
-    hipLaunchKernelGGL((k_conv3x3<COT_, EPI_, FEAT_>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout, mask, out, \
-                       head, pool)
+#define S3DG_CONV(COT_, EPI_, FEAT_, SINGLE_)                                                                          \
+    hipLaunchKernelGGL((k_conv3x3<COT_, EPI_, FEAT_, SINGLE_>), grid, dim3(256), 0, s, I, N, H, W, wt, bias, Cout,      \
+                       mask, out, head, pool)
     // the fused head needs the 32-channel tiles (all of a pixel's channels in one wave)
     if (Cout % 64 == 0 && epi != EPI_HEAD) {
         const long long items = (long long)N * tx * ((H + 15) / 16) * (Cout / 64);
         const dim3 grid((unsigned)std::min(items, wgs));
-        if (epi == EPI_MASK) S3DG_CONV(64, EPI_MASK, false);
-        else if (epi == EPI_LINEAR) S3DG_CONV(64, EPI_LINEAR, false);
-        else S3DG_CONV(64, EPI_FORWARD, false);
+        if (epi == EPI_MASK) S3DG_CONV(64, EPI_MASK, false, false);
+        else if (epi == EPI_LINEAR) S3DG_CONV(64, EPI_LINEAR, false, false);
+        else S3DG_CONV(64, EPI_FORWARD, false, false);
     } else {
         const long long items = (long long)N * tx * ((H + 31) / 32) * ((Cout + 31) / 32);
         const dim3 grid((unsigned)std::min(items, wgs));
-        if (epi == EPI_HEAD) S3DG_CONV(32, EPI_HEAD, false);
-        else if (epi == EPI_MASK) S3DG_CONV(32, EPI_MASK, false);
-        else if (epi == EPI_LINEAR) S3DG_CONV(32, EPI_LINEAR, false);
-        else if (I.feat) S3DG_CONV(32, EPI_FORWARD, true);
-        else S3DG_CONV(32, EPI_FORWARD, false);
+        // one input slice and one output-channel block: the weights stay in
+        // LDS and two tiles' loads are in flight (the forward layers at
+        // 32 -> 32 channels and the first layer, whose full-resolution
+        // input comes from HBM; not the head, whose epilogue needs the
+        // registers the second stage would take, nor the half-resolution
+        // 32 -> 64 layer or the data gradients, measured no faster)
+        const bool single = I.C1 + I.C2 == 32 && Cout <= 32;
+        if (epi == EPI_HEAD) S3DG_CONV(32, EPI_HEAD, false, false);
+        else if (epi == EPI_MASK) S3DG_CONV(32, EPI_MASK, false, false);
+        else if (epi == EPI_LINEAR) S3DG_CONV(32, EPI_LINEAR, false, false);
+        else if (I.feat) S3DG_CONV(32, EPI_FORWARD, true, true);
+        else if (single) S3DG_CONV(32, EPI_FORWARD, false, true);
+        else S3DG_CONV(32, EPI_FORWARD, false, false);
     }
 #undef S3DG_CONV
 }
