@@ -58,9 +58,9 @@ struct Compressed {
 // depth-first position of the prim stored at tri_offset + j (triangles, or
 // shape records for top-level trees).  Returns the root node index (offset by
 // out.node_offset, like every node index written).  `margin`
-// is the absolute amount by which every child box is grown before
-// quantisation so that rounding in the kernels' slab tests can never reject a
-// box containing an accepted hit.
+// is an absolute amount by which every child box is grown before
+// quantisation (outward to the 8-bit grid); the scenes pass 0, since the
+// kernels' slab tests widen their distances by a bound on their rounding.
 uint32_t compress_tree(const RefTree &tree, const std::vector<Aabb> &prim_boxes, uint32_t tri_offset,
                        bool top_level, float margin, Compressed &out, std::vector<int32_t> &leaf_order);
 

@@ -257,10 +257,11 @@ uint32_t build_tlas(const s3dg_context *ctx, const std::vector<GInst> &inst, con
     RefTree top;
     if (ctx->leaf_size) build_traversal_tree(wbox, 1, top);
     else build_reference_tree(wbox, 8, top);
-    const Aabb all = top.nodes[top.root].box;
     comp.max_depth = 0;
     std::vector<int32_t> top_order;
-    const uint32_t root = compress_tree(top, wbox, 0u, true, scale_margin(all), comp, top_order);
+    // no margin: the traversal widens its slab distances by their rounding
+    // bound (the instance boxes in wbox carry their own for the transform)
+    const uint32_t root = compress_tree(top, wbox, 0u, true, 0.0f, comp, top_order);
     inst_dev.resize(inst.size());
     inst_pos.assign(inst.size(), 0);
     for (size_t k = 0; k < inst.size(); ++k) {
@@ -396,7 +397,7 @@ extern "C" int s3dg_scene_create(s3dg_context *ctx, const s3dg_scene_desc *d, s3
         const Aabb &rb = tree.nodes[tree.root].box;
         mesh_box[mi] = rb;
         comp.max_depth = 0;
-        info.root = compress_tree(walk, boxes, info.tri_offset, false, scale_margin(rb), comp, leaf_order);
+        info.root = compress_tree(walk, boxes, info.tri_offset, false, 0.0f, comp, leaf_order);
         blas_depth = std::max(blas_depth, comp.max_depth);
         // triangles in the order of the compressed leaves, ranked by their
         // depth-first position in the reference tree

@@ -133,6 +133,23 @@ __device__ __forceinline__ DRay to_local(const GInst &in, const DRay &r) {
 __device__ __forceinline__ float cull_hi(float t) { return t + fabsf(t) * 1e-4f + 1e-6f; }
 __device__ __forceinline__ float cull_lo(float t) { return t - fabsf(t) * 1e-4f - 1e-6f; }
 
+// Rounding bound of a slab distance, in distance units, for one axis of a
+// node: a plane at origin + q * 2^e (q <= 255) gives t = fma(q, a, b) with
+// a = 2^e * inv (exact), b = (origin - o) * inv and inv the approximate
+// reciprocal of the direction (v_rcp_f32, 1 ulp).  With D = origin - o,
+// E = q * 2^e, the computed t differs from (D + E) / d by at most
+// 2.5 * 2^-23 * (|D| + |E|) / |d| (the reciprocal, the subtraction, the
+// product and the fma).  slab_scale = |b| + 255 |a| bounds
+// (|D| + |E|) * |inv|, and kSlabSlack = 2^-20 is 3.2 times the bound (also
+// covering the rounding of the shifted b): entry distances are computed from
+// b - kSlabSlack * scale, exit distances from b + kSlabSlack * scale, so no
+// box holding an accepted hit is rejected, and the boxes themselves are not
+// grown.  A ray leaving a flat surface almost parallel to it (a grazing
+// sample off the ground plane) then no longer meets every leaf box of that
+// surface, as it did with boxes grown by a fixed fraction of the scene size.
+constexpr float kSlabSlack = 0x1p-20f;
+__device__ __forceinline__ float slab_scale(float a, float b) { return fmaf(255.0f, fabsf(a), fabsf(b)); }
+
 // Bit permutation taking slot order to visiting order k = slot ^ oct.
 __device__ __forceinline__ uint32_t to_korder(uint32_t m, uint32_t oct) {
     if (oct & 1u) m = ((m & 0x55u) << 1) | ((m >> 1) & 0x55u);
@@ -436,6 +453,12 @@ struct Tracer {
         const float sz = __uint_as_float(((h0.w >> 16) & 0xffu) << 23);
         const float ax = sx * pre.ix, ay = sy * pre.iy, az = sz * pre.iz;
         const float bx = (ox - ray.ox) * pre.ix, by = (oy - ray.oy) * pre.iy, bz = (oz - ray.oz) * pre.iz;
+        // entry distances lowered and exit distances raised by a bound on
+        // their rounding (kSlabSlack)
+        const float sx_ = slab_scale(ax, bx), sy_ = slab_scale(ay, by), sz_ = slab_scale(az, bz);
+        const float bnx = fmaf(-kSlabSlack, sx_, bx), bux = fmaf(kSlabSlack, sx_, bx);
+        const float bny = fmaf(-kSlabSlack, sy_, by), buy = fmaf(kSlabSlack, sy_, by);
+        const float bnz = fmaf(-kSlabSlack, sz_, bz), buz = fmaf(kSlabSlack, sz_, bz);
         const uint32_t top = h1.x & kTopLevelBit;
         const uint32_t child_base = h1.x & ~kTopLevelBit;
         const uint32_t lbase = h1.y, cnt_lo = h1.z, cnt_hi = h1.w;
@@ -455,12 +478,12 @@ struct Tracer {
             const uint32_t wnx = s < 4 ? nx0 : nx1, wux = s < 4 ? ux0 : ux1;
             const uint32_t wny = s < 4 ? ny0 : ny1, wuy = s < 4 ? uy0 : uy1;
             const uint32_t wnz = s < 4 ? nz0 : nz1, wuz = s < 4 ? uz0 : uz1;
-            const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bx);
-            const float tux = fmaf((float)((wux >> sh) & 0xffu), ax, bx);
-            const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, by);
-            const float tuy = fmaf((float)((wuy >> sh) & 0xffu), ay, by);
-            const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bz);
-            const float tuz = fmaf((float)((wuz >> sh) & 0xffu), az, bz);
+            const float tnx = fmaf((float)((wnx >> sh) & 0xffu), ax, bnx);
+            const float tux = fmaf((float)((wux >> sh) & 0xffu), ax, bux);
+            const float tny = fmaf((float)((wny >> sh) & 0xffu), ay, bny);
+            const float tuy = fmaf((float)((wuy >> sh) & 0xffu), ay, buy);
+            const float tnz = fmaf((float)((wnz >> sh) & 0xffu), az, bnz);
+            const float tuz = fmaf((float)((wuz >> sh) & 0xffu), az, buz);
             const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, lo_cull));
             const float tf = fminf(fminf(tux, tuy), fminf(tuz, bound));
             if (tn <= tf) hits |= 1u << s;
