@@ -1,8 +1,9 @@
 """Traversal edge cases, GPU against the oracle bit for bit: axis-aligned
 (flat) triangles on an integer grid hit exactly on vertices and shared edges
 (where Triangle::IsHit's strict edge tests reject both neighbours), rays along
-the coordinate axes (infinite reciprocals in the slab tests), grazing rays,
-ray ends exactly at a hit distance, and scenes far from the origin or tiny."""
+the coordinate axes (infinite reciprocals in the slab tests), grazing rays in
+the plane and leaving it at slopes down to 1e-7, ray ends exactly at a hit
+distance, and scenes far from the origin or tiny."""
 import numpy as np
 import pytest
 
@@ -60,6 +61,17 @@ def _edge_rays(n, offset, scale):
         a = rng.uniform(0, 2 * np.pi)
         o.append((rng.uniform(0, n), rng.uniform(0, n), 0.0))
         dirs.append((np.cos(a), np.sin(a), 0.0))
+        tmin.append(1e-3)
+        tmax.append(1e12)
+    # rays leaving the plane z = 0 almost parallel to it, from on, just above
+    # and just below it (the case the slab test's rounding slack must keep
+    # exact with flat leaf boxes)
+    for k in range(600):
+        a = rng.uniform(0, 2 * np.pi)
+        s = rng.choice([0.0, 1e-7, -1e-7, 1e-6, -1e-6, 1e-5, -1e-5, 1e-3, -1e-3])
+        d = np.array([np.cos(a), np.sin(a), s])
+        o.append((rng.uniform(0, n), rng.uniform(0, n), rng.choice([0.0, 1e-7, -1e-7, 1e-6, -1e-6])))
+        dirs.append(tuple(d / np.linalg.norm(d)))
         tmin.append(1e-3)
         tmax.append(1e12)
     for k in range(3000):
