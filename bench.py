@@ -258,6 +258,11 @@ def soup_run(args, steps, warmup, cpu_seconds):
         res[name] = (time.perf_counter() - t) / steps
     bytes_per_ray = nodes * NODE_BYTES + tris * TRI_BYTES + 32 + 16
     achieved = bytes_per_ray * n / res["closest"] / 1e9
+    # the same closest-hit trace through the host-buffer entry point (rays
+    # in and hits out over PCIe): reported beside the line, never its value
+    t = time.perf_counter()
+    scene.trace(rays, abi.TRACE_CLOSEST)
+    host_s = time.perf_counter() - t
     line = {
         "metric": "Grays/s, closest-hit batch traversal (any-hit beside it)",
         "value": n / res["closest"] / 1e9,
@@ -273,6 +278,7 @@ def soup_run(args, steps, warmup, cpu_seconds):
         "data": "synthetic (seeded triangle soup and rays)",
         "config": {"workload": f"triangle soup, {args.soup_tris} triangles, {n} rays, tmin 1e-4, tmax inf",
                    "any_hit_grays_per_s": n / res["any"] / 1e9, "any_hit_ms": res["any"] * 1e3,
+                   "host_buffer_grays_per_s": n / host_s / 1e9, "host_buffer_ms": host_s * 1e3,
                    "scene_build_s": build_s, "generate_s": gen_s,
                    "device_scene_bytes": scene.stats().device_bytes,
                    "refill": picked or None},
@@ -650,6 +656,15 @@ def main():
     # launch (on the context's stream, where the kernels run)
     kst = scene.render_device(cfg(args.spp, abi.RENDER_TIME_KERNELS, first_pass=0), d_accum)
     launches = kst.extend_launches
+    # the same frame through the host-buffer entry point (s3dg_render copies
+    # the accumulation buffer in and out over PCIe): reported beside the
+    # line, never as its value
+    host_acc = np.zeros((H, W, 3), np.float32)
+    t = time.perf_counter()
+    _, hst = scene.render(W, H, args.spp, args.depth, seed=args.seed, accum=host_acc,
+                          passes_in_flight=args.passes_in_flight)
+    pcie_s = time.perf_counter() - t
+    pcie_grays = (hst.extension_rays + hst.shadow_rays) / pcie_s / 1e9
 
     rays = rays_ext + rays_sh
     grays = rays / elapsed / 1e9
@@ -689,6 +704,8 @@ def main():
             "timed_path": "HIP-graph bounce loop (default render flags)",
             "device_ms_per_step": device_ms / args.steps,
             "instrumented_frame_ms": kst.milliseconds,
+            "host_buffer_grays_per_s": pcie_grays,
+            "host_buffer_ms": pcie_s * 1e3,
         },
         "roofline": {
             "kernel": "k_extend (closest-hit traversal of the compressed BVH8)",
