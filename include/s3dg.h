@@ -485,6 +485,13 @@ int s3dg_cnn_set_params(s3dg_cnn *cnn, const float *host, uint64_t count);
  * into a HIP graph, later ones replay it (same results). */
 int s3dg_cnn_infer_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
                           float *d_out_rgb);
+/* No reference counterpart: the denoiser's input features [n][h][w][10] from
+ * a render's device buffers (accumulated RGB sum of spp passes, guides from
+ * s3dg_render_guides_device): sum / spp, albedo, normal, depth, as a caller
+ * assembles them on the host, bit for bit. */
+int s3dg_denoiser_features_device(s3dg_context *ctx, int32_t n, int32_t h, int32_t w, const float *d_accum,
+                                  int32_t spp, const float *d_albedo, const float *d_normal, const float *d_depth,
+                                  float *d_features);
 /* Mean loss and the gradient of every parameter (host copy optional). */
 int s3dg_cnn_gradient_device(s3dg_cnn *cnn, int32_t n, int32_t h, int32_t w, const float *d_features,
                              const float *d_target_rgb, double *loss, float *host_grad, uint64_t count);

@@ -76,6 +76,8 @@ _SIGNATURES = {
     "s3dg_cnn_get_params": [C.c_void_p, C.c_void_p, C.c_uint64],
     "s3dg_cnn_set_params": [C.c_void_p, C.c_void_p, C.c_uint64],
     "s3dg_cnn_infer_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p],
+    "s3dg_denoiser_features_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_int32,
+                                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "s3dg_cnn_gradient_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                  C.POINTER(C.c_double), C.c_void_p, C.c_uint64],
     "s3dg_cnn_train_step_device": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_float,
@@ -262,6 +264,14 @@ class Context:
         self.check(self.lib.s3dg_nlm_device(self.handle, w, h, coeff, search_radius, patch_radius,
                                             C.c_void_p(d_in), C.c_void_p(d_out)), "s3dg_nlm_device")
 
+    def features_device(self, n: int, h: int, w: int, d_accum: int, spp: int, d_albedo: int, d_normal: int,
+                        d_depth: int, d_features: int):
+        """The denoiser's 10 input features from a render's device buffers."""
+        self.check(self.lib.s3dg_denoiser_features_device(self.handle, n, h, w, C.c_void_p(d_accum), spp,
+                                                          C.c_void_p(d_albedo), C.c_void_p(d_normal),
+                                                          C.c_void_p(d_depth), C.c_void_p(d_features)),
+                   "s3dg_denoiser_features_device")
+
 
 class Scene:
     """A scene uploaded to the device (two-level compressed BVH + shading data)."""
@@ -389,6 +399,47 @@ class SceneSet:
         self.ctx.check(self.ctx.lib.s3dg_render_batch_device(self.handle, len(frames), jobs, C.byref(cfg),
                                                              C.byref(st)), "s3dg_render_batch_device")
         return st
+
+
+class DeviceTimer:
+    """Device milliseconds of the work a context's stream runs between
+    start() and stop(), from a pair of HIP events recorded on that stream
+    (the HIP runtime the library already loaded, through ctypes)."""
+
+    _hip = None
+
+    def __init__(self, ctx: "Context"):
+        if DeviceTimer._hip is None:
+            hip = C.CDLL("libamdhip64.so")
+            hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+            hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+            hip.hipEventSynchronize.argtypes = [C.c_void_p]
+            hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+            hip.hipEventDestroy.argtypes = [C.c_void_p]
+            DeviceTimer._hip = hip
+        self.ctx = ctx
+        self.ev = [C.c_void_p(), C.c_void_p()]
+        for e in self.ev:
+            if DeviceTimer._hip.hipEventCreate(C.byref(e)) != 0:
+                raise S3dgError("hipEventCreate failed")
+
+    def start(self):
+        DeviceTimer._hip.hipEventRecord(self.ev[0], C.c_void_p(self.ctx.stream))
+
+    def stop(self) -> float:
+        hip = DeviceTimer._hip
+        hip.hipEventRecord(self.ev[1], C.c_void_p(self.ctx.stream))
+        hip.hipEventSynchronize(self.ev[1])
+        ms = C.c_float()
+        if hip.hipEventElapsedTime(C.byref(ms), self.ev[0], self.ev[1]) != 0:
+            raise S3dgError("hipEventElapsedTime failed")
+        return float(ms.value)
+
+    def close(self):
+        for e in self.ev:
+            if e:
+                DeviceTimer._hip.hipEventDestroy(e)
+        self.ev = []
 
 
 class Denoiser:

@@ -871,6 +871,24 @@ __global__ void k_split_grad(const bf16 *__restrict__ dcat, int32_t N, int32_t H
     }
 }
 
+// The denoiser's 10 input features per pixel from a render's device buffers:
+// mean radiance (the accumulated sum divided by the sample count, an IEEE
+// division as the host's mean), albedo, normal, depth.  One thread per pixel.
+__global__ void k_features(long long npix, const float *__restrict__ accum, float spp,
+                           const float *__restrict__ albedo, const float *__restrict__ normal,
+                           const float *__restrict__ depth, float *__restrict__ out) {
+    for (long long p = blockIdx.x * 256LL + threadIdx.x; p < npix; p += (long long)gridDim.x * 256) {
+        float *o = out + p * kInChannels;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o[c] = accum[3 * p + c] / spp;
+            o[3 + c] = albedo[3 * p + c];
+            o[6 + c] = normal[3 * p + c];
+        }
+        o[9] = depth[p];
+    }
+}
+
 // Input features -> padded bf16: log1p radiance, albedo, normal, log1p depth.
 template <bool PAIRS>
 __global__ void k_pack_input(const float *__restrict__ f, long long npix, bf16 *__restrict__ x) {
@@ -1505,6 +1523,21 @@ extern "C" int s3dg_cnn_create(s3dg_context *ctx, uint64_t seed, s3dg_cnn **out)
     pack_weights(c);
     S3DG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *out = c;
+    return S3DG_OK;
+}
+
+extern "C" int s3dg_denoiser_features_device(s3dg_context *ctx, int32_t n, int32_t h, int32_t w,
+                                             const float *d_accum, int32_t spp, const float *d_albedo,
+                                             const float *d_normal, const float *d_depth, float *d_features) {
+    if (!ctx || !d_accum || !d_albedo || !d_normal || !d_depth || !d_features || spp <= 0)
+        return S3DG_ERR_INVALID;
+    if (n <= 0 || h <= 0 || w <= 0 || (long long)n * h * w > (1LL << 27))
+        return set_error(ctx, S3DG_ERR_INVALID, "denoiser features: bad size");
+    S3DG_HIP(ctx, hipSetDevice(ctx->device));
+    const long long npix = (long long)n * h * w;
+    hipLaunchKernelGGL(k_features, dim3(grid1(npix)), dim3(256), 0, ctx->stream, npix, d_accum, (float)spp,
+                       d_albedo, d_normal, d_depth, d_features);
+    S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
 }
 

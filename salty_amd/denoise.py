@@ -74,10 +74,14 @@ class FrameRenderer:
         return out
 
     def guides(self):
-        t = time.perf_counter()
+        """(albedo, normal, depth) on the host, and the device milliseconds of
+        the guide render (HIP events on the context stream)."""
+        import salty_amd
+        timer = salty_amd.DeviceTimer(self.ctx)
+        timer.start()
         self.scene.guides_device(self.cfg(1), self.d_alb, self.d_nrm, self.d_dep)
-        self.ctx.synchronize()
-        ms = (time.perf_counter() - t) * 1e3
+        ms = timer.stop()
+        timer.close()
         a = self.ctx.download(self.d_alb, (self.h, self.w, 3), np.float32)
         n = self.ctx.download(self.d_nrm, (self.h, self.w, 3), np.float32)
         d = self.ctx.download(self.d_dep, (self.h, self.w, 1), np.float32)
@@ -275,7 +279,10 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65
     guides + U-Net inference.  Images come from one progressive render; the
     time of a sample count is that of one render of that many samples (timed
     after an untimed one), and times are interpolated between the measured
-    sample counts (relMSE of plain path tracing falls as 1/spp)."""
+    sample counts (relMSE of plain path tracing falls as 1/spp).  The guide
+    render and the feature assembly + U-Net are timed with HIP events on the
+    context stream, like the renders."""
+    import salty_amd
     fr = FrameRenderer(ctx, desc, width, height, max_depth, seed=seed)
     gt, gt_ms = fr.mean(gt_spp, first_pass=1 << 24)
     a, n, d, guide_ms = fr.guides()
@@ -301,16 +308,19 @@ def time_to_relmse(ctx, net, desc, target=2e-3, width=256, height=256, gt_spp=65
         net.infer_device(1, height, width, d_f, d_o)
     ctx.synchronize()
     den = []
+    timer = salty_amd.DeviceTimer(ctx)
     for spp, img, _ in fr.ladder(den_levels):
+        # render_ms leaves the sum of the spp passes (the ladder's image times
+        # spp, bit for bit) in the device accumulation buffer; the features
+        # are assembled there and the U-Net runs on them, device-timed
         ms = fr.render_ms(spp)
-        ctx.copy_to_device(d_f, features(img, a, n, d)[None])
-        ctx.synchronize()
-        t = time.perf_counter()
+        timer.start()
+        ctx.features_device(1, height, width, fr.d_acc, spp, fr.d_alb, fr.d_nrm, fr.d_dep, d_f)
         net.infer_device(1, height, width, d_f, d_o)
-        ctx.synchronize()
-        inf_ms = (time.perf_counter() - t) * 1e3
+        inf_ms = timer.stop()
         out = ctx.download(d_o, (height, width, 3), np.float32)
         den.append((spp, ms, inf_ms, rel_mse(out, gt)))
+    timer.close()
     best = [(ms + guide_ms + inf, r) for spp, ms, inf, r in den]
     den_ms = None
     for tot, r in sorted(best):
@@ -385,9 +395,11 @@ def evaluate_heldout(ctx, net, seeds, width=256, height=256, spp_in=4, gt_spp=65
     on that input with the guides, of NLM on that input at its best strength
     (nlm_sweep), and of a plain plain_spp render, with the device times of
     each path."""
+    import salty_amd
     d_f = ctx.malloc(width * height * 10 * 4)
     d_o = ctx.malloc(width * height * 3 * 4)
     nlm_ms = nlm_device_ms(ctx, width, height)
+    timer = salty_amd.DeviceTimer(ctx)
     rows = []
     try:
         for s in seeds:
@@ -395,14 +407,16 @@ def evaluate_heldout(ctx, net, seeds, width=256, height=256, spp_in=4, gt_spp=65
             gt, _ = fr.mean(gt_spp, first_pass=1 << 24)
             a, n, d, guide_ms = fr.guides()
             img, _ = fr.mean(spp_in)
+            # the sum of the spp_in passes stays in the device accumulation
+            # buffer: features assembled there, then the U-Net (the untimed
+            # run captures the graph the timed one replays)
             in_ms = fr.render_ms(spp_in)
-            ctx.copy_to_device(d_f, features(img, a, n, d)[None])
+            ctx.features_device(1, height, width, fr.d_acc, spp_in, fr.d_alb, fr.d_nrm, fr.d_dep, d_f)
             net.infer_device(1, height, width, d_f, d_o)
-            ctx.synchronize()
-            t = time.perf_counter()
+            timer.start()
+            ctx.features_device(1, height, width, fr.d_acc, spp_in, fr.d_alb, fr.d_nrm, fr.d_dep, d_f)
             net.infer_device(1, height, width, d_f, d_o)
-            ctx.synchronize()
-            inf_ms = (time.perf_counter() - t) * 1e3
+            inf_ms = timer.stop()
             den = ctx.download(d_o, (height, width, 3), np.float32)
             best, _ = nlm_sweep(ctx, img, gt)
             plain, _ = fr.mean(plain_spp, first_pass=1 << 16)
@@ -415,6 +429,7 @@ def evaluate_heldout(ctx, net, seeds, width=256, height=256, spp_in=4, gt_spp=65
             if log:
                 log(row)
     finally:
+        timer.close()
         ctx.free(d_f)
         ctx.free(d_o)
     return rows
