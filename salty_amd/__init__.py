@@ -410,7 +410,10 @@ class DeviceTimer:
 
     def __init__(self, ctx: "Context"):
         if DeviceTimer._hip is None:
-            hip = C.CDLL("libamdhip64.so")
+            try:
+                hip = C.CDLL("libamdhip64.so")
+            except OSError:
+                hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
             hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
             hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
             hip.hipEventSynchronize.argtypes = [C.c_void_p]
