@@ -427,12 +427,13 @@ class DeviceTimer:
                 raise S3dgError("hipEventCreate failed")
 
     def start(self):
-        DeviceTimer._hip.hipEventRecord(self.ev[0], C.c_void_p(self.ctx.stream))
+        if DeviceTimer._hip.hipEventRecord(self.ev[0], C.c_void_p(self.ctx.stream)) != 0:
+            raise S3dgError("hipEventRecord failed")
 
     def stop(self) -> float:
         hip = DeviceTimer._hip
-        hip.hipEventRecord(self.ev[1], C.c_void_p(self.ctx.stream))
-        hip.hipEventSynchronize(self.ev[1])
+        if hip.hipEventRecord(self.ev[1], C.c_void_p(self.ctx.stream)) != 0 or hip.hipEventSynchronize(self.ev[1]) != 0:
+            raise S3dgError("hipEventRecord / hipEventSynchronize failed")
         ms = C.c_float()
         if hip.hipEventElapsedTime(C.byref(ms), self.ev[0], self.ev[1]) != 0:
             raise S3dgError("hipEventElapsedTime failed")

@@ -145,24 +145,19 @@ __global__ __launch_bounds__(256) void k_tonemap(int32_t type, const float *__re
 }
 
 // ---------------------------------------------------------------------------
-// Non-local means: FilterNLM (src/src/s3d_denoiser.cpp:19-95).  One thread per
-// pixel walks the search window and the patch in the reference's order
-// (x offset outer, y offset inner; RGBA distance summed as (xx+yy)+(zz+ww)),
-// with the clamped source tile staged in LDS.
+// Non-local means: FilterNLM (src/src/s3d_denoiser.cpp:19-95), search radius
+// rs, patch radius rp, the source tile clamped to the image and staged in LDS.
+// Patch radius 2 (the reference's) runs as box sums (k_nlm_box below); other
+// patch radii walk the search window and the patch per pixel in the
+// reference's order (x offset outer, y offset inner; RGBA distance summed as
+// (xx+yy)+(zz+ww)), one thread per pixel (k_nlm).
 // ---------------------------------------------------------------------------
 constexpr int kTile = 16;
 constexpr int kMaxHalo = 12;   // search radius + patch radius
 
-// RP >= 0: the patch radius is RP, and the centre patch's (2 RP + 1)^2 texels
-// are held in registers across the search offsets, halving the LDS reads
-// (the sums are formed in the same order, so the result is the same);
-// RP < 0: the patch radius is the argument rp.
-template <int RP, int RS>
-__global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, float frac_h2, float sigma2, int32_t rs_arg,
-                                                      int32_t rp_arg, const float4 *__restrict__ src,
+__global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, float frac_h2, float sigma2, int32_t rs,
+                                                      int32_t rp, const float4 *__restrict__ src,
                                                       float4 *__restrict__ dst) {
-    const int rp = RP >= 0 ? RP : rp_arg;
-    const int rs = RS >= 0 ? RS : rs_arg;
     __shared__ float4 tile[(kTile + 2 * kMaxHalo) * (kTile + 2 * kMaxHalo)];
     const int halo = rs + rp;
     const int tw = kTile + 2 * halo;
@@ -178,41 +173,17 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
     const int x = blockIdx.x * kTile + lx, y = blockIdx.y * kTile + ly;
     if (x >= W || y >= H) return;
     const int cx = lx + halo, cy = ly + halo;   // pixel position inside the tile
-    constexpr int PS = RP >= 0 ? 2 * RP + 1 : 1;
-    float4 ctr[PS][PS];
-    if (RP >= 0) {
-#pragma unroll
-        for (int px = 0; px < PS; ++px)
-#pragma unroll
-            for (int py = 0; py < PS; ++py) ctr[px][py] = tile[(cy + py - RP) * tw + (cx + px - RP)];
-    }
     float4 sum = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float sw = 0.0f;
     for (int ox = -rs; ox <= rs; ++ox) {
-        // with RS known the offsets of one column are unrolled: a window row
-        // read for one offset is read once for all the offsets that use it
-#pragma unroll
         for (int oy = -rs; oy <= rs; ++oy) {
             float acc = 0.0f;
-            if (RP >= 0) {
-#pragma unroll
-                for (int px = 0; px < PS; ++px) {
-#pragma unroll
-                    for (int py = 0; py < PS; ++py) {
-                        const float4 a = ctr[px][py];
-                        const float4 b = tile[(cy + oy + py - RP) * tw + (cx + ox + px - RP)];
-                        const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
-                        acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
-                    }
-                }
-            } else {
-                for (int px = -rp; px <= rp; ++px) {
-                    for (int py = -rp; py <= rp; ++py) {
-                        const float4 a = tile[(cy + py) * tw + (cx + px)];
-                        const float4 b = tile[(cy + oy + py) * tw + (cx + ox + px)];
-                        const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
-                        acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
-                    }
+            for (int px = -rp; px <= rp; ++px) {
+                for (int py = -rp; py <= rp; ++py) {
+                    const float4 a = tile[(cy + py) * tw + (cx + px)];
+                    const float4 b = tile[(cy + oy + py) * tw + (cx + ox + px)];
+                    const float d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z, d3 = a.w - b.w;
+                    acc += ((d0 * d0) + (d1 * d1)) + ((d2 * d2) + (d3 * d3));
                 }
             }
             const float arg = -fmaxf(acc - 2.0f * sigma2, 0.0f) * frac_h2;
@@ -226,6 +197,163 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
         }
     }
     dst[y * W + x] = make_float4(sum.x / sw, sum.y / sw, sum.z / sw, sum.w / sw);
+}
+
+// Non-local means for patch radius 2 (the reference's) by box sums.  For one
+// search offset o the patch distance of pixel p is the sum, over the 5 x 5
+// texels u around p, of D_o(u) = |I(u) - I(u + o)|^2 (RGBA, (xx+yy)+(zz+ww),
+// coordinates clamped to the image as above), so each D_o(u) is formed once
+// and shared by the 25 patches containing u instead of once per patch.  A
+// wave's 64 lanes are 64 consecutive columns u (60 output pixels and two
+// columns either side); the horizontal 5-sums come from the neighbouring
+// lanes (whole-wave DPP shifts by one lane), the vertical ones from a lane's
+// own rows: a band of 8 output rows takes 12 rows of D_o.  SPLIT = 1: the
+// four waves of a workgroup take four bands (a 60 x 32 pixel tile) and each
+// walks all search offsets in the reference's order (x offset outer, y
+// offset inner).  SPLIT = 4, for images with few bands: the four waves share
+// one band (a 60 x 8 tile) and take a quarter of the offsets each, in that
+// order, and their weighted sums are added in wave order.  The patch
+// distance is summed horizontally, then vertically, with fused multiply-adds,
+// and the weights come from the hardware exponential, so results differ from
+// the reference's by rounding only.
+constexpr int kBoxLanes = 64, kBoxOutCols = kBoxLanes - 4, kBoxBand = 8, kBoxWaves = 4;
+
+// v from the lane one below (CTRL 0x138, wave_shr:1) or one above (0x130,
+// wave_shl:1); lanes with no source receive 0
+template <int CTRL>
+__device__ __forceinline__ float wave_shift1(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// sum of v over lanes l - 2 .. l + 2 (exact only for lanes 2 .. 61)
+__device__ __forceinline__ float lane_sum5(float v) {
+    const float a1 = wave_shift1<0x138>(v), b1 = wave_shift1<0x130>(v);
+    const float a2 = wave_shift1<0x138>(a1), b2 = wave_shift1<0x130>(b1);
+    return ((a2 + a1) + v) + (b1 + b2);
+}
+
+template <int RS, int SPLIT>
+__global__ __launch_bounds__(kBoxLanes *kBoxWaves, 2) void k_nlm_box(int32_t W, int32_t H, float frac_h2,
+                                                                     float sigma2, int32_t rs_arg,
+                                                                     const float4 *__restrict__ src,
+                                                                     float4 *__restrict__ dst) {
+    static_assert(SPLIT == 1 || SPLIT == kBoxWaves, "one band per wave or one band per workgroup");
+    constexpr int kBands = kBoxWaves / SPLIT, kOutRows = kBands * kBoxBand;
+    constexpr int kRsMax = RS >= 0 ? RS : kMaxHalo - 2;
+    __shared__ float4 tile[(kOutRows + 4 + 2 * kRsMax) * (kBoxLanes + 2 * kRsMax)];
+    // SPLIT > 1: the weighted sums of waves 1 .. SPLIT-1
+    __shared__ float4 part_sum[SPLIT > 1 ? (SPLIT - 1) * kBoxBand * kBoxLanes : 1];
+    __shared__ float part_sw[SPLIT > 1 ? (SPLIT - 1) * kBoxBand * kBoxLanes : 1];
+    const int rs = RS >= 0 ? RS : rs_arg;
+    const int tw = kBoxLanes + 2 * rs, th = kOutRows + 4 + 2 * rs;
+    const int x0 = blockIdx.x * kBoxOutCols, y0 = blockIdx.y * kOutRows;
+    // tile origin: two patch texels and the search radius before the first output pixel
+    const int tx0 = x0 - 2 - rs, ty0 = y0 - 2 - rs;
+    for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
+        const int tx = i % tw, ty = i / tw;
+        const int gx = min(max(tx0 + tx, 0), W - 1), gy = min(max(ty0 + ty, 0), H - 1);
+        tile[i] = src[(size_t)gy * W + gx];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int band = wave / SPLIT, part = wave % SPLIT;
+    // this lane's column u = x0 - 2 + lane, its band's D rows y0 + 8 band - 2 + r
+    // (r = 0 .. 11): tile column lane + rs, tile rows 8 band + rs + r
+    const float4 *const base = tile + (band * kBoxBand + rs) * tw + lane + rs;
+    float4 c[kBoxBand + 4];
+#pragma unroll
+    for (int r = 0; r < kBoxBand + 4; ++r) c[r] = base[r * tw];
+    float4 sum[kBoxBand];
+    float sw[kBoxBand];
+#pragma unroll
+    for (int j = 0; j < kBoxBand; ++j) {
+        sum[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        sw[j] = 0.0f;
+    }
+    // this wave's offsets: k = (ox + rs) (2 rs + 1) + (oy + rs) in [k0, k1)
+    const int side = 2 * rs + 1, noff = side * side;
+    const int k0 = part * noff / SPLIT, k1 = (part + 1) * noff / SPLIT;
+    for (int k = k0; k < k1; ++k) {
+        const int ox = k / side - rs, oy = k % side - rs;
+        const float4 *const nb = base + oy * tw + ox;
+        float hs[kBoxBand + 4];
+        float4 s[kBoxBand];   // I(p + o) of the band's output pixels p
+#pragma unroll
+        for (int r = 0; r < kBoxBand + 4; ++r) {
+            const float4 n = nb[r * tw];
+            if (r >= 2 && r < kBoxBand + 2) s[r - 2] = n;
+            const float d0 = c[r].x - n.x, d1 = c[r].y - n.y, d2 = c[r].z - n.z, d3 = c[r].w - n.w;
+            hs[r] = lane_sum5(fmaf(d0, d0, d1 * d1) + fmaf(d2, d2, d3 * d3));
+        }
+#pragma unroll
+        for (int j = 0; j < kBoxBand; ++j) {
+            const float acc = (((hs[j] + hs[j + 1]) + hs[j + 2]) + hs[j + 3]) + hs[j + 4];
+            // the hardware exponential: the weights lie in (0, 1], and the
+            // centre offset's is exactly 1, so its relative error (a few
+            // ulp) is all that reaches the result
+            const float w = __expf(-fmaxf(acc - 2.0f * sigma2, 0.0f) * frac_h2);
+            sw[j] += w;
+            sum[j].x = fmaf(s[j].x, w, sum[j].x);
+            sum[j].y = fmaf(s[j].y, w, sum[j].y);
+            sum[j].z = fmaf(s[j].z, w, sum[j].z);
+            sum[j].w = fmaf(s[j].w, w, sum[j].w);
+        }
+    }
+    if constexpr (SPLIT > 1) {
+        if (part > 0) {
+#pragma unroll
+            for (int j = 0; j < kBoxBand; ++j) {
+                const int e = ((part - 1) * kBoxBand + j) * kBoxLanes + lane;
+                part_sum[e] = sum[j];
+                part_sw[e] = sw[j];
+            }
+        }
+        __syncthreads();
+        if (part > 0) return;
+#pragma unroll
+        for (int q = 0; q < SPLIT - 1; ++q) {
+#pragma unroll
+            for (int j = 0; j < kBoxBand; ++j) {
+                const int e = (q * kBoxBand + j) * kBoxLanes + lane;
+                const float4 ps = part_sum[e];
+                sum[j].x += ps.x;
+                sum[j].y += ps.y;
+                sum[j].z += ps.z;
+                sum[j].w += ps.w;
+                sw[j] += part_sw[e];
+            }
+        }
+    }
+    const int x = x0 + lane - 2;
+    if (lane >= 2 && lane < kBoxLanes - 2 && x < W) {
+#pragma unroll
+        for (int j = 0; j < kBoxBand; ++j) {
+            const int y = y0 + band * kBoxBand + j;
+            if (y < H)
+                dst[(size_t)y * W + x] =
+                    make_float4(sum[j].x / sw[j], sum[j].y / sw[j], sum[j].z / sw[j], sum[j].w / sw[j]);
+        }
+    }
+}
+
+// Launches k_nlm_box with the band layout that fills the GPU: one band per
+// wave from kNlmSplitBelow bands of 60 x 8 pixels up (two waves' worth per
+// SIMD of 1,024 and more), one band per workgroup below that.
+constexpr int kNlmSplitBelow = 4096;
+
+template <int RS>
+void launch_nlm_box(hipStream_t stream, int32_t w, int32_t h, float frac_h2, float sigma2, int32_t rs,
+                    const float4 *s4, float4 *d4) {
+    const int cols = (w + kBoxOutCols - 1) / kBoxOutCols;
+    const int bands = cols * ((h + kBoxBand - 1) / kBoxBand);
+    const dim3 blk(kBoxLanes * kBoxWaves);
+    if (bands >= kNlmSplitBelow) {
+        const dim3 grid(cols, (h + kBoxBand * kBoxWaves - 1) / (kBoxBand * kBoxWaves));
+        hipLaunchKernelGGL((k_nlm_box<RS, 1>), grid, blk, 0, stream, w, h, frac_h2, sigma2, rs, s4, d4);
+    } else {
+        const dim3 grid(cols, (h + kBoxBand - 1) / kBoxBand);
+        hipLaunchKernelGGL((k_nlm_box<RS, kBoxWaves>), grid, blk, 0, stream, w, h, frac_h2, sigma2, rs, s4, d4);
+    }
 }
 
 // Checks a tonemap call's arguments; returns the pixel count through n.
@@ -341,20 +469,21 @@ extern "C" int s3dg_nlm_device(s3dg_context *ctx, int32_t w, int32_t h, float co
     const float sigma = std::max(0.0001f, coeff);
     const float frac_h2 = 1.0f / (param_h * param_h);
     const float sigma2 = sigma * sigma;
-    dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
-    // the reference's patch radius (2) and search radii (6 there, 7 in the
-    // denoiser comparison) have their own instantiations
     const float4 *s4 = (const float4 *)d_src;
     float4 *d4 = (float4 *)d_dst;
-    const dim3 blk(kTile * kTile);
-    if (rp == 2 && rs == 6)
-        hipLaunchKernelGGL((k_nlm<2, 6>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
-    else if (rp == 2 && rs == 7)
-        hipLaunchKernelGGL((k_nlm<2, 7>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
-    else if (rp == 2)
-        hipLaunchKernelGGL((k_nlm<2, -1>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
-    else
-        hipLaunchKernelGGL((k_nlm<-1, -1>), grid, blk, 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
+    if (rp == 2) {
+        // box sums (rs <= 10 here); the reference's search radius (6) and the
+        // denoiser comparison's (7) have their own instantiations
+        if (rs == 6)
+            launch_nlm_box<6>(ctx->stream, w, h, frac_h2, sigma2, rs, s4, d4);
+        else if (rs == 7)
+            launch_nlm_box<7>(ctx->stream, w, h, frac_h2, sigma2, rs, s4, d4);
+        else
+            launch_nlm_box<-1>(ctx->stream, w, h, frac_h2, sigma2, rs, s4, d4);
+    } else {
+        const dim3 grid((w + kTile - 1) / kTile, (h + kTile - 1) / kTile);
+        hipLaunchKernelGGL(k_nlm, grid, dim3(kTile * kTile), 0, ctx->stream, w, h, frac_h2, sigma2, rs, rp, s4, d4);
+    }
     S3DG_HIP(ctx, hipGetLastError());
     return S3DG_OK;
 }

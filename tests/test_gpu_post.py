@@ -52,8 +52,12 @@ def test_capture_of_exact_values(ctx):
     assert len(np.unique(out8.reshape(-1, 3), axis=0)) == 1
 
 
-@pytest.mark.parametrize("rs,rp,coeff", [(6, 2, 0.1), (7, 2, 0.05), (3, 1, 0.4), (6, 2, 1e-6)])
-@pytest.mark.parametrize("shape", [(48, 40), (5, 3), (17, 70)])
+# patch radius 2 runs as box sums over 60 x 32 pixel tiles (search radii 6
+# and 7 compiled in, others at run time, up to 10); other patch radii run
+# per pixel; the shapes cover one tile, partial tiles and 3 x 3 tiles
+@pytest.mark.parametrize("rs,rp,coeff", [(6, 2, 0.1), (7, 2, 0.05), (3, 1, 0.4), (6, 2, 1e-6), (10, 2, 0.2),
+                                         (2, 2, 0.3)])
+@pytest.mark.parametrize("shape", [(48, 40), (5, 3), (17, 70), (70, 130)])
 def test_nlm_matches_oracle(ctx, rs, rp, coeff, shape):
     rng = np.random.default_rng(rs * 100 + shape[1])
     h, w = shape
@@ -64,3 +68,31 @@ def test_nlm_matches_oracle(ctx, rs, rp, coeff, shape):
     want = O.nlm(img, coeff, rs, rp)
     assert np.isfinite(got).all()
     assert np.allclose(got, want, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("rs", [6, 7])
+def test_nlm_1080p_matches_oracle_on_crops(ctx, rs):
+    """At 1920x1080 the box-sum NLM gives each wave its own band of rows (the
+    small-image tests above run the layout whose waves share a band).  An
+    output pixel depends only on the input within rs + 2 texels, clamped at
+    the image's edges, so the oracle run on a crop reproduces the pixels at
+    least rs + 2 texels inside it and, where the crop reaches the image's
+    edge, up to that edge: the two corners and a window across tile seams
+    (columns 60k, rows 32k) are compared."""
+    h, w = 1080, 1920
+    rng = np.random.default_rng(42 + rs)
+    base = np.linspace(0, 1, h * w, dtype=np.float32).reshape(h, w, 1)
+    img = np.concatenate([base + rng.normal(scale=0.08, size=(h, w, 3)).astype(np.float32),
+                          rng.uniform(0.5, 1.0, (h, w, 1)).astype(np.float32)], -1).astype(np.float32)
+    got = ctx.nlm(img, 0.1, rs, 2)
+    assert np.isfinite(got).all()
+    m = rs + 2
+    # (crop rows, crop columns, kept rows and columns inside the crop)
+    for y0, y1, x0, x1 in ((0, 70, 0, 130), (h - 70, h, w - 130, w), (300, 400, 900, 1040)):
+        want = O.nlm(np.ascontiguousarray(img[y0:y1, x0:x1]), 0.1, rs, 2)
+        ky0 = 0 if y0 == 0 else m
+        ky1 = (y1 - y0) if y1 == h else (y1 - y0) - m
+        kx0 = 0 if x0 == 0 else m
+        kx1 = (x1 - x0) if x1 == w else (x1 - x0) - m
+        assert np.allclose(got[y0 + ky0:y0 + ky1, x0 + kx0:x0 + kx1], want[ky0:ky1, kx0:kx1], rtol=1e-4, atol=1e-6), \
+            (rs, y0, x0)
