@@ -207,12 +207,13 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
 // wave's 64 lanes are 64 consecutive columns u (60 output pixels and two
 // columns either side); the horizontal 5-sums come from the neighbouring
 // lanes (whole-wave DPP shifts by one lane), the vertical ones from a lane's
-// own rows: a band of 8 output rows takes 12 rows of D_o.  SPLIT = 1: the
-// four waves of a workgroup take four bands (a 60 x 32 pixel tile) and each
-// walks all search offsets in the reference's order (x offset outer, y
-// offset inner).  SPLIT = 4, for images with few bands: the four waves share
-// one band (a 60 x 8 tile) and take a quarter of the offsets each, in that
-// order, and their weighted sums are added in wave order.  The patch
+// own rows: a band of 8 output rows takes 12 rows of D_o.  The four waves of
+// a workgroup share one band (a 60 x 8 pixel tile) and take a quarter of the
+// search offsets each, in the reference's order (x offset outer, y offset
+// inner), and their weighted sums are added in wave order.  (A band per
+// wave and all offsets per wave, 60 x 32 tiles, left a 1080p frame's 1,088
+// equal workgroups 2.1 rounds of the 512 that fit at once; quarter-offset
+// workgroups make 8.4 rounds: 0.64 -> 0.55 ms at r_s = 7.)  The patch
 // distance is summed horizontally, then vertically, with fused multiply-adds,
 // and the weights come from the hardware exponential, so results differ from
 // the reference's by rounding only.
@@ -232,18 +233,17 @@ __device__ __forceinline__ float lane_sum5(float v) {
     return ((a2 + a1) + v) + (b1 + b2);
 }
 
-template <int RS, int SPLIT>
+template <int RS>
 __global__ __launch_bounds__(kBoxLanes *kBoxWaves, 2) void k_nlm_box(int32_t W, int32_t H, float frac_h2,
                                                                      float sigma2, int32_t rs_arg,
                                                                      const float4 *__restrict__ src,
                                                                      float4 *__restrict__ dst) {
-    static_assert(SPLIT == 1 || SPLIT == kBoxWaves, "one band per wave or one band per workgroup");
-    constexpr int kBands = kBoxWaves / SPLIT, kOutRows = kBands * kBoxBand;
+    constexpr int SPLIT = kBoxWaves, kOutRows = kBoxBand;
     constexpr int kRsMax = RS >= 0 ? RS : kMaxHalo - 2;
     __shared__ float4 tile[(kOutRows + 4 + 2 * kRsMax) * (kBoxLanes + 2 * kRsMax)];
-    // SPLIT > 1: the weighted sums of waves 1 .. SPLIT-1
-    __shared__ float4 part_sum[SPLIT > 1 ? (SPLIT - 1) * kBoxBand * kBoxLanes : 1];
-    __shared__ float part_sw[SPLIT > 1 ? (SPLIT - 1) * kBoxBand * kBoxLanes : 1];
+    // the weighted sums of waves 1 .. SPLIT-1
+    __shared__ float4 part_sum[(SPLIT - 1) * kBoxBand * kBoxLanes];
+    __shared__ float part_sw[(SPLIT - 1) * kBoxBand * kBoxLanes];
     const int rs = RS >= 0 ? RS : rs_arg;
     const int tw = kBoxLanes + 2 * rs, th = kOutRows + 4 + 2 * rs;
     const int x0 = blockIdx.x * kBoxOutCols, y0 = blockIdx.y * kOutRows;
@@ -256,10 +256,10 @@ __global__ __launch_bounds__(kBoxLanes *kBoxWaves, 2) void k_nlm_box(int32_t W, 
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int band = wave / SPLIT, part = wave % SPLIT;
-    // this lane's column u = x0 - 2 + lane, its band's D rows y0 + 8 band - 2 + r
-    // (r = 0 .. 11): tile column lane + rs, tile rows 8 band + rs + r
-    const float4 *const base = tile + (band * kBoxBand + rs) * tw + lane + rs;
+    const int part = wave;
+    // this lane's column u = x0 - 2 + lane, the band's D rows y0 - 2 + r
+    // (r = 0 .. 11): tile column lane + rs, tile rows rs + r
+    const float4 *const base = tile + rs * tw + lane + rs;
     float4 c[kBoxBand + 4];
 #pragma unroll
     for (int r = 0; r < kBoxBand + 4; ++r) c[r] = base[r * tw];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kBoxLanes *kBoxWaves, 2) void k_nlm_box(int32_t W, 
             sum[j].w = fmaf(s[j].w, w, sum[j].w);
         }
     }
-    if constexpr (SPLIT > 1) {
+    {
         if (part > 0) {
 #pragma unroll
             for (int j = 0; j < kBoxBand; ++j) {
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kBoxLanes *kBoxWaves, 2) void k_nlm_box(int32_t W, 
     if (lane >= 2 && lane < kBoxLanes - 2 && x < W) {
 #pragma unroll
         for (int j = 0; j < kBoxBand; ++j) {
-            const int y = y0 + band * kBoxBand + j;
+            const int y = y0 + j;
             if (y < H)
                 dst[(size_t)y * W + x] =
                     make_float4(sum[j].x / sw[j], sum[j].y / sw[j], sum[j].z / sw[j], sum[j].w / sw[j]);
@@ -336,24 +336,12 @@ __global__ __launch_bounds__(kBoxLanes *kBoxWaves, 2) void k_nlm_box(int32_t W, 
     }
 }
 
-// Launches k_nlm_box with the band layout that fills the GPU: one band per
-// wave from kNlmSplitBelow bands of 60 x 8 pixels up (two waves' worth per
-// SIMD of 1,024 and more), one band per workgroup below that.
-constexpr int kNlmSplitBelow = 4096;
-
 template <int RS>
 void launch_nlm_box(hipStream_t stream, int32_t w, int32_t h, float frac_h2, float sigma2, int32_t rs,
                     const float4 *s4, float4 *d4) {
-    const int cols = (w + kBoxOutCols - 1) / kBoxOutCols;
-    const int bands = cols * ((h + kBoxBand - 1) / kBoxBand);
-    const dim3 blk(kBoxLanes * kBoxWaves);
-    if (bands >= kNlmSplitBelow) {
-        const dim3 grid(cols, (h + kBoxBand * kBoxWaves - 1) / (kBoxBand * kBoxWaves));
-        hipLaunchKernelGGL((k_nlm_box<RS, 1>), grid, blk, 0, stream, w, h, frac_h2, sigma2, rs, s4, d4);
-    } else {
-        const dim3 grid(cols, (h + kBoxBand - 1) / kBoxBand);
-        hipLaunchKernelGGL((k_nlm_box<RS, kBoxWaves>), grid, blk, 0, stream, w, h, frac_h2, sigma2, rs, s4, d4);
-    }
+    const dim3 grid((w + kBoxOutCols - 1) / kBoxOutCols, (h + kBoxBand - 1) / kBoxBand);
+    hipLaunchKernelGGL((k_nlm_box<RS>), grid, dim3(kBoxLanes * kBoxWaves), 0, stream, w, h, frac_h2, sigma2, rs, s4,
+                       d4);
 }
 
 // Checks a tonemap call's arguments; returns the pixel count through n.

@@ -52,9 +52,9 @@ def test_capture_of_exact_values(ctx):
     assert len(np.unique(out8.reshape(-1, 3), axis=0)) == 1
 
 
-# patch radius 2 runs as box sums over 60 x 32 pixel tiles (search radii 6
+# patch radius 2 runs as box sums over 60 x 8 pixel tiles (search radii 6
 # and 7 compiled in, others at run time, up to 10); other patch radii run
-# per pixel; the shapes cover one tile, partial tiles and 3 x 3 tiles
+# per pixel; the shapes cover one tile, partial tiles and many tiles
 @pytest.mark.parametrize("rs,rp,coeff", [(6, 2, 0.1), (7, 2, 0.05), (3, 1, 0.4), (6, 2, 1e-6), (10, 2, 0.2),
                                          (2, 2, 0.3)])
 @pytest.mark.parametrize("shape", [(48, 40), (5, 3), (17, 70), (70, 130)])
@@ -72,13 +72,12 @@ def test_nlm_matches_oracle(ctx, rs, rp, coeff, shape):
 
 @pytest.mark.parametrize("rs", [6, 7])
 def test_nlm_1080p_matches_oracle_on_crops(ctx, rs):
-    """At 1920x1080 the box-sum NLM gives each wave its own band of rows (the
-    small-image tests above run the layout whose waves share a band).  An
+    """NLM at 1920x1080 (4,320 bands of 60 x 8 pixels, one per workgroup).  An
     output pixel depends only on the input within rs + 2 texels, clamped at
     the image's edges, so the oracle run on a crop reproduces the pixels at
     least rs + 2 texels inside it and, where the crop reaches the image's
     edge, up to that edge: the two corners and a window across tile seams
-    (columns 60k, rows 32k) are compared."""
+    (columns 60k, rows 8k) are compared."""
     h, w = 1080, 1920
     rng = np.random.default_rng(42 + rs)
     base = np.linspace(0, 1, h * w, dtype=np.float32).reshape(h, w, 1)
