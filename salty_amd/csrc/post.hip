@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
         const int tx = i % tw, ty = i / tw;
         const int gx = min(max(x0 + tx, 0), W - 1);
         const int gy = min(max(y0 + ty, 0), H - 1);
-        tile[i] = src[gy * W + gx];
+        tile[i] = src[(size_t)gy * W + gx];
     }
     __syncthreads();
     const int lx = threadIdx.x % kTile, ly = threadIdx.x / kTile;
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(kTile *kTile) void k_nlm(int32_t W, int32_t H, floa
             sum.w += s.w * w;
         }
     }
-    dst[y * W + x] = make_float4(sum.x / sw, sum.y / sw, sum.z / sw, sum.w / sw);
+    dst[(size_t)y * W + x] = make_float4(sum.x / sw, sum.y / sw, sum.z / sw, sum.w / sw);
 }
 
 // Non-local means for patch radius 2 (the reference's) by box sums.  For one
